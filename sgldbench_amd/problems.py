@@ -1,0 +1,102 @@
+"""Synthetic voxel problems of the benchmark workloads, in the reference's
+conventions.
+
+Voxel arrays are (ny, nx, nz) with row 0 at the top (y = ny), as
+FEA_CreateVoxelizedModel.m:6 / FEA_VoxelBasedDiscretization.m:26 produce them.
+Lattice node id g = row + (ny+1) col + (ny+1)(nx+1) layer; compressed nodes are
+the solid lattice nodes in ascending g (FEA_VoxelBasedDiscretization.m:81-85);
+DOFs are interleaved 3 per node (FEA_ApplyBoundaryCondition.m:21,27-29).
+"""
+import numpy as np
+
+
+def solid_nodes(voxels):
+    """Ascending 0-based lattice ids of the nodes of solid voxels."""
+    vox = np.asarray(voxels) != 0
+    ny, nx, nz = vox.shape
+    node = np.zeros((ny + 1, nx + 1, nz + 1), dtype=bool)
+    for dr in (0, 1):
+        for dc in (0, 1):
+            for dl in (0, 1):
+                node[dr:dr + ny, dc:dc + nx, dl:dl + nz] |= vox
+    return np.flatnonzero(node.ravel(order="F"))
+
+
+def node_coords(voxels, nodes):
+    """(x, y, z) of lattice node ids (x = col, y = ny - row, z = layer)."""
+    ny, nx, nz = np.asarray(voxels).shape
+    g = np.asarray(nodes, dtype=np.int64)
+    row = g % (ny + 1)
+    col = (g // (ny + 1)) % (nx + 1)
+    lay = g // ((ny + 1) * (nx + 1))
+    return np.stack([col, ny - row, lay], axis=1)
+
+
+def cantilever3(voxels, nelx=None):
+    """'Cuboid - Cantilever 3' (FEA_BuiltInBoundaryConditions4CuboidDesignDomain.m:34-44)
+    with FEA_ApplyBoundaryCondition.m:9-41: nodes at x = min x fixed in x/y/z;
+    total force -1 in z spread over the nodes at x = nelx, z = min z; loaded
+    nodes are dropped from the fixations.  Returns (F, fixed uint8)."""
+    nodes = solid_nodes(voxels)
+    xyz = node_coords(voxels, nodes)
+    nx = np.asarray(voxels).shape[1] if nelx is None else nelx
+    fixed_n = xyz[:, 0] == xyz[:, 0].min()
+    loaded = (xyz[:, 0] == nx) & (xyz[:, 2] == xyz[:, 2].min())
+    fixed_n &= ~loaded
+    n = len(nodes)
+    F = np.zeros(3 * n)
+    F[3 * np.flatnonzero(loaded) + 2] = -1.0 / loaded.sum()
+    fixed = np.zeros(3 * n, dtype=np.uint8)
+    for a in range(3):
+        fixed[3 * np.flatnonzero(fixed_n) + a] = 1
+    return F, fixed
+
+
+def simp_modulus(x, p=3.0, e0=1.0, emin=1e-9):
+    """TopOpti_MaterialInterpolationSIMP.m:5: E = Emin + x^p (E0 - Emin)."""
+    return emin + np.asarray(x, dtype=np.float64) ** p * (e0 - emin)
+
+
+def gaussian_random_density(shape, seed, sigma=6.0, lo=0.001, hi=1.0):
+    """Seeded white noise, Gaussian-smoothed (reflect, truncate 4), min-max
+    scaled to [lo, hi]."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    g = gaussian_filter(w, sigma=sigma, mode="reflect", truncate=4.0)
+    g = (g - g.min()) / (g.max() - g.min())
+    return lo + g * (hi - lo)
+
+
+def cuboid(nx, ny, nz):
+    return np.ones((ny, nx, nz), dtype=np.uint8)
+
+
+def cfg0():
+    """64x32x32 solid cuboid, E = 1, Cantilever 3, 4 levels, omega 0.6, tol 1e-8."""
+    vox = cuboid(64, 32, 32)
+    F, fixed = cantilever3(vox)
+    E = simp_modulus(np.ones(int(vox.sum())))
+    return dict(name="cuboid64x32x32", voxels=vox, E=E, F=F, fixed=fixed, num_levels=4,
+                omega=0.6, tol=1e-8, max_it=500)
+
+
+CFG1_SEED = 20241
+CFG3_SEED = 20243
+
+
+def random_density_cuboid(nx, ny, nz, seed):
+    """Density in voxel order (ny, nx, nz) column-major, as eleMapBack lists
+    the elements of a full cuboid."""
+    x = gaussian_random_density((ny, nx, nz), seed)
+    return x.ravel(order="F")
+
+
+def cfg1(nx=256, ny=128, nz=128, levels=6, seed=CFG1_SEED):
+    """256x128x128 cuboid, SIMP E from a seeded smoothed random field,
+    Cantilever 3, 6 levels."""
+    vox = cuboid(nx, ny, nz)
+    F, fixed = cantilever3(vox)
+    E = simp_modulus(random_density_cuboid(nx, ny, nz, seed))
+    return dict(name="cuboid%dx%dx%d-grf" % (nx, ny, nz), voxels=vox, E=E, F=F, fixed=fixed,
+                num_levels=levels, omega=0.6, tol=1e-8, max_it=500)

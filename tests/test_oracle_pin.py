@@ -1,0 +1,286 @@
+"""Pins the CPU oracle against independent mathematics (CPU only).
+
+Each check is chosen so that a plausible mistake in the restatement (a wrong
+node order or y orientation, a transposed operand, a dropped term, a wrong
+multiplicity weight) fails it: exact-integration Ke, physics invariants,
+scipy-assembled global K, dense trilinear P with R = P^T, Galerkin P^T K P,
+the closed-form additive V-cycle, scipy spsolve and the energy identity.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as orc
+from sgldbench_amd import problems
+from tests import brute
+
+CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                    [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+
+def exact_ke(nu, h):
+    """Exact integral of B^T D B for the trilinear hex, by 1D integrals."""
+    M = h / 6.0 * np.array([[2.0, 1.0], [1.0, 2.0]])  # int phi_i phi_j
+    K1 = 1.0 / h * np.array([[1.0, -1.0], [-1.0, 1.0]])  # int phi_i' phi_j'
+    G = 0.5 * np.array([[-1.0, -1.0], [1.0, 1.0]])  # int phi_i' phi_j
+    lam = nu / ((1 + nu) * (1 - 2 * nu))
+    mu = 1.0 / (2 * (1 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] = lam + 2 * mu
+    D[3, 3] = D[4, 4] = D[5, 5] = mu
+    c = np.zeros((6, 3, 3))  # strain s <- comp a, derivative p
+    c[0, 0, 0] = c[1, 1, 1] = c[2, 2, 2] = 1
+    c[3, 1, 2] = c[3, 2, 1] = 1
+    c[4, 0, 2] = c[4, 2, 0] = 1
+    c[5, 0, 1] = c[5, 1, 0] = 1
+
+    def I(p, q, i, j):
+        v = 1.0
+        for d in range(3):
+            a, b = CORNERS[i, d], CORNERS[j, d]
+            if d == p and d == q:
+                v *= K1[a, b]
+            elif d == p:
+                v *= G[a, b]
+            elif d == q:
+                v *= G[b, a]
+            else:
+                v *= M[a, b]
+        return v
+
+    Ke = np.zeros((24, 24))
+    for i in range(8):
+        for j in range(8):
+            Ipq = np.array([[I(p, q, i, j) for q in range(3)] for p in range(3)])
+            for a in range(3):
+                for b in range(3):
+                    Ke[3 * i + a, 3 * j + b] = np.einsum("st,sp,tq,pq->", D, c[:, a, :], c[:, b, :], Ipq)
+    return Ke
+
+
+@pytest.mark.parametrize("nu,h", [(0.3, 1.0), (0.25, 2.0)])
+def test_ke_matches_exact_integration(nu, h):
+    Ke = orc.element_stiffness(nu, h)
+    ref = exact_ke(nu, h)
+    assert np.abs(Ke - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_ke_physics():
+    nu, h = 0.3, 1.0
+    Ke = orc.element_stiffness(nu, h)
+    assert np.allclose(Ke, Ke.T, atol=1e-15)
+    assert np.linalg.matrix_rank(Ke, tol=1e-10) == 18
+    X = CORNERS * h
+    modes = []
+    for a in range(3):  # translations
+        u = np.zeros(24); u[a::3] = 1; modes.append(u)
+    for (a, b) in ((0, 1), (1, 2), (2, 0)):  # rotations
+        u = np.zeros(24); u[a::3] = -X[:, b]; u[b::3] = X[:, a]; modes.append(u)
+    for u in modes:
+        assert np.abs(Ke @ u).max() < 1e-13
+    lam = nu / ((1 + nu) * (1 - 2 * nu)); mu = 1 / (2 * (1 + nu))
+    eps = 1e-2
+    u = np.zeros(24); u[0::3] = eps * X[:, 0]  # uniform e_xx
+    assert u @ Ke @ u == pytest.approx((lam + 2 * mu) * eps ** 2 * h ** 3, rel=1e-13)
+    u = np.zeros(24); u[0::3] = eps * X[:, 1]  # simple shear gamma_xy = eps
+    assert u @ Ke @ u == pytest.approx(mu * eps ** 2 * h ** 3, rel=1e-13)
+    Ke2 = orc.element_stiffness(nu, 2.0)
+    assert np.allclose(Ke2, 2.0 * Ke, rtol=1e-13, atol=1e-15)
+
+
+def random_voxels(shape, fill, seed):
+    rng = np.random.default_rng(seed)
+    v = (rng.random(shape) < fill).astype(np.uint8)
+    v.ravel()[0] = 1
+    return v
+
+
+def test_kbyu_mex_vs_scipy_global_K():
+    vox = random_voxels((6, 8, 4), 0.7, 1)
+    m = orc.Mesh(vox, 2)
+    info = m.info[0]
+    eN = m.get(1, "eNodMat")
+    rng = np.random.default_rng(2)
+    E = rng.random(info["numElements"]) + 0.1
+    Ke = m.get(1, "Ke")
+    K = brute.global_K(Ke, E, eN, info["numNodes"])
+    u = rng.standard_normal(info["numDOFs"])
+    y = orc.kbyu_mex(u, eN, Ke, E, m.colors(1))
+    ref = K @ u
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_colours_are_independent_sets():
+    vox = random_voxels((8, 6, 4), 0.8, 3)
+    m = orc.Mesh(vox, 2)
+    eN = m.get(1, "eNodMat")
+    cols = m.colors(1)
+    assert sorted(np.concatenate(cols).tolist()) == list(range(1, m.info[0]["numElements"] + 1))
+    for c in cols:
+        nodes = eN[c - 1].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+
+
+def test_mesh_geometry_patch_test():
+    """Linear displacement in true node coordinates (y reversed) -> zero
+    forces at interior nodes; pins eNodMat ordering against Ke."""
+    vox = problems.cuboid(4, 4, 2)
+    m = orc.Mesh(vox, 2)
+    nodes = m.get(1, "nodMapBack") - 1
+    xyz = problems.node_coords(vox, nodes).astype(float)
+    A = np.array([[0.3, -0.2, 0.5], [0.1, 0.7, -0.4], [0.2, 0.6, -0.1]])
+    u = (xyz @ A.T).ravel()
+    y = orc.kbyu_mex(u, m.get(1, "eNodMat"), m.get(1, "Ke"), np.ones(m.info[0]["numElements"]), m.colors(1))
+    interior = np.all((xyz > 0) & (xyz < np.array([4, 4, 2])), axis=1)
+    assert interior.sum() > 0
+    f = y.reshape(-1, 3)
+    assert np.abs(f[interior]).max() < 1e-13
+    assert np.abs(f[~interior]).max() > 1e-3
+
+
+def _hierarchy_P(m, vox_dims_per_level, ii):
+    F, Cc = m.info[ii - 2], m.info[ii - 1]
+    Pn = brute.dense_prolongation(m.get(ii - 1, "nodMapBack"), m.get(ii, "nodMapBack"),
+                                  (F["resX"], F["resY"], F["resZ"]),
+                                  (Cc["resX"], Cc["resY"], Cc["resZ"]), Cc["span"])
+    return Pn
+
+
+@pytest.mark.parametrize("non_dyadic", [0, 1])
+def test_transfers_equal_dense_trilinear(non_dyadic):
+    vox = random_voxels((16, 16, 16), 0.55, 5)
+    m = orc.Mesh(vox, 4 if non_dyadic else 3, non_dyadic)
+    rng = np.random.default_rng(6)
+    for ii in range(2, m.num_levels + 1):
+        Pn = _hierarchy_P(m, None, ii)
+        P = brute.dof_prolongation(Pn)
+        xc = rng.standard_normal(P.shape[1])
+        rf = rng.standard_normal(P.shape[0])
+        assert np.abs(m.interpolate(ii, xc) - P @ xc).max() < 1e-13
+        assert np.abs(m.restrict(ii, rf) - P.T @ rf).max() < 1e-12
+        # every fine node is fully interpolated (rows of P sum to 1)
+        assert np.allclose(Pn.sum(axis=1), 1.0)
+
+
+def test_transfer_tables_match_reference_values():
+    g = np.load("tests/golden/transfer_tables.npz")
+    for s in (2, 4):
+        assert np.array_equal(orc.transfer_table(s), g["RI%d" % s])
+        assert np.array_equal(orc.sub_ele_nod_mat(s), g["sub%d" % s])
+
+
+@pytest.mark.parametrize("non_dyadic", [0, 1])
+def test_galerkin_and_diag_equal_PtKP(non_dyadic):
+    vox = random_voxels((16, 16, 8), 0.6, 7)
+    m = orc.Mesh(vox, 4 if non_dyadic else 3, non_dyadic)
+    rng = np.random.default_rng(8)
+    E = 10.0 ** rng.uniform(-3, 0, m.info[0]["numElements"])
+    m.set_modulus(E)
+    fixed = np.zeros(m.info[0]["numDOFs"], dtype=np.uint8)
+    fixed[:9] = 1
+    m.set_fixed(fixed)
+    m.assemble(check=False)  # a random voxel set may hold floating parts
+    K = brute.global_K(m.get(1, "Ke"), E, m.get(1, "eNodMat"), m.info[0]["numNodes"]).toarray()
+    assert np.allclose(m.get(1, "diagK"), np.diag(K), rtol=1e-14, atol=0)
+    Kk = K
+    for ii in range(2, m.num_levels + 1):
+        P = brute.dof_prolongation(_hierarchy_P(m, None, ii))
+        Kk = P.T @ Kk @ P
+        Ks = m.get(ii, "Ks")
+        info = m.info[ii - 1]
+        eN = m.get(ii, "eNodMat")
+        Kasm = np.zeros_like(Kk)
+        for e in range(info["numElements"]):
+            d = (3 * (eN[e][:, None] - 1) + np.arange(3)[None, :]).ravel()
+            Kasm[np.ix_(d, d)] += Ks[:, :, e]
+        assert np.abs(Kasm - Kk).max() <= 1e-12 * np.abs(Kk).max()
+        if ii < m.num_levels:
+            assert np.allclose(m.get(ii, "diagK"), np.diag(Kk), rtol=1e-12)
+
+
+def test_coarse_fixed_mask_and_vcycle_closed_form():
+    vox = random_voxels((8, 16, 8), 0.75, 9)
+    vox[:, 0, :] = 1
+    m = orc.Mesh(vox, 3)
+    n1 = m.info[0]["numDOFs"]
+    F, fixed = problems.cantilever3(vox)
+    rng = np.random.default_rng(10)
+    E = rng.uniform(0.05, 1.0, m.info[0]["numElements"])
+    m.set_modulus(E)
+    m.set_fixed(fixed)
+    m.assemble()
+    omega = 0.6
+    K1 = brute.global_K(m.get(1, "Ke"), E, m.get(1, "eNodMat"), m.info[0]["numNodes"]).toarray()
+    Ps = [brute.dof_prolongation(_hierarchy_P(m, None, ii)) for ii in range(2, m.num_levels + 1)]
+    # fixed masks: restricted indicator != 0
+    ind = fixed.astype(float)
+    for ii, P in zip(range(2, m.num_levels + 1), Ps):
+        ind = P.T @ ind
+        assert np.array_equal(m.get(ii, "fixedDOFs"), (ind != 0).astype(np.uint8))
+    # M^-1 r = mask(sum_k P_1k 2w D_k^-1 R_k1 r + P_1L K_L^-1 R_L1 r)
+    Kl, Pc = [K1], [np.eye(n1)]
+    for P in Ps:
+        Kl.append(P.T @ Kl[-1] @ P)
+        Pc.append(Pc[-1] @ P)
+    r = rng.standard_normal(n1)
+    r[fixed == 1] = 0
+    z = np.zeros(n1)
+    for k in range(m.num_levels - 1):
+        z += Pc[k] @ (2 * omega * (Pc[k].T @ r) / np.diag(Kl[k]))
+    fL = m.get(m.num_levels, "fixedDOFs") == 0
+    rL = Pc[-1].T @ r
+    xL = np.zeros_like(rL)
+    xL[fL] = np.linalg.solve(Kl[-1][np.ix_(fL, fL)], rL[fL])
+    z += Pc[-1] @ xL
+    z[fixed == 1] = 0
+    zo = m.vcycle(r, omega)
+    assert np.abs(zo - z).max() <= 1e-11 * np.abs(z).max()
+    # the preconditioner is symmetric
+    s = rng.standard_normal(n1); s[fixed == 1] = 0
+    assert m.vcycle(s, omega) @ r == pytest.approx(zo @ s, rel=1e-11)
+
+
+@pytest.mark.parametrize("dims,levels", [((16, 8, 8), 3), ((32, 16, 16), 4)])
+def test_pcg_converges_to_spsolve(dims, levels):
+    vox = problems.cuboid(*dims)
+    m = orc.Mesh(vox, levels)
+    F, fixed = problems.cantilever3(vox)
+    E = problems.simp_modulus(problems.random_density_cuboid(*dims, seed=11))
+    m.set_modulus(E)
+    m.set_fixed(fixed)
+    m.assemble()
+    u, its, hist, rc = m.pcg(F, 0.6, 1e-10, 500)
+    assert rc == 0 and its == len(hist) and hist[-1] < 1e-10 and np.all(hist[:-1] >= 1e-10)
+    K = brute.global_K(m.get(1, "Ke"), E, m.get(1, "eNodMat"), m.info[0]["numNodes"]).tocsc()
+    free = fixed == 0
+    ref = np.zeros_like(F)
+    ref[free] = spla.spsolve(K[free][:, free], F[free])
+    assert np.linalg.norm(u - ref) <= 1e-7 * np.linalg.norm(ref)
+    assert np.all(u[~free] == 0)
+    # compliance: f^T u = sum_e E_e ce_e = u^T K u
+    ce = m.unit_compliance(u)
+    c = E @ ce
+    assert c == pytest.approx(F @ u, rel=1e-8)
+    assert c == pytest.approx(u @ (K @ u), rel=1e-9)  # cancellation in u_e^T Ke u_e
+
+
+def test_mex_level_functions_agree_with_mesh_methods():
+    vox = random_voxels((8, 8, 8), 0.7, 12)
+    m = orc.Mesh(vox, 3)
+    rng = np.random.default_rng(13)
+    for ii in (2, 3):
+        F, Cc = m.info[ii - 2], m.info[ii - 1]
+        RI = m.get(ii, "RI")
+        args = (m.get(ii, "transferMat"), None, m.get(ii, "eNodMat"))
+        rf = rng.standard_normal((F["numNodes"], 3))
+        y = orc.restrict_mex(rf, args[0], np.ascontiguousarray(RI.T), args[2],
+                             Cc["numNodes"], Cc["intermediateNumNodes"],
+                             m.get(ii, "solidNodeMapCoarser2Finer"), m.get(ii, "transferMatCoeffi"),
+                             m.colors(ii))
+        assert np.array_equal(y, m.restrict(ii, rf.ravel()).reshape(-1, 3))
+        xc = rng.standard_normal((Cc["numNodes"], 3))
+        x = orc.interpolate_mex(xc, args[0], RI, args[2], F["numNodes"], Cc["intermediateNumNodes"],
+                                m.get(ii, "solidNodeMapCoarser2Finer"), m.get(ii, "transferMatCoeffi"),
+                                m.colors(ii))
+        assert np.array_equal(x, m.interpolate(ii, xc.ravel()).reshape(-1, 3))
