@@ -1,0 +1,146 @@
+"""Benchmark: MG-PCG time to rel-res 1e-8 and K.u GDOF/s on one MI355X.
+
+Workload: 256x128x128 voxel cuboid (12,830,211 DOF), SIMP modulus from a
+seeded Gaussian random field, cantilever boundary conditions, 6 multigrid
+levels (span 2), weighted Jacobi omega 0.6, tolerance 1e-8.  One step = one
+full solve from U = 0 to FP64 true relative residual 1e-8, with F and U
+resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", default="256x128x128")
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kbyu-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
+    """Oracle (C/OpenMP restatement of the reference algorithm) on the host:
+    `sample_its` FP64 PCG iterations of the same workload, extrapolated to the
+    GPU FP64 iteration count."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    m = orc.Mesh(p["voxels"], p["num_levels"])
+    m.set_fixed(p["fixed"])
+    m.set_modulus(p["E"])
+    m.assemble()
+    t_setup = time.time() - t0
+    t_it, t_k, t_v = m.time_pcg_iterations(p["F"], p["omega"], sample_its)
+    per_it = t_it / sample_its
+    n = p["F"].size
+    t_solve = per_it * its_fp64
+    return {"value": n / t_solve / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
+            "sample": "%d FP64 PCG iterations of the same workload on the oracle "
+                      "(%.2f s/iteration: K.u %.2f s, V-cycle %.2f s; setup %.1f s), "
+                      "extrapolated to %d iterations" % (sample_its, per_it, t_k / sample_its,
+                                                         t_v / sample_its, t_setup, its_fp64),
+            "solve_s_est": t_solve}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
+              "multi-GPU sharding is described in DESIGN.md, not run." % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from sgldbench_amd import problems, solver
+
+    nx, ny, nz = (int(v) for v in a.size.split("x"))
+    p = problems.cfg1(nx, ny, nz, a.levels)
+    n = p["F"].size
+    ne = int(p["E"].size)
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s.set_fixed(p["fixed"])
+    s.set_modulus(p["E"])
+    t0 = time.time()
+    s.assemble(p["omega"])
+    setup_s = time.time() - t0
+    dF = torch.from_numpy(p["F"]).to(dev)
+    dU = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    def solve(prec):
+        dU.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], 2000, prec)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, its, rr, s.stats()
+
+    res = {}
+    for prec, name in ((solver.PREC_FP64, "fp64"), (solver.PREC_MIXED, "mixed")):
+        for _ in range(a.warmup):
+            solve(prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        runs = [solve(prec) for _ in range(a.steps)]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        res[name] = dict(ms_per_step=1e3 * wall / a.steps, its=runs[-1][1], relres=max(r[2] for r in runs),
+                         outer=runs[-1][3]["outer"], converged=all(r[2] <= p["tol"] for r in runs))
+    U = dU.cpu().numpy()
+    comp_mixed = float(p["F"] @ U)
+    solve(solver.PREC_FP64)
+    U64 = dU.cpu().numpy()
+    comp_fp64 = float(p["F"] @ U64)
+    # K.u roofline (FP64 and FP32), hipEvent-timed on the solver stream
+    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
+    ms32 = s.bench_kbyu(solver.PREC_MIXED, a.kbyu_reps)
+    bytes64 = 2 * n * 8 + ne * 8
+    bytes32 = 2 * n * 4 + ne * 4
+    best = "mixed" if res["mixed"]["converged"] else "fp64"
+    t_best = res[best]["ms_per_step"] / 1e3
+    out = {
+        "metric": "MG-PCG time to rel-res 1e-8 & K.u GDOF/s, FP64 vs mixed/batched, 1 MI355X",
+        "value": n / t_best / 1e9,
+        "unit": "GDOF/s (DOF solved to rel-res 1e-8 per second)",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res[best]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if best == "mixed" else "f64",
+        "data": "synthetic (seeded Gaussian random density field)",
+        "config": {"workload": "cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, "
+                               "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]),
+                   "path": best, "dofs": n, "elements": ne},
+        "solve": {"fp64": res["fp64"], "mixed": res["mixed"],
+                  "speedup_mixed_vs_fp64": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
+                  "compliance_fp64": comp_fp64, "compliance_mixed": comp_mixed,
+                  "compliance_rel_diff": abs(comp_mixed - comp_fp64) / abs(comp_fp64),
+                  "u_rel_diff": float(np.linalg.norm(U - U64) / np.linalg.norm(U64)),
+                  "setup_s": setup_s},
+        "kbyu": {"fp64_ms": ms64, "fp32_ms": ms32, "fp64_gdofs": n / ms64 / 1e6,
+                 "fp32_gdofs": n / ms32 / 1e6, "fp64_GBs": bytes64 / ms64 / 1e6,
+                 "fp32_GBs": bytes32 / ms32 / 1e6},
+        "roofline": {"bound": "hbm", "achieved": bytes64 / ms64 / 1e6, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_kbyu<double>", "bytes_per_launch": bytes64},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=a.cpu_its)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,155 @@
+/*
+ * sgldfea.h -- C ABI of libsgldfea, the MI355X (gfx950) HIP implementation of
+ * SGLDBench's voxel-FEA solve: matrix-free K.u, additive geometric-multigrid
+ * V-cycle, PCG driver and unit compliance.
+ *
+ * Conventions (those of the reference's MEX interface):
+ *   - int32 ids are 1-based; matrices are column-major;
+ *   - DOF vectors are FP64, interleaved [ux uy uz] per compressed node;
+ *   - n x 3 node arrays (transfers) are column-major (component-major);
+ *   - voxel arrays are resY x resX x resZ column-major, row 0 at the top.
+ * Every entry point returns an int status (SGLD_OK = 0); on failure
+ * sgld_last_error() returns a thread-local message that starts with the
+ * reference's error identifier where one exists.
+ *
+ * The library is thread-compatible, not thread-safe: one context must not be
+ * used from two threads at once.  Stateless entry points take host pointers,
+ * allocate device memory per call and copy results back, as the MEX calls
+ * they replace hand arrays back to MATLAB.
+ */
+#ifndef SGLDFEA_H
+#define SGLDFEA_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SGLD_OK = 0,
+    SGLD_ERR_ARG = 1,       /* invalid argument (shape, range, null) */
+    SGLD_ERR_HIP = 2,       /* HIP runtime failure */
+    SGLD_ERR_NOT_SPD = 3,   /* coarsest matrix not positive definite */
+    SGLD_ERR_MAXIT = 4,     /* PCG hit maxIT without reaching tol */
+    SGLD_ERR_STATE = 5,     /* call out of order (e.g. solve before assemble) */
+    SGLD_ERR_NOMEM = 6      /* device allocation failed */
+};
+
+enum { SGLD_PREC_FP64 = 0, SGLD_PREC_MIXED = 1 };
+
+const char *sgld_last_error(void);
+/* Library version string and the gfx target it was built for. */
+const char *sgld_version(void);
+
+/* ===================== stateless MEX replacements ===================== */
+
+/* Replaces Solving_KbyU_MatrixFree_mex (src/MEXfuncs/Solving_KbyU_MatrixFree_mex.cpp:6-103),
+ * called at src/Solving_KbyU_MatrixFree.m:9.  Y = sum_e E_e scatter(Ke gather_e(U)),
+ * accumulated colour by colour (colors: concatenated 1-based element ids of
+ * numColors colours, colorOffsets[numColors+1] prefix offsets). */
+int sgld_kbyu(const double *U, int64_t numDOFs, const int32_t *eNodMat, int64_t numElements,
+              const double *Ke, int64_t keRows, int64_t keCols, const double *E, int64_t numE,
+              const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
+
+/* Replaces Solving_Restriction_MatrixFree_mex
+ * (src/MEXfuncs/Solving_Restriction_MatrixFree_mex.cpp:6-191), called at
+ * src/Solving_RestrictResidual.m:6-8.  rF: nF x 3, RIT: 8 x m (m = 27 or 125),
+ * transferMat: m x numElesC, Y: nC x 3. */
+int sgld_restrict(const double *rF, int64_t nF, const int32_t *transferMat, int64_t m,
+                  const double *RIT, int64_t ritRows, int64_t ritCols,
+                  const int32_t *eNodMatC, int64_t numElesC, int64_t nC, int64_t nGhost,
+                  const int32_t *solidNodeMapCoarser2Finer, const double *transferMatCoeffi,
+                  const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
+
+/* Replaces Solving_Interpolation_MatrixFree_mex
+ * (src/MEXfuncs/Solving_Interpolation_MatrixFree_mex.cpp:6-190), called at
+ * src/Solving_InterpolationDeviation.m:7-9.  xC: nC x 3, RI: m x 8, Y: nF x 3. */
+int sgld_interpolate(const double *xC, int64_t nC, const int32_t *transferMat, int64_t m,
+                     const double *RI, int64_t riRows, int64_t riCols,
+                     const int32_t *eNodMatC, int64_t numElesC, int64_t nF, int64_t nGhost,
+                     const int32_t *solidNodeMapCoarser2Finer, const double *transferMatCoeffi,
+                     const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
+
+/* Replaces Solving_AssembleCmptStencilFromFinestLevel
+ * (src/MEXfuncs/Solving_AssembleCmptStencilFromFinestLevel.cpp:12-169), called at
+ * src/Solving_AssembleFEAstencil.m:33.  The trilinear interpolatingKe / localMapping
+ * pair of the MEX is implied by span (2: 8 children, 4: 64).  Ks: 24 x 24 x nC. */
+int sgld_assemble_finest(const double *Ke, int64_t keRows, int64_t keCols, const double *E,
+                         int64_t numE, const int32_t *elementUpwardMap, int64_t nC,
+                         int64_t numChildren, double *Ks);
+
+/* Replaces Solving_AssembleCmptStencilFromNonFinestLevel
+ * (src/MEXfuncs/Solving_AssembleCmptStencilFromNonFinestLevel.cpp:12-143), called at
+ * src/Solving_AssembleFEAstencil.m:56.  KsPrev: 24 x 24 x numPrev. */
+int sgld_assemble_nonfinest(const double *KsPrev, int64_t numPrev,
+                            const int32_t *elementUpwardMap, int64_t nC, int64_t numChildren,
+                            double *Ks);
+
+/* Replaces TopOpti_CmptUnitCompliance_mex (src/MEXfuncs/TopOpti_CmptUnitCompliance_mex.cpp:6-59),
+ * called at src/TopOpti_ComputeUnitCompliance.m:7.  ce_e = Ue^T Ke Ue. */
+int sgld_unit_compliance(const double *U, int64_t numDOFs, const int32_t *eNodMat,
+                         int64_t numElements, const double *Ke, int64_t keRows, int64_t keCols,
+                         double *ce);
+
+/* Replaces FEA_VoxelBasedElementStiffnessMatrix (src/FEA_VoxelBasedElementStiffnessMatrix.m:1-24). */
+int sgld_element_stiffness(double poissonRatio, double cellSize, double *Ke);
+
+/* ================ stateful solver (device-resident hierarchy) ================ */
+
+typedef struct sgld_ctx sgld_ctx;
+
+/* Builds meshHierarchy_ on the device: FEA_VoxelBasedDiscretization.m:56-89,
+ * Solving_BuildingMeshHierarchy.m:1-165, FEA_SetupVoxelBased.m:6-10.
+ * voxels: resY x resX x resZ uint8 col-major (nonzero = solid) or NULL for a full
+ * cuboid.  numLevels and nonDyadic have the meaning of numLevels_ / nonDyadic_
+ * (Solving_BuildingMeshHierarchy.m:12-23). */
+int sgld_ctx_create(sgld_ctx **ctx, int resX, int resY, int resZ, const uint8_t *voxels,
+                    int numLevels, int nonDyadic, double poissonRatio, double cellSize);
+int sgld_ctx_destroy(sgld_ctx *ctx);
+/* out[8]: resX resY resZ spanWidth numElements numNodes numDOFs latticeNodes
+ * of level (1-based); returns the level count when level == 0 (out unused). */
+int sgld_ctx_info(const sgld_ctx *ctx, int level, int64_t *out);
+
+/* Fixed-DOF mask of level 1 (FEA_ApplyBoundaryCondition.m:37-41); coarse masks
+ * follow FEA_SetupVoxelBased.m:13-26. */
+int sgld_ctx_set_fixed(sgld_ctx *ctx, const uint8_t *fixedDOFs);
+/* meshHierarchy_(1).eleModulus (numElements, compressed order). */
+int sgld_ctx_set_modulus(sgld_ctx *ctx, const double *E);
+/* Solving_AssembleFEAstencil.m:1-137 plus the Jacobi weight omega
+ * (weightFactorJacobi_): Galerkin coarse operators, diagonals, coarsest factor. */
+int sgld_ctx_assemble(sgld_ctx *ctx, double omega);
+
+/* productMV = Solving_KbyU_MatrixFree(u) (src/Solving_KbyU_MatrixFree.m:1-41). */
+int sgld_ctx_kbyu(sgld_ctx *ctx, const double *u, double *y);
+/* rTilde = Solving_Vcycle(r) (src/Solving_Vcycle.m:1-52). */
+int sgld_ctx_vcycle(sgld_ctx *ctx, const double *r, double *z);
+/* [U, its] = Solving_PreconditionedConjugateGradientSolver(@KbyU, @Vcycle, F, tol,
+ * maxIT, printP, U) (src/Solving_PreconditionedConjugateGradientSolver.m:1-37),
+ * as Solving_CG_GMGS.m:7 calls it.  U: in x0, out solution.  precision:
+ * SGLD_PREC_FP64 (the reference algorithm) or SGLD_PREC_MIXED (FP32 inner PCG,
+ * V-cycle and K.u with FP64 iterative refinement on the true residual).
+ * relres receives the final FP64 true relative residual ||F - K U|| / ||F||.
+ * Returns SGLD_ERR_MAXIT (with U filled) when maxIT is reached. */
+int sgld_ctx_pcg(sgld_ctx *ctx, const double *F, double *U, double tol, int maxIT,
+                 int precision, int *its, double *relres);
+/* Same with F and U in device memory (compressed layout). */
+int sgld_ctx_pcg_device(sgld_ctx *ctx, const double *dF, double *dU, double tol, int maxIT,
+                        int precision, int *its, double *relres);
+/* ceList = TopOpti_ComputeUnitCompliance() (src/TopOpti_ComputeUnitCompliance.m:1-21). */
+int sgld_ctx_unit_compliance(sgld_ctx *ctx, const double *U, double *ce);
+/* K.u with u and y in device memory (compressed layout); times K.u alone. */
+int sgld_ctx_kbyu_device(sgld_ctx *ctx, const double *du, double *dy, int precision);
+
+/* Times `reps` back-to-back K.u launches on the context's internal vectors
+ * (hipEvents on the context stream); msPerLaunch = average kernel time. */
+int sgld_ctx_bench_kbyu(sgld_ctx *ctx, int precision, int reps, double *msPerLaunch);
+
+/* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
+ * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
+ * last assemble, [5] K.u launches, [6] average K.u kernel milliseconds
+ * (hipEvent, FP64), [7] average K.u kernel milliseconds (FP32). */
+int sgld_ctx_stats(const sgld_ctx *ctx, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+"""ctypes binding of libsgldfea.so (the HIP C ABI declared in include/sgldfea.h).
+
+The library must have been built (sgldbench_amd/build.py or
+__graft_entry__.build()); there is no fallback path: a missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsgldfea.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i64 = C.c_int64
+_vp = C.c_void_p
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "sgld_last_error": [],
+    "sgld_version": [],
+    "sgld_kbyu": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _dp, _i64, _ip, _lp, C.c_int, _dp],
+    "sgld_restrict": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,
+                      _ip, _lp, C.c_int, _dp],
+    "sgld_interpolate": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,
+                         _ip, _lp, C.c_int, _dp],
+    "sgld_assemble_finest": [_dp, _i64, _i64, _dp, _i64, _ip, _i64, _i64, _dp],
+    "sgld_assemble_nonfinest": [_dp, _i64, _ip, _i64, _i64, _dp],
+    "sgld_unit_compliance": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _dp],
+    "sgld_element_stiffness": [C.c_double, C.c_double, _dp],
+    "sgld_ctx_create": [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+                        C.c_double, C.c_double],
+    "sgld_ctx_destroy": [_vp],
+    "sgld_ctx_info": [_vp, C.c_int, _lp],
+    "sgld_ctx_set_fixed": [_vp, _bp],
+    "sgld_ctx_set_modulus": [_vp, _dp],
+    "sgld_ctx_assemble": [_vp, C.c_double],
+    "sgld_ctx_kbyu": [_vp, _dp, _dp],
+    "sgld_ctx_vcycle": [_vp, _dp, _dp],
+    "sgld_ctx_pcg": [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
+                     C.POINTER(C.c_double)],
+    "sgld_ctx_pcg_device": [_vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
+                            C.POINTER(C.c_double)],
+    "sgld_ctx_unit_compliance": [_vp, _dp, _dp],
+    "sgld_ctx_kbyu_device": [_vp, _vp, _vp, C.c_int],
+    "sgld_ctx_bench_kbyu": [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "sgld_ctx_stats": [_vp, _dp],
+}
+_RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsgldfea.so is not built (run __graft_entry__.build()); "
+                               "there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+class SGLDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+        self.identifier = msg.split(":")[0] + ":" + msg.split(":")[1] if msg.count(":") >= 2 else msg
+
+
+ERR_MAXIT = 4
+
+
+def check(rc, allow=()):
+    if rc != 0 and rc not in allow:
+        raise SGLDError(rc, lib().sgld_last_error().decode())
+    return rc

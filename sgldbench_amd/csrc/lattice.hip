@@ -1,0 +1,1301 @@
+// Device-resident voxel FEA solver on the node lattice: K.u, additive
+// multigrid V-cycle, PCG (FP64, and FP32-inner with FP64 refinement), unit
+// compliance and the Galerkin setup.  See lattice_kernels.hpp for the layout.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "lattice_kernels.hpp"
+
+namespace sgld {
+namespace {
+
+__constant__ double cKe64[576];
+__constant__ float cKe32[576];
+
+template <class T> __device__ __forceinline__ T ke(int i);
+template <> __device__ __forceinline__ double ke<double>(int i) { return cKe64[i]; }
+template <> __device__ __forceinline__ float ke<float>(int i) { return cKe32[i]; }
+
+// Local node j of the corner (x, y, z) bits: node order of FEA_GaussianIntegral.m:2-9.
+__host__ __device__ constexpr int corner_index(int x, int y, int z) {
+    return (z ? 4 : 0) + (y ? (x ? 2 : 3) : (x ? 1 : 0));
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[16];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    v = (threadIdx.x < nw) ? sh[threadIdx.x] : 0.0;
+    if (w == 0)
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;  // valid in thread 0
+}
+
+__device__ __forceinline__ bool stopped(const Ctl *ctl) { return ctl && ctl->done; }
+
+// ----------------------------------------------------------------------------
+// K.u: y = sum_e E_e Ke u_e at every lattice node, gathered node-centrically
+// over the node's 8 incident voxels (no atomics, no colours).  Fixed and void
+// DOFs are set to zero (Solving_KbyU_MatrixFree.m:39).  Optional partial of
+// u.y for the PCG step length.  Grid-stride over nodes.
+// ----------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__restrict__ y,
+                                              const T *__restrict__ E,
+                                              const uint8_t *__restrict__ nflag, Grid gd,
+                                              double *__restrict__ partial, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const int64_t NY = gd.NY(), NX = gd.NX(), nN = gd.nodes();
+    const int64_t pl = NY * NX;
+    double dotacc = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nN;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / pl);
+        const uint8_t f = nflag[g];
+        T acc[3] = {0, 0, 0};
+        if (f & kSolid) {
+#pragma unroll
+            for (int dl = 0; dl < 2; ++dl)
+#pragma unroll
+                for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+                    for (int dr = 0; dr < 2; ++dr) {
+                        const int er = r - 1 + dr, ec = c - 1 + dc, el = l - 1 + dl;
+                        if (er < 0 || er >= gd.ny || ec < 0 || ec >= gd.nx || el < 0 || el >= gd.nz)
+                            continue;
+                        // this node is corner (x, y, z) = (1-dc, dr, 1-dl) of the element
+                        const int j = corner_index(1 - dc, dr, 1 - dl);
+                        const T Ee = E[er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el];
+                        T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+                            const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+                            const int zk = (k >= 4);
+                            const int64_t gk = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
+                            const T u0 = u[3 * gk], u1 = u[3 * gk + 1], u2 = u[3 * gk + 2];
+                            s0 += ke<T>((3 * j + 0) + 24 * (3 * k + 0)) * u0 +
+                                  ke<T>((3 * j + 0) + 24 * (3 * k + 1)) * u1 +
+                                  ke<T>((3 * j + 0) + 24 * (3 * k + 2)) * u2;
+                            s1 += ke<T>((3 * j + 1) + 24 * (3 * k + 0)) * u0 +
+                                  ke<T>((3 * j + 1) + 24 * (3 * k + 1)) * u1 +
+                                  ke<T>((3 * j + 1) + 24 * (3 * k + 2)) * u2;
+                            s2 += ke<T>((3 * j + 2) + 24 * (3 * k + 0)) * u0 +
+                                  ke<T>((3 * j + 2) + 24 * (3 * k + 1)) * u1 +
+                                  ke<T>((3 * j + 2) + 24 * (3 * k + 2)) * u2;
+                        }
+                        acc[0] += Ee * s0;
+                        acc[1] += Ee * s1;
+                        acc[2] += Ee * s2;
+                    }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const T v = ((f & kSolid) && !(f & (1 << a))) ? acc[a] : T(0);
+            y[3 * g + a] = v;
+            if (partial) dotacc += (double)u[3 * g + a] * (double)v;
+        }
+    }
+    if (partial) {
+        const double s = block_sum(dotacc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Restriction r_c = P^T r_f with trilinear P of span S: coarse node N gathers
+// fine nodes S N + d, |d_i| < S, weight prod (1 - |d_i|/S).
+// ----------------------------------------------------------------------------
+template <class T, int S>
+__global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid gf,
+                                                  T *__restrict__ rc, Grid gc,
+                                                  const uint8_t *__restrict__ cflag,
+                                                  const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const int64_t G = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= gc.nodes()) return;
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    const int R = (int)(G % NYc), Cc = (int)((G / NYc) % NXc), Lc = (int)(G / (NYc * NXc));
+    const int64_t NYf = gf.NY(), NXf = gf.NX(), NZf = gf.NZ();
+    T a0 = 0, a1 = 0, a2 = 0;
+    if (cflag[G] & kSolid) {
+#pragma unroll
+        for (int dl = -(S - 1); dl <= S - 1; ++dl) {
+            const int64_t lf = (int64_t)S * Lc + dl;
+            if (lf < 0 || lf >= NZf) continue;
+            const T wz = T(S - (dl < 0 ? -dl : dl)) / T(S);
+#pragma unroll
+            for (int dc = -(S - 1); dc <= S - 1; ++dc) {
+                const int64_t cf = (int64_t)S * Cc + dc;
+                if (cf < 0 || cf >= NXf) continue;
+                const T wx = T(S - (dc < 0 ? -dc : dc)) / T(S);
+#pragma unroll
+                for (int dr = -(S - 1); dr <= S - 1; ++dr) {
+                    const int64_t rr = (int64_t)S * R + dr;
+                    if (rr < 0 || rr >= NYf) continue;
+                    const T w = wz * wx * (T(S - (dr < 0 ? -dr : dr)) / T(S));
+                    const int64_t n = rr + NYf * cf + NYf * NXf * lf;
+                    a0 += w * rf[3 * n];
+                    a1 += w * rf[3 * n + 1];
+                    a2 += w * rf[3 * n + 2];
+                }
+            }
+        }
+    }
+    rc[3 * G] = a0;
+    rc[3 * G + 1] = a1;
+    rc[3 * G + 2] = a2;
+}
+
+// ----------------------------------------------------------------------------
+// Prolongation + both Jacobi terms of one V-cycle level (Solving_Vcycle.m:35-49):
+// x_f = P x_c + dinv_f r_f with dinv = 2 omega / diag; on the finest level the
+// fixed DOFs are zeroed (Solving_Vcycle.m:50) and the partial of x_f . r_f is
+// produced for the next PCG rho.
+// ----------------------------------------------------------------------------
+template <class T, int S>
+__global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid gc,
+                                                 T *__restrict__ xf, Grid gf,
+                                                 const T *__restrict__ rf,
+                                                 const T *__restrict__ dinv,
+                                                 const uint8_t *__restrict__ fflag, int finest,
+                                                 double *__restrict__ partial, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const int64_t NYf = gf.NY(), NXf = gf.NX(), nN = gf.nodes();
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    double dotacc = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nN;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(n % NYf), c = (int)((n / NYf) % NXf), l = (int)(n / (NYf * NXf));
+        const uint8_t f = fflag[n];
+        T v0 = 0, v1 = 0, v2 = 0;
+        if (f & kSolid) {
+            const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+#pragma unroll
+            for (int bl = 0; bl < 2; ++bl) {
+                if (bl && !ml) continue;
+                const T wl = bl ? T(ml) / T(S) : T(S - ml) / T(S);
+#pragma unroll
+                for (int bc = 0; bc < 2; ++bc) {
+                    if (bc && !mc) continue;
+                    const T wc = bc ? T(mc) / T(S) : T(S - mc) / T(S);
+#pragma unroll
+                    for (int br = 0; br < 2; ++br) {
+                        if (br && !mr) continue;
+                        const T w = wl * wc * (br ? T(mr) / T(S) : T(S - mr) / T(S));
+                        const int64_t G = (qr + br) + NYc * (qc + bc) + NYc * NXc * (ql + bl);
+                        v0 += w * xc[3 * G];
+                        v1 += w * xc[3 * G + 1];
+                        v2 += w * xc[3 * G + 2];
+                    }
+                }
+            }
+        }
+        T out[3] = {v0, v1, v2};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T val = out[a] + dinv[3 * n + a] * rf[3 * n + a];
+            if (finest && (f & (1 << a))) val = T(0);
+            xf[3 * n + a] = val;
+            if (partial) dotacc += (double)val * (double)rf[3 * n + a];
+        }
+    }
+    if (partial) {
+        const double s = block_sum(dotacc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// Coarsest level: x = Minv r with the dense inverse of K_L on free DOFs
+// (zero rows/columns elsewhere); one wavefront per row.
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_gemv(const double *__restrict__ Minv, int64_t n,
+                                                     const T *__restrict__ r, T *__restrict__ x,
+                                                     const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int64_t j = lane; j < n; j += 64) acc += Minv[row * n + j] * (double)r[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) x[row] = (T)acc;
+}
+
+// ----------------------------- PCG vector kernels ---------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b,
+                                             int64_t n, double *__restrict__ partial,
+                                             const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc += (double)a[i] * (double)b[i];
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// r = b - q (q may be null: r = b).
+template <class T>
+__global__ void k_residual(const T *__restrict__ b, const T *__restrict__ q, T *__restrict__ r,
+                           int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = q ? b[i] - q[i] : b[i];
+}
+
+// Solving_PreconditionedConjugateGradientSolver.m:15-18:
+// y += lambda p; r -= lambda z; partial r.r.
+template <class T>
+__global__ void __launch_bounds__(256) k_update_xr(T *__restrict__ x, T *__restrict__ r,
+                                                   const T *__restrict__ p,
+                                                   const T *__restrict__ q, int64_t n,
+                                                   double *__restrict__ partial, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const double lam = ctl->lambda;
+    const T lt = (T)lam;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = x[i] + lt * p[i];
+        const T rn = r[i] - lt * q[i];
+        r[i] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// :28 p = z + (x2/x1) p
+template <class T>
+__global__ void k_update_p(T *__restrict__ p, const T *__restrict__ z, int64_t n,
+                           const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const T beta = (T)ctl->beta;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
+template <class T>
+__global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// dst = (To)(src * scale)
+template <class To, class Ti>
+__global__ void k_convert(To *__restrict__ dst, const Ti *__restrict__ src, double scale,
+                          int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (To)((double)src[i] * scale);
+}
+
+// x += scale * d  (FP64 x, FP32 correction d)
+__global__ void k_axpy_mixed(double *__restrict__ x, const float *__restrict__ d, double scale,
+                             int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += scale * (double)d[i];
+}
+
+enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_PQ, FIN_RR, FIN_RHO };
+
+// Second reduction stage (fixed order, deterministic) and the scalar updates
+// of the PCG recurrence.
+__global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial, int nb, int op,
+                                              Ctl *ctl) {
+    if (op != FIN_NORMB && op != FIN_RHO0 && ctl->done) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
+    const double s = block_sum(acc);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_NORMB: ctl->normb = sqrt(s); break;
+        case FIN_RHO0: ctl->rho = s; break;
+        case FIN_PQ:
+            ctl->pq = s;
+            ctl->lambda = ctl->rho / s;
+            break;
+        case FIN_RR:
+            ctl->rr = s;
+            ctl->res = sqrt(s) / ctl->normb;
+            ctl->its += 1;
+            if (ctl->res < ctl->tol) ctl->done = 1;
+            else if (ctl->its >= ctl->maxit) ctl->done = 2;
+            break;
+        case FIN_RHO:
+            ctl->beta = s / ctl->rho;
+            ctl->rho = s;
+            break;
+    }
+}
+
+// ------------------------------- setup kernels ------------------------------
+// Finest diagonal: diag = sum_e E_e Ke_jj (Solving_AssembleFEAstencil.m:102-121);
+// dinv = 2 omega / diag on solid nodes.
+__global__ void k_diag0(const double *__restrict__ E, Grid gd, const uint8_t *__restrict__ nflag,
+                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= gd.nodes()) return;
+    const int64_t NY = gd.NY(), NX = gd.NX();
+    const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / (NY * NX));
+    double d[3] = {0, 0, 0};
+    for (int dl = 0; dl < 2; ++dl)
+        for (int dc = 0; dc < 2; ++dc)
+            for (int dr = 0; dr < 2; ++dr) {
+                const int er = r - 1 + dr, ec = c - 1 + dc, el = l - 1 + dl;
+                if (er < 0 || er >= gd.ny || ec < 0 || ec >= gd.nx || el < 0 || el >= gd.nz) continue;
+                const int j = corner_index(1 - dc, dr, 1 - dl);
+                const double Ee = E[er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el];
+                for (int a = 0; a < 3; ++a) d[a] += cKe64[(3 * j + a) * 25] * Ee;
+            }
+    const bool solid = nflag[g] & kSolid;
+    for (int a = 0; a < 3; ++a) {
+        const double v = solid ? 2.0 * omega / d[a] : 0.0;
+        dinv[3 * g + a] = v;
+        dinv32[3 * g + a] = (float)v;
+    }
+}
+
+// Coarse diagonal from element matrices (Solving_AssembleFEAstencil.m:82-99).
+__global__ void k_diagk(const double *__restrict__ Ks, Grid gd, const uint8_t *__restrict__ nflag,
+                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= gd.nodes()) return;
+    const int64_t NY = gd.NY(), NX = gd.NX();
+    const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / (NY * NX));
+    double d[3] = {0, 0, 0};
+    for (int dl = 0; dl < 2; ++dl)
+        for (int dc = 0; dc < 2; ++dc)
+            for (int dr = 0; dr < 2; ++dr) {
+                const int er = r - 1 + dr, ec = c - 1 + dc, el = l - 1 + dl;
+                if (er < 0 || er >= gd.ny || ec < 0 || ec >= gd.nx || el < 0 || el >= gd.nz) continue;
+                const int j = corner_index(1 - dc, dr, 1 - dl);
+                const int64_t e = er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el;
+                for (int a = 0; a < 3; ++a) d[a] += Ks[576 * e + (3 * j + a) * 25];
+            }
+    const bool solid = nflag[g] & kSolid;
+    for (int a = 0; a < 3; ++a) {
+        const double v = solid ? 2.0 * omega / d[a] : 0.0;
+        dinv[3 * g + a] = v;
+        dinv32[3 * g + a] = (float)v;
+    }
+}
+
+// Level-2 element matrices from the fine moduli and the span-s templates
+// H_q = P_q^T Ke P_q (Solving_AssembleCmptStencilFromFinestLevel.cpp:71-169).
+__global__ void k_galerkin_from_E(const double *__restrict__ H, int s, const double *__restrict__ E,
+                                  Grid gf, Grid gc, double *__restrict__ Ks) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= gc.elems() * 576) return;
+    const int64_t e = idx / 576;
+    const int t = (int)(idx % 576);
+    const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
+    double acc = 0.0;
+    const int nq = s * s * s;
+    for (int q = 0; q < nq; ++q) {
+        const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
+        const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
+                          (int64_t)gf.ny * gf.nx * (s * el + ql);
+        acc += E[v] * H[576 * q + t];
+    }
+    Ks[idx] = acc;
+}
+
+// Level >= 3: Ks_e = sum_q P_q^T Ks_prev(child q) P_q
+// (Solving_AssembleCmptStencilFromNonFinestLevel.cpp:78-143).  One block of 576
+// threads per coarse lattice element.
+__global__ void __launch_bounds__(576) k_galerkin_from_Ks(const double *__restrict__ Kf, Grid gf,
+                                                          Grid gc, int s,
+                                                          const double *__restrict__ P,
+                                                          double *__restrict__ Ks) {
+    __shared__ double sK[576], sT[576], sP[64];
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x;
+    const int row = t % 24, col = t / 24;
+    const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
+    double out = 0.0;
+    const int nq = s * s * s;
+    for (int q = 0; q < nq; ++q) {
+        const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
+        const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
+                          (int64_t)gf.ny * gf.nx * (s * el + ql);
+        __syncthreads();
+        sK[t] = Kf[576 * v + t];
+        if (t < 64) sP[t] = P[64 * q + t];
+        __syncthreads();
+        {
+            const int J = col / 3, a = col % 3;
+            double acc = 0.0;
+            for (int j = 0; j < 8; ++j) {
+                const double w = sP[j * 8 + J];
+                if (w != 0.0) acc += w * sK[row + 24 * (3 * j + a)];
+            }
+            sT[t] = acc;
+        }
+        __syncthreads();
+        {
+            const int I = row / 3, a = row % 3;
+            double acc = 0.0;
+            for (int i = 0; i < 8; ++i) {
+                const double w = sP[i * 8 + I];
+                if (w != 0.0) acc += w * sT[3 * i + a + 24 * col];
+            }
+            out += acc;
+        }
+    }
+    Ks[576 * e + t] = out;
+}
+
+// Compressed <-> lattice.
+template <class T>
+__global__ void k_scatter_nodes(const T *__restrict__ src, const int32_t *__restrict__ map,
+                                int64_t n, T *__restrict__ dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t g = map[j];
+    dst[3 * g] = src[3 * j];
+    dst[3 * g + 1] = src[3 * j + 1];
+    dst[3 * g + 2] = src[3 * j + 2];
+}
+template <class T>
+__global__ void k_gather_nodes(const T *__restrict__ src, const int32_t *__restrict__ map,
+                               int64_t n, T *__restrict__ dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t g = map[j];
+    dst[3 * j] = src[3 * g];
+    dst[3 * j + 1] = src[3 * g + 1];
+    dst[3 * j + 2] = src[3 * g + 2];
+}
+__global__ void k_scatter_elems(const double *__restrict__ src, const int32_t *__restrict__ map,
+                                int64_t n, double *__restrict__ dst, float *__restrict__ dst32) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t v = map ? map[j] : j;
+    dst[v] = src[j];
+    dst32[v] = (float)src[j];
+}
+
+// Unit compliance ce_e = u_e^T Ke u_e per lattice element
+// (TopOpti_CmptUnitCompliance_mex.cpp:61-107), then gathered to compressed order.
+__global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u, Grid gd,
+                                                    const int32_t *__restrict__ emap, int64_t ne,
+                                                    double *__restrict__ ce) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    const int64_t v = emap ? emap[i] : i;
+    const int er = (int)(v % gd.ny), ec = (int)((v / gd.ny) % gd.nx), el = (int)(v / ((int64_t)gd.ny * gd.nx));
+    const int64_t NY = gd.NY(), pl = gd.NY() * gd.NX();
+    double ue[24];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+        const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+        const int zk = (k >= 4);
+        const int64_t g = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
+        ue[3 * k] = u[3 * g];
+        ue[3 * k + 1] = u[3 * g + 1];
+        ue[3 * k + 2] = u[3 * g + 2];
+    }
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 24; ++r) {
+        double y = 0.0;
+#pragma unroll
+        for (int c = 0; c < 24; ++c) y += cKe64[r + 24 * c] * ue[c];
+        q += ue[r] * y;
+    }
+    ce[i] = q;
+}
+
+}  // namespace
+}  // namespace sgld
+
+// ============================== host context ================================
+using namespace sgld;
+
+namespace {
+struct Level {
+    Grid g{};
+    int span = 1;
+    int64_t numElements = 0, numNodes = 0;
+    std::vector<uint8_t> esolid;  // host, lattice elements
+    std::vector<uint8_t> hflag;   // host, lattice nodes
+    DevBuf<uint8_t> nflag;
+    DevBuf<double> dinv, r, x;
+    DevBuf<float> dinv32, r32, x32;
+};
+}  // namespace
+
+struct sgld_ctx {
+    int L = 0, nonDyadic = 0;
+    double nu = 0.3, h = 1.0, omega = 0.6;
+    double Ke[576];
+    std::vector<Level> lv;
+    bool full = true;
+    std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
+    DevBuf<int32_t> nodMap, eleMap;
+    DevBuf<double> E;
+    DevBuf<float> E32;
+    DevBuf<double> Minv;
+    int64_t nCoarseDof = 0;
+    DevBuf<double> b, x, r, z, p, q;
+    DevBuf<float> b32, r32, z32, p32, q32, x32;
+    DevBuf<double> partial;
+    DevBuf<Ctl> ctl;
+    Ctl *hctl = nullptr;  // pinned host mirror
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool haveFixed = false, haveModulus = false, assembled = false;
+    double stats[8] = {0};
+    int64_t nDof() const { return 3 * lv[0].g.nodes(); }
+};
+
+namespace {
+
+int grid_blocks(int64_t n) { return (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 255) / 256)); }
+
+template <class T> T *lvl_r(Level &l);
+template <> double *lvl_r<double>(Level &l) { return l.r.p; }
+template <> float *lvl_r<float>(Level &l) { return l.r32.p; }
+template <class T> T *lvl_x(Level &l);
+template <> double *lvl_x<double>(Level &l) { return l.x.p; }
+template <> float *lvl_x<float>(Level &l) { return l.x32.p; }
+template <class T> const T *lvl_dinv(Level &l);
+template <> const double *lvl_dinv<double>(Level &l) { return l.dinv.p; }
+template <> const float *lvl_dinv<float>(Level &l) { return l.dinv32.p; }
+template <class T> const T *ctx_E(sgld_ctx *C);
+template <> const double *ctx_E<double>(sgld_ctx *C) { return C->E.p; }
+template <> const float *ctx_E<float>(sgld_ctx *C) { return C->E32.p; }
+
+double *slot(sgld_ctx *C, int k) { return C->partial.p + (int64_t)k * kRedBlocks; }
+
+template <class T>
+int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl, int *nbOut) {
+    const Level &l0 = C->lv[0];
+    const int nb = grid_blocks(l0.g.nodes());
+    hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
+                       l0.g, partial, ctl);
+    SGLD_HIP(hipGetLastError());
+    if (nbOut) *nbOut = nb;
+    return SGLD_OK;
+}
+
+template <class T>
+int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
+    Level &F = C->lv[k - 1], &Cc = C->lv[k];
+    const int nb = blocks_for(Cc.g.nodes(), 256);
+    if (Cc.span == 2)
+        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g,
+                           Cc.nflag.p, ctl);
+    else
+        hipLaunchKernelGGL((k_restrict<T, 4>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g,
+                           Cc.nflag.p, ctl);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+template <class T>
+int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double *partial,
+                    const Ctl *ctl, int *nbOut) {
+    Level &F = C->lv[k - 1], &Cc = C->lv[k];
+    const int nb = grid_blocks(F.g.nodes());
+    const int finest = (k == 1);
+    if (Cc.span == 2)
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
+                           lvl_dinv<T>(F), F.nflag.p, finest, partial, ctl);
+    else
+        hipLaunchKernelGGL((k_prolong<T, 4>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
+                           lvl_dinv<T>(F), F.nflag.p, finest, partial, ctl);
+    SGLD_HIP(hipGetLastError());
+    if (nbOut) *nbOut = nb;
+    return SGLD_OK;
+}
+
+// z = M r (Solving_Vcycle.m), partial z.r into `partial` when non-null.
+template <class T>
+int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ctl, int *nbOut) {
+    const int L = C->L;
+    const T *rf = r;
+    for (int k = 1; k < L; ++k) {
+        SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
+        rf = lvl_r<T>(C->lv[k]);
+    }
+    Level &Lc = C->lv[L - 1];
+    const int64_t n = C->nCoarseDof;
+    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->Minv.p, n,
+                       lvl_r<T>(Lc), lvl_x<T>(Lc), ctl);
+    SGLD_HIP(hipGetLastError());
+    for (int k = L - 1; k >= 2; --k)
+        SGLD_TRY(enqueue_prolong<T>(C, k, lvl_x<T>(C->lv[k]), lvl_x<T>(C->lv[k - 1]),
+                                    lvl_r<T>(C->lv[k - 1]), nullptr, ctl, nullptr));
+    SGLD_TRY(enqueue_prolong<T>(C, 1, lvl_x<T>(C->lv[1]), z, r, partial, ctl, nbOut));
+    return SGLD_OK;
+}
+
+int fin(sgld_ctx *C, const double *partial, int nb, int op) {
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, C->s, partial, nb, op, C->ctl.p);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+// One PCG iteration (Solving_PreconditionedConjugateGradientSolver.m:14-29).
+template <class T>
+int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *z, T *p, T *q) {
+    const int64_t n = C->nDof();
+    Ctl *ctl = C->ctl.p;
+    int nb = 0;
+    SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
+    SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
+    const int nbv = grid_blocks(n);
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(nbv), dim3(256), 0, C->s, x, r, p, q, n, slot(C, 1), ctl);
+    SGLD_HIP(hipGetLastError());
+    SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
+    SGLD_TRY(enqueue_vcycle<T>(C, r, z, slot(C, 2), ctl, &nb));
+    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO));
+    hipLaunchKernelGGL(k_update_p<T>, dim3(nbv), dim3(256), 0, C->s, p, z, n, ctl);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+// PCG on lattice vectors.  b, x in T; x0zero skips the initial K.u.
+template <class T>
+int run_pcg(sgld_ctx *C, const T *b, T *x, T *r, T *z, T *p, T *q, bool x0zero, double tol,
+            int maxit, int *itsOut, double *resOut, int *doneOut) {
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    Ctl init{};
+    init.tol = tol;
+    init.maxit = maxit;
+    SGLD_HIP(hipMemcpyAsync(C->ctl.p, &init, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
+    hipLaunchKernelGGL(k_dot<T>, dim3(nbv), dim3(256), 0, C->s, b, b, n, slot(C, 3), nullptr);
+    SGLD_HIP(hipGetLastError());
+    SGLD_TRY(fin(C, slot(C, 3), nbv, FIN_NORMB));
+    if (!x0zero) SGLD_TRY(enqueue_kbyu<T>(C, x, q, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(k_residual<T>, dim3(nbv), dim3(256), 0, C->s, b, x0zero ? nullptr : q, r, n);
+    SGLD_HIP(hipGetLastError());
+    int nb = 0;
+    SGLD_TRY(enqueue_vcycle<T>(C, r, z, slot(C, 2), nullptr, &nb));
+    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));
+    hipLaunchKernelGGL(k_copy<T>, dim3(nbv), dim3(256), 0, C->s, p, z, n);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    if (!(C->hctl->normb > 0.0)) {  // F = 0: solution 0, nothing to iterate
+        *itsOut = 0;
+        *resOut = 0.0;
+        *doneOut = 1;
+        return SGLD_OK;
+    }
+    const int batch = 8;
+    while (true) {
+        for (int i = 0; i < batch; ++i) SGLD_TRY(enqueue_iteration<T>(C, x, r, z, p, q));
+        SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
+        SGLD_HIP(hipStreamSynchronize(C->s));
+        if (C->hctl->done) break;
+    }
+    *itsOut = C->hctl->its;
+    *resOut = C->hctl->res;
+    *doneOut = C->hctl->done;
+    return SGLD_OK;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Dense Cholesky (lower, col-major, left-looking) in place; 0 on success.
+int host_cholesky(std::vector<double> &A, int64_t n) {
+    for (int64_t j = 0; j < n; ++j) {
+        double *Aj = A.data() + n * j;
+        for (int64_t k = 0; k < j; ++k) {
+            const double *Ak = A.data() + n * k;
+            const double ljk = Ak[j];
+            if (ljk == 0.0) continue;
+            for (int64_t i = j; i < n; ++i) Aj[i] -= Ak[i] * ljk;
+        }
+        const double d = Aj[j];
+        if (!(d > 0.0)) return 1;
+        const double sd = std::sqrt(d), inv = 1.0 / sd;
+        Aj[j] = sd;
+        for (int64_t i = j + 1; i < n; ++i) Aj[i] *= inv;
+    }
+    return 0;
+}
+
+// Inverse of SPD A from its Cholesky factor L: A^-1 = W^T W with W = L^-1.
+void host_chol_inverse(const std::vector<double> &Lf, int64_t n, std::vector<double> &Ainv) {
+    std::vector<double> W((size_t)n * n, 0.0);  // lower, col-major
+    for (int64_t c = 0; c < n; ++c) {  // column c of L^-1: L w = e_c
+        double *w = W.data() + n * c;
+        w[c] = 1.0;
+        for (int64_t k = c; k < n; ++k) {
+            const double wk = w[k] / Lf[k + n * k];
+            w[k] = wk;
+            if (wk == 0.0) continue;
+            const double *Lk = Lf.data() + n * k;
+            for (int64_t i = k + 1; i < n; ++i) w[i] -= Lk[i] * wk;
+        }
+    }
+    Ainv.assign((size_t)n * n, 0.0);
+    for (int64_t j = 0; j < n; ++j) {
+        const double *wj = W.data() + n * j;
+        for (int64_t i = 0; i <= j; ++i) {
+            const double *wi = W.data() + n * i;
+            double acc = 0.0;
+            for (int64_t k = j; k < n; ++k) acc += wi[k] * wj[k];
+            Ainv[i + n * j] = acc;
+            Ainv[j + n * i] = acc;
+        }
+    }
+}
+
+int upload_flags(Level &l) {
+    if (!l.nflag.p) SGLD_TRY(l.nflag.alloc(l.g.nodes()));
+    SGLD_HIP(hipMemcpy(l.nflag.p, l.hflag.data(), l.hflag.size(), hipMemcpyHostToDevice));
+    return SGLD_OK;
+}
+
+int to_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
+    const int64_t n = C->nDof();
+    if (C->full) {
+        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * n, hipMemcpyDeviceToDevice, C->s));
+        return SGLD_OK;
+    }
+    SGLD_HIP(hipMemsetAsync(ddst, 0, sizeof(double) * n, C->s));
+    const int64_t nn = C->lv[0].numNodes;
+    hipLaunchKernelGGL(k_scatter_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
+                       C->nodMap.p, nn, ddst);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int from_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
+    if (C->full) {
+        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * C->nDof(), hipMemcpyDeviceToDevice, C->s));
+        return SGLD_OK;
+    }
+    const int64_t nn = C->lv[0].numNodes;
+    hipLaunchKernelGGL(k_gather_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
+                       C->nodMap.p, nn, ddst);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int check_ready(sgld_ctx *C) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    if (!C->assembled) {
+        set_error("sgld:state: call sgld_ctx_set_fixed, sgld_ctx_set_modulus and sgld_ctx_assemble first");
+        return SGLD_ERR_STATE;
+    }
+    return SGLD_OK;
+}
+
+// FP64 solve on lattice vectors b -> x (x holds x0).
+int solve_fp64(sgld_ctx *C, double tol, int maxit, int *its, double *res, int *done) {
+    return run_pcg<double>(C, C->b.p, C->x.p, C->r.p, C->z.p, C->p.p, C->q.p, false, tol, maxit,
+                           its, res, done);
+}
+
+// True FP64 relative residual ||b - K x|| / ||b||; leaves b - Kx in C->r.
+int true_residual(sgld_ctx *C, double *res, double *normr) {
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, n);
+    SGLD_HIP(hipGetLastError());
+    std::vector<double> part(2 * kRedBlocks);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->r.p, C->r.p, n, slot(C, 0), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->b.p, n, slot(C, 1), nullptr);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
+                            hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    double rr = 0, bb = 0;
+    for (int i = 0; i < nbv; ++i) { rr += part[i]; bb += part[kRedBlocks + i]; }
+    *normr = std::sqrt(rr);
+    *res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    return SGLD_OK;
+}
+
+// Mixed precision: FP64 outer iterative refinement on the true residual, inner
+// PCG entirely in FP32 (K.u, V-cycle, vectors) with FP64 dot products and
+// alpha/beta, FP64 coarsest inverse.
+int solve_mixed(sgld_ctx *C, double tol, int maxit, int *itsTot, double *resOut, int *outer) {
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    double res = 0, normr = 0;
+    int total = 0, nouter = 0;
+    SGLD_TRY(true_residual(C, &res, &normr));
+    const double innerTol = 1e-4;
+    while (res >= tol && total < maxit && nouter < 50) {
+        // d solves K d = r / ||r|| in FP32
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->b32.p, C->r.p,
+                           1.0 / normr, n);
+        SGLD_HIP(hipMemsetAsync(C->x32.p, 0, sizeof(float) * n, C->s));
+        int its = 0, done = 0;
+        double rin = 0;
+        const double want = std::max(innerTol, 0.5 * tol / std::max(res, 1e-300));
+        SGLD_TRY(run_pcg<float>(C, C->b32.p, C->x32.p, C->r32.p, C->z32.p, C->p32.p, C->q32.p, true,
+                                std::min(want, 0.5), maxit - total, &its, &rin, &done));
+        total += its;
+        ++nouter;
+        hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, normr, n);
+        SGLD_HIP(hipGetLastError());
+        SGLD_TRY(true_residual(C, &res, &normr));
+        if (its == 0) break;
+    }
+    *itsTot = total;
+    *resOut = res;
+    *outer = nouter;
+    return SGLD_OK;
+}
+
+}  // namespace
+
+// ================================= C ABI ====================================
+extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, const uint8_t *voxels,
+                               int numLevels, int nonDyadic, double nu, double h) {
+    SGLD_ARG(out, "sgld:arg: ctx out-pointer is null");
+    *out = nullptr;
+    SGLD_ARG(resX > 0 && resY > 0 && resZ > 0, "sgld:arg: resolution must be positive");
+    SGLD_ARG(numLevels >= 2 && numLevels <= 12, "sgld:arg: numLevels must be in 2..12");
+    SGLD_ARG(h > 0 && nu > -1 && nu < 0.5, "sgld:arg: need cellSize > 0, -1 < nu < 0.5");
+    const int64_t nvox = (int64_t)resX * resY * resZ;
+    SGLD_ARG((int64_t)(resX + 1) * (resY + 1) * (resZ + 1) * 3 < ((int64_t)1 << 31),
+             "sgld:arg: lattice too large for 32-bit DOF ids");
+    std::unique_ptr<sgld_ctx> C(new sgld_ctx());
+    // Solving_BuildingMeshHierarchy.m:12-16
+    if (nonDyadic == 1 && numLevels >= 4) numLevels -= 1; else nonDyadic = 0;
+    C->L = numLevels;
+    C->nonDyadic = nonDyadic;
+    C->nu = nu;
+    C->h = h;
+    element_stiffness(nu, h, C->Ke);
+    C->lv.resize(numLevels);
+    Level &l0 = C->lv[0];
+    l0.g = Grid{resX, resY, resZ};
+    l0.esolid.assign(nvox, 1);
+    if (voxels) for (int64_t v = 0; v < nvox; ++v) l0.esolid[v] = voxels[v] != 0;
+    for (int k = 1; k < numLevels; ++k) {
+        const int s = (k == 1 && nonDyadic) ? 4 : 2;
+        const Grid &gf = C->lv[k - 1].g;
+        SGLD_ARG(gf.nx % s == 0 && gf.ny % s == 0 && gf.nz % s == 0,
+                 "sgld:arg: level %d resolution %dx%dx%d not divisible by span %d", k, gf.nx, gf.ny,
+                 gf.nz, s);
+        Level &lc = C->lv[k];
+        lc.span = s;
+        lc.g = Grid{gf.nx / s, gf.ny / s, gf.nz / s};
+        lc.esolid.assign(lc.g.elems(), 0);
+        const auto &ef = C->lv[k - 1].esolid;
+        // Solving_BuildingMeshHierarchy.m:79 (coarse element exists iff a child is solid)
+        for (int64_t v = 0; v < gf.elems(); ++v) {
+            if (!ef[v]) continue;
+            const int64_t r = v % gf.ny, c = (v / gf.ny) % gf.nx, l = v / ((int64_t)gf.ny * gf.nx);
+            lc.esolid[(r / s) + (int64_t)lc.g.ny * (c / s) + (int64_t)lc.g.ny * lc.g.nx * (l / s)] = 1;
+        }
+    }
+    for (int k = 0; k < numLevels; ++k) {
+        Level &l = C->lv[k];
+        const Grid &g = l.g;
+        l.hflag.assign(g.nodes(), 0);
+        const int64_t NY = g.NY(), NX = g.NX();
+        int64_t ne = 0;
+        for (int64_t v = 0; v < g.elems(); ++v) {
+            if (!l.esolid[v]) continue;
+            ++ne;
+            const int64_t r = v % g.ny, c = (v / g.ny) % g.nx, ll = v / ((int64_t)g.ny * g.nx);
+            for (int dz = 0; dz < 2; ++dz)
+                for (int dx = 0; dx < 2; ++dx)
+                    for (int dy = 0; dy < 2; ++dy) l.hflag[(r + dy) + NY * (c + dx) + NY * NX * (ll + dz)] = kSolid;
+        }
+        l.numElements = ne;
+        int64_t nn = 0;
+        for (auto f : l.hflag) nn += (f & kSolid) ? 1 : 0;
+        l.numNodes = nn;
+    }
+    Level &f0 = C->lv[0];
+    C->full = (f0.numElements == f0.g.elems());
+    if (!C->full) {
+        C->hNodMap.reserve(f0.numNodes);
+        for (int64_t g = 0; g < f0.g.nodes(); ++g)
+            if (f0.hflag[g] & kSolid) C->hNodMap.push_back((int32_t)g);
+        C->hEleMap.reserve(f0.numElements);
+        for (int64_t v = 0; v < f0.g.elems(); ++v)
+            if (f0.esolid[v]) C->hEleMap.push_back((int32_t)v);
+        SGLD_TRY(C->nodMap.upload(C->hNodMap.data(), C->hNodMap.size()));
+        SGLD_TRY(C->eleMap.upload(C->hEleMap.data(), C->hEleMap.size()));
+    }
+    SGLD_HIP(hipStreamCreateWithFlags(&C->s, hipStreamNonBlocking));
+    SGLD_HIP(hipEventCreate(&C->e0));
+    SGLD_HIP(hipEventCreate(&C->e1));
+    SGLD_HIP(hipHostMalloc((void **)&C->hctl, sizeof(Ctl), hipHostMallocDefault));
+    // device buffers
+    const int64_t nd = C->nDof();
+    for (int k = 0; k < numLevels; ++k) {
+        Level &l = C->lv[k];
+        const int64_t n = 3 * l.g.nodes();
+        SGLD_TRY(upload_flags(l));
+        SGLD_TRY(l.dinv.alloc(n));
+        SGLD_TRY(l.dinv32.alloc(n));
+        if (k > 0) {
+            SGLD_TRY(l.r.alloc(n)); SGLD_TRY(l.x.alloc(n));
+            SGLD_TRY(l.r32.alloc(n)); SGLD_TRY(l.x32.alloc(n));
+        }
+    }
+    SGLD_TRY(C->E.alloc(f0.g.elems()));
+    SGLD_TRY(C->E32.alloc(f0.g.elems()));
+    SGLD_TRY(C->E.zero(C->s));
+    SGLD_HIP(hipMemsetAsync(C->E32.p, 0, sizeof(float) * f0.g.elems(), C->s));
+    for (DevBuf<double> *v : {&C->b, &C->x, &C->r, &C->z, &C->p, &C->q}) {
+        SGLD_TRY(v->alloc(nd));
+        SGLD_TRY(v->zero(C->s));
+    }
+    for (DevBuf<float> *v : {&C->b32, &C->r32, &C->z32, &C->p32, &C->q32, &C->x32}) {
+        SGLD_TRY(v->alloc(nd));
+        SGLD_HIP(hipMemsetAsync(v->p, 0, sizeof(float) * nd, C->s));
+    }
+    SGLD_TRY(C->partial.alloc((int64_t)kRedSlots * kRedBlocks));
+    SGLD_TRY(C->ctl.alloc(1));
+    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe64), C->Ke, sizeof(double) * 576));
+    float k32[576];
+    for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
+    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    *out = C.release();
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_destroy(sgld_ctx *C) {
+    if (!C) return SGLD_OK;
+    if (C->s) (void)hipStreamSynchronize(C->s);
+    if (C->hctl) (void)hipHostFree(C->hctl);
+    if (C->e0) (void)hipEventDestroy(C->e0);
+    if (C->e1) (void)hipEventDestroy(C->e1);
+    hipStream_t s = C->s;
+    delete C;
+    if (s) (void)hipStreamDestroy(s);
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_info(const sgld_ctx *C, int level, int64_t *out) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    if (level == 0) return C->L;
+    SGLD_ARG(level >= 1 && level <= C->L && out, "sgld:arg: level out of range");
+    const Level &l = C->lv[level - 1];
+    out[0] = l.g.nx; out[1] = l.g.ny; out[2] = l.g.nz; out[3] = l.span;
+    out[4] = l.numElements; out[5] = l.numNodes; out[6] = 3 * l.numNodes; out[7] = l.g.nodes();
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_set_fixed(sgld_ctx *C, const uint8_t *fixed) {
+    SGLD_ARG(C && fixed, "sgld:arg: null argument");
+    Level &l0 = C->lv[0];
+    for (auto &f : l0.hflag) f &= kSolid;
+    const int64_t nn = l0.numNodes;
+    std::vector<double> ind(C->nDof(), 0.0);
+    for (int64_t j = 0; j < nn; ++j) {
+        const int64_t g = C->full ? j : C->hNodMap[j];
+        for (int a = 0; a < 3; ++a)
+            if (fixed[3 * j + a]) {
+                l0.hflag[g] |= (uint8_t)(1 << a);
+                ind[3 * g + a] = 1.0;
+            }
+    }
+    SGLD_TRY(upload_flags(l0));
+    // FEA_SetupVoxelBased.m:13-26: restrict the indicator, fixed iff != 0.
+    SGLD_HIP(hipMemcpy(C->r.p, ind.data(), sizeof(double) * ind.size(), hipMemcpyHostToDevice));
+    const double *rf = C->r.p;
+    for (int k = 1; k < C->L; ++k) {
+        Level &lc = C->lv[k];
+        SGLD_TRY(enqueue_restrict<double>(C, k, rf, lc.r.p, nullptr));
+        std::vector<double> hv(3 * lc.g.nodes());
+        SGLD_HIP(hipMemcpyAsync(hv.data(), lc.r.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, C->s));
+        SGLD_HIP(hipStreamSynchronize(C->s));
+        for (int64_t g = 0; g < lc.g.nodes(); ++g) {
+            lc.hflag[g] &= kSolid;
+            for (int a = 0; a < 3; ++a)
+                if (hv[3 * g + a] != 0.0) lc.hflag[g] |= (uint8_t)(1 << a);
+        }
+        SGLD_TRY(upload_flags(lc));
+        rf = lc.r.p;
+    }
+    C->haveFixed = true;
+    C->assembled = false;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_set_modulus(sgld_ctx *C, const double *E) {
+    SGLD_ARG(C && E, "sgld:arg: null argument");
+    const int64_t ne = C->lv[0].numElements;
+    for (int64_t i = 0; i < ne; ++i)
+        SGLD_ARG(std::isfinite(E[i]) && E[i] > 0.0,
+                 "sgld:arg: eleModulus[%lld] = %g must be finite and > 0", (long long)i, E[i]);
+    DevBuf<double> tmp;
+    SGLD_TRY(tmp.upload(E, ne));
+    hipLaunchKernelGGL(k_scatter_elems, dim3(blocks_for(ne, 256)), dim3(256), 0, C->s, tmp.p,
+                       C->full ? nullptr : C->eleMap.p, ne, C->E.p, C->E32.p);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    C->haveModulus = true;
+    C->assembled = false;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    SGLD_ARG(omega > 0 && omega < 2, "sgld:arg: omega must be in (0, 2)");
+    if (!C->haveFixed || !C->haveModulus) {
+        set_error("sgld:state: set fixed DOFs and modulus before assembling");
+        return SGLD_ERR_STATE;
+    }
+    const double t0 = now_s();
+    C->omega = omega;
+    Level &l0 = C->lv[0];
+    hipLaunchKernelGGL(k_diag0, dim3(blocks_for(l0.g.nodes(), 256)), dim3(256), 0, C->s, C->E.p, l0.g,
+                       l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p);
+    SGLD_HIP(hipGetLastError());
+    DevBuf<double> KsPrev, KsCur;
+    for (int k = 1; k < C->L; ++k) {
+        Level &lf = C->lv[k - 1], &lc = C->lv[k];
+        const int s = lc.span, nq = s * s * s;
+        SGLD_TRY(KsCur.alloc(576 * lc.g.elems()));
+        if (k == 1) {
+            std::vector<double> H(576 * nq, 0.0);
+            for (int q = 0; q < nq; ++q) {
+                double p[8][8];
+                child_prolongation(s, q, p);
+                galerkin_child_add(p, C->Ke, H.data() + 576 * q);
+            }
+            DevBuf<double> dH;
+            SGLD_TRY(dH.upload(H.data(), H.size()));
+            hipLaunchKernelGGL(k_galerkin_from_E, dim3(blocks_for(576 * lc.g.elems(), 256)), dim3(256), 0,
+                               C->s, dH.p, s, C->E.p, lf.g, lc.g, KsCur.p);
+            SGLD_HIP(hipGetLastError());
+            SGLD_HIP(hipStreamSynchronize(C->s));
+        } else {
+            std::vector<double> P(64 * nq);
+            for (int q = 0; q < nq; ++q) {
+                double p[8][8];
+                child_prolongation(s, q, p);
+                std::memcpy(P.data() + 64 * q, p, sizeof(p));
+            }
+            DevBuf<double> dP;
+            SGLD_TRY(dP.upload(P.data(), P.size()));
+            hipLaunchKernelGGL(k_galerkin_from_Ks, dim3((unsigned)lc.g.elems()), dim3(576), 0, C->s,
+                               KsPrev.p, lf.g, lc.g, s, dP.p, KsCur.p);
+            SGLD_HIP(hipGetLastError());
+            SGLD_HIP(hipStreamSynchronize(C->s));
+        }
+        if (k < C->L - 1) {
+            hipLaunchKernelGGL(k_diagk, dim3(blocks_for(lc.g.nodes(), 256)), dim3(256), 0, C->s, KsCur.p,
+                               lc.g, lc.nflag.p, omega, lc.dinv.p, lc.dinv32.p);
+            SGLD_HIP(hipGetLastError());
+        }
+        KsPrev = std::move(KsCur);
+    }
+    // Coarsest: assemble K_L on its lattice DOFs, keep solid non-fixed ones,
+    // factor, invert (Solving_AssembleFEAstencil.m:123-135, Solving_Vcycle.m:29-31).
+    Level &lc = C->lv[C->L - 1];
+    const int64_t ne = lc.g.elems(), nn = lc.g.nodes(), nd = 3 * nn;
+    std::vector<double> Ks(576 * ne);
+    SGLD_HIP(hipMemcpyAsync(Ks.data(), KsPrev.p, sizeof(double) * Ks.size(), hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    KsPrev.release();
+    std::vector<int64_t> pos(nd, -1), freeIdx;
+    for (int64_t g = 0; g < nn; ++g)
+        for (int a = 0; a < 3; ++a)
+            if ((lc.hflag[g] & kSolid) && !(lc.hflag[g] & (1 << a))) {
+                pos[3 * g + a] = (int64_t)freeIdx.size();
+                freeIdx.push_back(3 * g + a);
+            }
+    const int64_t nf = (int64_t)freeIdx.size();
+    std::vector<double> A((size_t)nf * nf, 0.0);
+    const int64_t NY = lc.g.NY(), NX = lc.g.NX();
+    for (int64_t e = 0; e < ne; ++e) {
+        if (!lc.esolid[e]) continue;
+        const int64_t er = e % lc.g.ny, ec = (e / lc.g.ny) % lc.g.nx, el = e / ((int64_t)lc.g.ny * lc.g.nx);
+        int64_t dof[24];
+        for (int k = 0; k < 8; ++k) {
+            const int64_t g = (er + 1 - kCornerY[k]) + NY * (ec + kCornerX[k]) + NY * NX * (el + kCornerZ[k]);
+            for (int a = 0; a < 3; ++a) dof[3 * k + a] = pos[3 * g + a];
+        }
+        for (int c = 0; c < 24; ++c) {
+            if (dof[c] < 0) continue;
+            for (int r = 0; r < 24; ++r)
+                if (dof[r] >= 0) A[dof[r] + nf * dof[c]] += Ks[576 * e + r + 24 * c];
+        }
+    }
+    if (host_cholesky(A, nf)) {
+        set_error("sgld:notspd: coarsest-level matrix is not positive definite (free DOFs %lld)", (long long)nf);
+        return SGLD_ERR_NOT_SPD;
+    }
+    std::vector<double> Ainv;
+    host_chol_inverse(A, nf, Ainv);
+    std::vector<double> M((size_t)nd * nd, 0.0);
+    for (int64_t j = 0; j < nf; ++j)
+        for (int64_t i = 0; i < nf; ++i) M[freeIdx[i] * nd + freeIdx[j]] = Ainv[i + nf * j];
+    SGLD_TRY(C->Minv.upload(M.data(), M.size()));
+    C->nCoarseDof = nd;
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    C->stats[4] = now_s() - t0;
+    C->assembled = true;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_kbyu(sgld_ctx *C, const double *u, double *y) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(u && y, "sgld:arg: null vector");
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    DevBuf<double> du, dy;
+    SGLD_TRY(du.upload(u, nc));
+    SGLD_TRY(dy.alloc(nc));
+    SGLD_TRY(to_lattice(C, du.p, C->z.p));
+    SGLD_TRY(enqueue_kbyu<double>(C, C->z.p, C->q.p, nullptr, nullptr, nullptr));
+    SGLD_TRY(from_lattice(C, C->q.p, dy.p));
+    SGLD_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * nc, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_vcycle(sgld_ctx *C, const double *r, double *z) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(r && z, "sgld:arg: null vector");
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    DevBuf<double> dr, dz;
+    SGLD_TRY(dr.upload(r, nc));
+    SGLD_TRY(dz.alloc(nc));
+    SGLD_TRY(to_lattice(C, dr.p, C->r.p));
+    SGLD_TRY(enqueue_vcycle<double>(C, C->r.p, C->z.p, nullptr, nullptr, nullptr));
+    SGLD_TRY(from_lattice(C, C->z.p, dz.p));
+    SGLD_HIP(hipMemcpyAsync(z, dz.p, sizeof(double) * nc, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_pcg_device(sgld_ctx *C, const double *dF, double *dU, double tol, int maxIT,
+                                   int precision, int *itsOut, double *relres) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(dF && dU, "sgld:arg: null vector");
+    SGLD_ARG(tol > 0 && maxIT >= 1, "sgld:arg: need tol > 0 and maxIT >= 1");
+    SGLD_ARG(precision == SGLD_PREC_FP64 || precision == SGLD_PREC_MIXED, "sgld:arg: unknown precision mode");
+    SGLD_HIP(hipDeviceSynchronize());
+    const double t0 = now_s();
+    SGLD_TRY(to_lattice(C, dF, C->b.p));
+    SGLD_TRY(to_lattice(C, dU, C->x.p));
+    int its = 0, done = 0, outer = 0;
+    double res = 0, tres = 0, normr = 0;
+    if (precision == SGLD_PREC_FP64) {
+        SGLD_TRY(solve_fp64(C, tol, maxIT, &its, &res, &done));
+    } else {
+        SGLD_TRY(solve_mixed(C, tol, maxIT, &its, &res, &outer));
+        done = res < tol ? 1 : 2;
+    }
+    SGLD_TRY(from_lattice(C, C->x.p, dU));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    C->stats[0] = now_s() - t0;
+    SGLD_TRY(true_residual(C, &tres, &normr));
+    C->stats[1] = its;
+    C->stats[2] = tres;
+    C->stats[3] = outer;
+    if (itsOut) *itsOut = its;
+    if (relres) *relres = tres;
+    if (done != 1) {
+        set_error("sgld:maxit: PCG stopped at maxIT = %d with relative residual %.3e", maxIT, tres);
+        return SGLD_ERR_MAXIT;
+    }
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_pcg(sgld_ctx *C, const double *F, double *U, double tol, int maxIT,
+                            int precision, int *its, double *relres) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(F && U, "sgld:arg: null vector");
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    DevBuf<double> dF, dU;
+    SGLD_TRY(dF.upload(F, nc));
+    SGLD_TRY(dU.upload(U, nc));
+    const int rc = sgld_ctx_pcg_device(C, dF.p, dU.p, tol, maxIT, precision, its, relres);
+    if (rc != SGLD_OK && rc != SGLD_ERR_MAXIT) return rc;
+    SGLD_HIP(hipMemcpy(U, dU.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+extern "C" int sgld_ctx_unit_compliance(sgld_ctx *C, const double *U, double *ce) {
+    SGLD_ARG(C && U && ce, "sgld:arg: null argument");
+    const int64_t nc = 3 * C->lv[0].numNodes, ne = C->lv[0].numElements;
+    DevBuf<double> dU, dce;
+    SGLD_TRY(dU.upload(U, nc));
+    SGLD_TRY(dce.alloc(ne));
+    SGLD_TRY(to_lattice(C, dU.p, C->z.p));
+    hipLaunchKernelGGL(k_compliance, dim3(blocks_for(ne, 256)), dim3(256), 0, C->s, C->z.p, C->lv[0].g,
+                       C->full ? nullptr : C->eleMap.p, ne, dce.p);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipMemcpyAsync(ce, dce.p, sizeof(double) * ne, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, int precision) {
+    SGLD_ARG(C && du && dy, "sgld:arg: null argument");
+    SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
+    SGLD_TRY(to_lattice(C, du, C->z.p));
+    if (precision == SGLD_PREC_FP64) {
+        SGLD_HIP(hipEventRecord(C->e0, C->s));
+        SGLD_TRY(enqueue_kbyu<double>(C, C->z.p, C->q.p, nullptr, nullptr, nullptr));
+        SGLD_HIP(hipEventRecord(C->e1, C->s));
+        SGLD_TRY(from_lattice(C, C->q.p, dy));
+    } else {
+        const int64_t n = C->nDof();
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, C->s, C->z32.p,
+                           C->z.p, 1.0, n);
+        SGLD_HIP(hipEventRecord(C->e0, C->s));
+        SGLD_TRY(enqueue_kbyu<float>(C, C->z32.p, C->q32.p, nullptr, nullptr, nullptr));
+        SGLD_HIP(hipEventRecord(C->e1, C->s));
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_blocks(n)), dim3(256), 0, C->s, C->q.p,
+                           C->q32.p, 1.0, n);
+        SGLD_TRY(from_lattice(C, C->q.p, dy));
+    }
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    float ms = 0;
+    SGLD_HIP(hipEventElapsedTime(&ms, C->e0, C->e1));
+    C->stats[precision == SGLD_PREC_FP64 ? 6 : 7] = ms;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_bench_kbyu(sgld_ctx *C, int precision, int reps, double *msPerLaunch) {
+    SGLD_ARG(C && msPerLaunch && reps > 0, "sgld:arg: bad argument");
+    SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    SGLD_HIP(hipEventRecord(C->e0, C->s));
+    for (int i = 0; i < reps; ++i) {
+        if (precision == SGLD_PREC_FP64)
+            SGLD_TRY(enqueue_kbyu<double>(C, C->p.p, C->q.p, nullptr, nullptr, nullptr));
+        else
+            SGLD_TRY(enqueue_kbyu<float>(C, C->p32.p, C->q32.p, nullptr, nullptr, nullptr));
+    }
+    SGLD_HIP(hipEventRecord(C->e1, C->s));
+    SGLD_HIP(hipEventSynchronize(C->e1));
+    float ms = 0;
+    SGLD_HIP(hipEventElapsedTime(&ms, C->e0, C->e1));
+    *msPerLaunch = ms / reps;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_stats(const sgld_ctx *C, double *out) {
+    SGLD_ARG(C && out, "sgld:arg: null argument");
+    for (int i = 0; i < 8; ++i) out[i] = C->stats[i];
+    return SGLD_OK;
+}

@@ -1,0 +1,43 @@
+// Device kernels of the structured (lattice) solver path.
+//
+// Layout in HBM.  Every multigrid level k is a full node lattice of
+// (ny+1) x (nx+1) x (nz+1) nodes, id g = row + (ny+1) col + (ny+1)(nx+1) layer
+// (row 0 at the top, as src/FEA_VoxelBasedDiscretization.m:71-85 numbers
+// nodes); DOF vectors are interleaved, 3g + {0,1,2}.  Void nodes hold zeros,
+// so the compressed numbering of the reference (solid nodes in ascending g) is
+// a gather of this layout and, for a full cuboid, identical to it.  Element
+// data (modulus E, Galerkin Ks) is indexed by the voxel lattice id
+// v = row + ny col + ny nx layer.  Per-node flags: bits 0-2 fixed x/y/z, bit
+// 3 solid node.
+//
+// Because the grid is implicit, the transfers are fixed trilinear stencils:
+// R = P^T with P(n, N) = prod_d (1 - |n_d - s N_d| / s), which is exactly the
+// reference's ghost-node operator (multiplicity-averaged duplicates of the same
+// trilinear value, Solving_Interpolation_MatrixFree_mex.cpp:75-129).
+#pragma once
+#include "common.hpp"
+
+namespace sgld {
+
+constexpr uint8_t kSolid = 8;
+
+struct Grid {
+    int nx, ny, nz;  // elements per axis
+    __host__ __device__ int64_t NX() const { return nx + 1; }
+    __host__ __device__ int64_t NY() const { return ny + 1; }
+    __host__ __device__ int64_t NZ() const { return nz + 1; }
+    __host__ __device__ int64_t nodes() const { return NX() * NY() * NZ(); }
+    __host__ __device__ int64_t elems() const { return (int64_t)nx * ny * nz; }
+};
+
+// Device-side PCG state (one per solve); all solver kernels read it so that a
+// batch of iterations can be queued (or graph-captured) without host syncs.
+struct Ctl {
+    double rho, pq, rr, normb, lambda, beta, res, tol;
+    int its, maxit, done, pad;
+};
+
+constexpr int kRedBlocks = 2048;  // grid size (and partial-sum slots) of reducing kernels
+constexpr int kRedSlots = 4;      // independent reductions per solve
+
+}  // namespace sgld

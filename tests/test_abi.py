@@ -1,0 +1,70 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and rejects malformed arguments (validated on the host before
+any device call) with the reference's error identifiers."""
+import re
+
+import numpy as np
+import pytest
+
+from sgldbench_amd import _lib, solver
+
+
+def header_symbols():
+    text = open("include/sgldfea.h").read()
+    return sorted(set(re.findall(r"\b(sgld_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = header_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_version_string():
+    assert b"gfx950" in _lib.lib().sgld_version()
+
+
+def test_element_stiffness_matches_oracle_bitwise():
+    from oracle import oracle as orc
+    for nu, h in ((0.3, 1.0), (0.25, 0.5)):
+        assert np.array_equal(solver.FEA_VoxelBasedElementStiffnessMatrix(nu, h),
+                              orc.element_stiffness(nu, h))
+
+
+@pytest.mark.parametrize("shape", [(23, 24), (24, 25)])
+def test_kbyu_rejects_bad_ke(shape):
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree_mex(np.zeros(24), np.arange(1, 9, dtype=np.int32)[None, :],
+                                           np.zeros(shape), np.ones(1), [np.array([1], np.int32)])
+    assert solver.identifier(e.value) == "MATLAB:mexFunction:inputNot24x24"
+
+
+def test_kbyu_rejects_out_of_range_node_and_colour_ids():
+    eN = np.arange(1, 9, dtype=np.int32)[None, :]
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree_mex(np.zeros(21), eN, np.zeros((24, 24)), np.ones(1),
+                                           [np.array([1], np.int32)])
+    assert "eNodMat" in str(e.value)
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree_mex(np.zeros(24), eN, np.zeros((24, 24)), np.ones(1),
+                                           [np.array([2], np.int32)])
+    assert "colors" in str(e.value)
+
+
+def test_restrict_rejects_bad_transfer_shapes():
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_Restriction_MatrixFree_mex(np.zeros((8, 3)), np.ones((26, 1), np.int32),
+                                                  np.zeros((8, 26)), np.ones((1, 8), np.int32), 8, 27,
+                                                  np.ones(8, np.int32), np.ones(27), [np.array([1])])
+    assert solver.identifier(e.value) == "MATLAB:mexFunction:inputNotMx8" or \
+        solver.identifier(e.value) == "MATLAB:mexFunction:inputNotMatrix"
+
+
+def test_assemble_rejects_bad_children_count():
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_AssembleCmptStencilFromFinestLevel(np.zeros((24, 24)), np.ones(4),
+                                                          np.ones((1, 7), np.int32))
+    assert solver.identifier(e.value) == "MATLAB:mexFunction:inputNotMatrix"

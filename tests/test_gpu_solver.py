@@ -1,0 +1,137 @@
+"""Device-resident solver (lattice path) against the oracle: K.u and the
+V-cycle to FP64 rounding, PCG iteration counts within one, FP64 relative
+residual <= 1e-8 and compliance relative error <= 1e-9 (FP64) on cuboids,
+void-containing domains and the non-dyadic hierarchy; the mixed-precision
+path reaches the same FP64 true residual."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from sgldbench_amd import problems, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def holes(nx, ny, nz, seed, nholes=6, r=3):
+    """Cuboid with spherical voids (connected solid, x=0 face kept)."""
+    vox = problems.cuboid(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    yy, xx, zz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    for _ in range(nholes):
+        c = rng.uniform([r, 2 * r, r], [ny - r, nx - r, nz - r])
+        vox[(yy - c[0]) ** 2 + (xx - c[1]) ** 2 + (zz - c[2]) ** 2 < r * r] = 0
+    return vox
+
+
+def build_pair(vox, levels, nd=0, seed=5, density=True):
+    F, fixed = problems.cantilever3(vox)
+    ne = int(vox.sum())
+    if density:
+        x = problems.gaussian_random_density(vox.shape, seed, sigma=2.0).ravel(order="F")
+        E = problems.simp_modulus(x[vox.ravel(order="F") != 0])
+    else:
+        E = np.ones(ne)
+    g = solver.FEASolver(vox, levels, nd)
+    g.set_fixed(fixed)
+    g.set_modulus(E)
+    g.assemble(0.6)
+    m = orc.Mesh(vox, levels, nd)
+    m.set_fixed(fixed)
+    m.set_modulus(E)
+    m.assemble()
+    return g, m, F, fixed, E
+
+
+CASES = {
+    "cuboid32x16x16": (lambda: problems.cuboid(32, 16, 16), 4, 0),
+    "holes32x16x16": (lambda: holes(32, 16, 16, 7), 3, 0),
+    "nondyadic32x16x16": (lambda: problems.cuboid(32, 16, 16), 4, 1),
+}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def pair(request):
+    mk, lv, nd = CASES[request.param]
+    print("case:", request.param, flush=True)
+    return build_pair(mk(), lv, nd)
+
+
+def test_hierarchy_sizes_match(pair):
+    g, m, *_ = pair
+    assert g.numLevels == m.num_levels
+    for a, b in zip(g.levels, m.info):
+        assert (a["resX"], a["resY"], a["resZ"], a["numElements"], a["numNodes"]) == \
+            (b["resX"], b["resY"], b["resZ"], b["numElements"], b["numNodes"])
+
+
+def test_kbyu_matches_oracle(pair):
+    g, m, F, fixed, E = pair
+    u = np.random.default_rng(8).standard_normal(g.numDOFs)
+    y, ref = g.kbyu(u), m.kbyu(u)
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.all(y[fixed == 1] == 0)
+
+
+def test_vcycle_matches_oracle(pair):
+    g, m, F, fixed, E = pair
+    r = np.random.default_rng(9).standard_normal(g.numDOFs)
+    r[fixed == 1] = 0
+    z, ref = g.vcycle(r), m.vcycle(r, 0.6)
+    assert np.abs(z - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_pcg_fp64_matches_oracle(pair):
+    g, m, F, fixed, E = pair
+    U, its, rr = g.pcg(F, tol=1e-8, maxIT=500)
+    Uo, itso, hist, rc = m.pcg(F, 0.6, 1e-8, 500)
+    print("its gpu %d oracle %d relres %.3e" % (its, itso, rr), flush=True)
+    assert rc == 0 and g.converged
+    assert abs(its - itso) <= 1
+    assert rr <= 1e-8
+    c, co = F @ U, F @ Uo
+    assert abs(c - co) <= 1e-9 * abs(co)
+    ce = g.unit_compliance(U)
+    assert E @ ce == pytest.approx(c, rel=1e-9)
+    assert np.abs(ce - m.unit_compliance(U)).max() <= 1e-10 * np.abs(ce).max()
+
+
+def test_pcg_mixed_reaches_fp64_residual(pair):
+    g, m, F, fixed, E = pair
+    U, its, rr = g.pcg(F, tol=1e-8, maxIT=1000, precision=solver.PREC_MIXED)
+    Uo, itso, hist, rc = m.pcg(F, 0.6, 1e-10, 1000)
+    st = g.stats()
+    print("mixed its %d outer %d relres %.3e" % (its, st["outer"], rr), flush=True)
+    assert g.converged and rr <= 1e-8
+    c, co = F @ U, F @ Uo
+    assert abs(c - co) <= 1e-8 * abs(co)
+    assert np.linalg.norm(U - Uo) <= 1e-5 * np.linalg.norm(Uo)
+
+
+def test_warm_start_and_zero_load():
+    vox = problems.cuboid(16, 8, 8)
+    g, m, F, fixed, E = build_pair(vox, 3)
+    U, its, rr = g.pcg(F, tol=1e-8)
+    U2, its2, rr2 = g.pcg(F, tol=1e-8, U0=U)
+    assert its2 <= 2 and rr2 <= 1e-8
+    U3, its3, rr3 = g.pcg(np.zeros_like(F), tol=1e-8)
+    assert its3 == 0 and np.all(U3 == 0)
+
+
+def test_maxit_reports_without_raising():
+    vox = problems.cuboid(16, 8, 8)
+    g, m, F, fixed, E = build_pair(vox, 3)
+    U, its, rr = g.pcg(F, tol=1e-14, maxIT=3)
+    assert its == 3 and not g.converged
+    Uo, itso, hist, rc = m.pcg(F, 0.6, 1e-14, 3)
+    assert rc == 2 and itso == 3
+    assert np.linalg.norm(U - Uo) <= 1e-10 * np.linalg.norm(Uo)
+
+
+def test_cfg0_fp64_against_oracle():
+    p = problems.cfg0()
+    g, m, F, fixed, E = build_pair(p["voxels"], 4, density=False)
+    U, its, rr = g.pcg(p["F"], tol=1e-8, maxIT=500)
+    Uo, itso, hist, rc = m.pcg(p["F"], 0.6, 1e-8, 500)
+    print("cfg0 its gpu %d oracle %d relres %.3e c %.12e" % (its, itso, rr, p["F"] @ U), flush=True)
+    assert abs(its - itso) <= 1 and rr <= 1e-8
+    assert abs(p["F"] @ U - p["F"] @ Uo) <= 1e-9 * abs(p["F"] @ Uo)
