@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -15,10 +16,17 @@ namespace {
 
 __constant__ double cKe64[576];
 __constant__ float cKe32[576];
+// Mode-space element stiffness (see k_kbyu_tiled): [0,24) diagonal kd[a][m],
+// [24,60) off-diagonal ko[a][b][4] for the 6 ordered pairs a != b.
+__constant__ double cKm64[60];
+__constant__ float cKm32[60];
 
 template <class T> __device__ __forceinline__ T ke(int i);
 template <> __device__ __forceinline__ double ke<double>(int i) { return cKe64[i]; }
 template <> __device__ __forceinline__ float ke<float>(int i) { return cKe32[i]; }
+template <class T> __device__ __forceinline__ T km(int i);
+template <> __device__ __forceinline__ double km<double>(int i) { return cKm64[i]; }
+template <> __device__ __forceinline__ float km<float>(int i) { return cKm32[i]; }
 
 // Local node j of the corner (x, y, z) bits: node order of FEA_GaussianIntegral.m:2-9.
 __host__ __device__ constexpr int corner_index(int x, int y, int z) {
@@ -101,6 +109,174 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
             const T v = ((f & kSolid) && !(f & (1 << a))) ? acc[a] : T(0);
             y[3 * g + a] = v;
             if (partial) dotacc += (double)u[3 * g + a] * (double)v;
+        }
+    }
+    if (partial) {
+        const double s = block_sum(dotacc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+
+// ----------------------------------------------------------------------------
+// K.u, tiled mode-space form.  A block owns a 15 x 15 (row x col) column of
+// output nodes and marches through a chunk of node layers.  Per element layer
+// it stages the 17 x 17 node layer in LDS (ring of 3 layers), computes each of
+// the 16 x 16 elements once (one per thread) and accumulates the element
+// forces into two node-layer accumulators with LDS atomics; a node layer is
+// written to HBM once both element layers touching it are done.
+//
+// Element product in the per-component Walsh-Hadamard basis: with H the 8x8
+// +-1 transform over the corner bits (x, y, z), Ke = H^T K' H, and K' =
+// H Ke H^T / 64 has 45 nonzeros for the isotropic hexahedron (cube
+// reflection symmetry): the 21 non-rigid diagonal modes and, per component
+// pair (a, b), 4 couplings between modes e_a, e_b, e_a+e_c, e_b+e_c.  Cost per
+// element: 72 add (H), 45 mul/FMA (K'), 21 mul (E), 72 add (H^T), instead of
+// the 576 FMA of the dense product.  The host derives K' from the actual Ke
+// and uses this kernel only if Ke has that structure.
+// ----------------------------------------------------------------------------
+constexpr int kTE = 16;        // elements per tile edge
+constexpr int kTN = kTE + 1;   // staged nodes per tile edge
+constexpr int kTO = kTE - 1;   // output nodes per tile edge
+
+template <class T>
+__global__ void __launch_bounds__(256) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
+                                                    const T *__restrict__ E,
+                                                    const uint8_t *__restrict__ nflag, Grid gd,
+                                                    int cz, int tilesR, int tilesC,
+                                                    double *__restrict__ partial, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    __shared__ T us[3][kTN * kTN * 3];
+    __shared__ T acc[2][kTO * kTO * 3];
+    const int t = threadIdx.x;
+    int bid = blockIdx.x;
+    const int tr = bid % tilesR;
+    bid /= tilesR;
+    const int tc = bid % tilesC;
+    const int tz = bid / tilesC;
+    const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * cz;
+    const int64_t NY = gd.NY(), NX = gd.NX(), NZ = gd.NZ();
+    const int64_t pl = NY * NX;
+    const int Zend = (int)min((int64_t)Z0 + cz, NZ);
+    const int ei = t % kTE, ej = t / kTE;
+    const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
+    const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
+    for (int k = t; k < 2 * kTO * kTO * 3; k += 256) (&acc[0][0])[k] = T(0);
+    auto load_layer = [&](int l) {
+        T *dst = us[(l + 3) % 3];
+        for (int k = t; k < kTN * kTN; k += 256) {
+            const int li = k % kTN, lj = k / kTN;
+            const int64_t r = R0 - 1 + li, c = C0 - 1 + lj;
+            T v0 = 0, v1 = 0, v2 = 0;
+            if (l >= 0 && l < NZ && r >= 0 && r < NY && c >= 0 && c < NX) {
+                const int64_t g = r + NY * c + pl * l;
+                v0 = u[3 * g];
+                v1 = u[3 * g + 1];
+                v2 = u[3 * g + 2];
+            }
+            dst[3 * k] = v0;
+            dst[3 * k + 1] = v1;
+            dst[3 * k + 2] = v2;
+        }
+    };
+    load_layer(Z0 - 1);
+    double dotacc = 0.0;
+    for (int el = Z0 - 1; el < Zend; ++el) {
+        load_layer(el + 1);
+        T Ee = T(0);
+        if (einside && el >= 0 && el < gd.nz) Ee = E[er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el];
+        __syncthreads();
+        if (Ee != T(0)) {
+            const T *lo = us[(el + 3) % 3], *hi = us[(el + 4) % 3];
+            T uh[3][8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int xb = b & 1, yb = (b >> 1) & 1, zb = b >> 2;
+                const T *src = (zb ? hi : lo) + 3 * ((ei + 1 - yb) + kTN * (ej + xb));
+                uh[0][b] = src[0];
+                uh[1][b] = src[1];
+                uh[2][b] = src[2];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int d = 1; d < 8; d <<= 1)
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        if (!(b & d)) {
+                            const T l0 = uh[a][b], h0 = uh[a][b | d];
+                            uh[a][b] = l0 + h0;
+                            uh[a][b | d] = h0 - l0;
+                        }
+            T w[3][8];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                w[a][0] = T(0);
+#pragma unroll
+                for (int m = 1; m < 8; ++m) w[a][m] = km<T>(8 * a + m) * uh[a][m];
+            }
+            constexpr int eb[3] = {1, 2, 4};
+            int pidx = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    if (a == b) continue;
+                    const int c = 3 - a - b;
+                    const int o = 24 + 4 * pidx;
+                    w[a][eb[a]] += km<T>(o + 0) * uh[b][eb[b]];
+                    w[a][eb[b]] += km<T>(o + 1) * uh[b][eb[a]];
+                    w[a][eb[a] + eb[c]] += km<T>(o + 2) * uh[b][eb[b] + eb[c]];
+                    w[a][eb[b] + eb[c]] += km<T>(o + 3) * uh[b][eb[a] + eb[c]];
+                    ++pidx;
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int m = 1; m < 8; ++m) w[a][m] *= Ee;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int d = 1; d < 8; d <<= 1)
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        if (!(b & d)) {
+                            const T s0 = w[a][b], d0 = w[a][b | d];
+                            w[a][b] = s0 - d0;
+                            w[a][b | d] = s0 + d0;
+                        }
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int xb = b & 1, yb = (b >> 1) & 1, zb = b >> 2;
+                const int orow = ei - yb, ocol = ej + xb - 1;
+                if (orow < 0 || orow >= kTO || ocol < 0 || ocol >= kTO) continue;
+                T *dst = acc[(el + zb + 2) & 1] + 3 * (orow + kTO * ocol);
+                atomicAdd(dst, w[0][b]);
+                atomicAdd(dst + 1, w[1][b]);
+                atomicAdd(dst + 2, w[2][b]);
+            }
+        }
+        __syncthreads();
+        if (t < kTO * kTO) {
+            T *a3 = acc[(el + 2) & 1] + 3 * t;
+            if (el >= Z0) {
+                const int oi = t % kTO, oj = t / kTO;
+                const int64_t r = R0 + oi, c = C0 + oj;
+                if (r < NY && c < NX) {
+                    const int64_t g = r + NY * c + pl * el;
+                    const uint8_t f = nflag[g];
+                    const T *un = us[(el + 3) % 3] + 3 * ((oi + 1) + kTN * (oj + 1));
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const T v = ((f & kSolid) && !(f & (1 << a))) ? a3[a] : T(0);
+                        y[3 * g + a] = v;
+                        dotacc += (double)un[a] * (double)v;
+                    }
+                }
+            }
+            a3[0] = T(0);
+            a3[1] = T(0);
+            a3[2] = T(0);
         }
     }
     if (partial) {
@@ -559,13 +735,15 @@ struct sgld_ctx {
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool haveFixed = false, haveModulus = false, assembled = false;
+    int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
+    int tileCz = 16, tilesR = 1, tilesC = 1, tilesZ = 1;
     double stats[8] = {0};
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }
 };
 
 namespace {
 
-int grid_blocks(int64_t n) { return (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 255) / 256)); }
+int grid_blocks(int64_t n) { return (int)std::min<int64_t>(kVecBlocks, std::max<int64_t>(1, (n + 255) / 256)); }
 
 template <class T> T *lvl_r(Level &l);
 template <> double *lvl_r<double>(Level &l) { return l.r.p; }
@@ -585,12 +763,73 @@ double *slot(sgld_ctx *C, int k) { return C->partial.p + (int64_t)k * kRedBlocks
 template <class T>
 int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl, int *nbOut) {
     const Level &l0 = C->lv[0];
-    const int nb = grid_blocks(l0.g.nodes());
-    hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
-                       l0.g, partial, ctl);
+    int nb;
+    if (C->kbyuVariant == 1) {
+        nb = C->tilesR * C->tilesC * C->tilesZ;
+        hipLaunchKernelGGL(k_kbyu_tiled<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
+                           l0.nflag.p, l0.g, C->tileCz, C->tilesR, C->tilesC, partial, ctl);
+    } else {
+        nb = grid_blocks(l0.g.nodes());
+        hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
+                           l0.g, partial, ctl);
+    }
     SGLD_HIP(hipGetLastError());
     if (nbOut) *nbOut = nb;
     return SGLD_OK;
+}
+
+// Mode-space coefficients K' = H Ke H^T / 64 (see k_kbyu_tiled); returns false
+// when Ke lacks the isotropic-hexahedron sparsity (the dense kernel is used).
+bool mode_coefficients(const double *Ke, double *km) {
+    double H[8][8];
+    for (int m = 0; m < 8; ++m)
+        for (int j = 0; j < 8; ++j) {
+            const int c[3] = {kCornerX[j], kCornerY[j], kCornerZ[j]};
+            double f = 1;
+            for (int d = 0; d < 3; ++d)
+                if ((m >> d) & 1) f *= c[d] ? 1.0 : -1.0;
+            H[m][j] = f;
+        }
+    double Kp[24][24];
+    double maxabs = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int m = 0; m < 8; ++m)
+            for (int b = 0; b < 3; ++b)
+                for (int mp = 0; mp < 8; ++mp) {
+                    double acc = 0;
+                    for (int j = 0; j < 8; ++j)
+                        for (int k = 0; k < 8; ++k)
+                            acc += H[m][j] * Ke[(3 * j + a) + 24 * (3 * k + b)] * H[mp][k];
+                    Kp[8 * a + m][8 * b + mp] = acc / 64.0;
+                    maxabs = std::max(maxabs, std::fabs(acc / 64.0));
+                }
+    bool used[24][24] = {};
+    const int eb[3] = {1, 2, 4};
+    for (int a = 0; a < 3; ++a) {
+        km[8 * a] = 0.0;
+        used[8 * a][8 * a] = true;
+        for (int m = 1; m < 8; ++m) {
+            km[8 * a + m] = Kp[8 * a + m][8 * a + m];
+            used[8 * a + m][8 * a + m] = true;
+        }
+    }
+    int pidx = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            if (a == b) continue;
+            const int c = 3 - a - b;
+            const int pr[4][2] = {{eb[a], eb[b]}, {eb[b], eb[a]}, {eb[a] + eb[c], eb[b] + eb[c]},
+                                  {eb[b] + eb[c], eb[a] + eb[c]}};
+            for (int k = 0; k < 4; ++k) {
+                km[24 + 4 * pidx + k] = Kp[8 * a + pr[k][0]][8 * b + pr[k][1]];
+                used[8 * a + pr[k][0]][8 * b + pr[k][1]] = true;
+            }
+            ++pidx;
+        }
+    for (int i = 0; i < 24; ++i)
+        for (int j = 0; j < 24; ++j)
+            if (!used[i][j] && std::fabs(Kp[i][j]) > 1e-13 * maxabs) return false;
+    return true;
 }
 
 template <class T>
@@ -973,6 +1212,24 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     float k32[576];
     for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
     SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
+    double km[60];
+    C->kbyuVariant = mode_coefficients(C->Ke, km) ? 1 : 0;
+    if (const char *env = std::getenv("SGLD_KBYU")) {
+        if (!std::strcmp(env, "dense")) C->kbyuVariant = 0;
+    }
+    if (C->kbyuVariant == 1) {
+        float km32[60];
+        for (int i = 0; i < 60; ++i) km32[i] = (float)km[i];
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 60));
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 60));
+        const Grid &g0 = f0.g;
+        C->tilesR = (int)((g0.NY() + 14) / 15);
+        C->tilesC = (int)((g0.NX() + 14) / 15);
+        C->tileCz = 16;
+        while ((int64_t)C->tilesR * C->tilesC * ((g0.NZ() + C->tileCz - 1) / C->tileCz) > kRedBlocks)
+            C->tileCz *= 2;
+        C->tilesZ = (int)((g0.NZ() + C->tileCz - 1) / C->tileCz);
+    }
     SGLD_HIP(hipStreamSynchronize(C->s));
     *out = C.release();
     return SGLD_OK;

@@ -37,7 +37,8 @@ struct Ctl {
     int its, maxit, done, pad;
 };
 
-constexpr int kRedBlocks = 2048;  // grid size (and partial-sum slots) of reducing kernels
+constexpr int kRedBlocks = 8192;  // partial-sum slots of one reduction (max grid of a reducing kernel)
+constexpr int kVecBlocks = 2048;  // grid of the grid-stride vector kernels
 constexpr int kRedSlots = 4;      // independent reductions per solve
 
 }  // namespace sgld
