@@ -59,6 +59,15 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libsgldfea.so is not built (run __graft_entry__.build()); "
                                "there is no CPU fallback")
+        # torch ships its own libamdhip64.so.7.  Loading it first lets this
+        # library bind to that same HIP runtime (same soname), so a process
+        # that also uses torch tensors holds a single runtime; loading this
+        # library first would pull in a second copy from /opt/rocm and torch
+        # would then fail to initialise the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)

@@ -16,17 +16,16 @@ namespace {
 
 __constant__ double cKe64[576];
 __constant__ float cKe32[576];
-// Mode-space element stiffness (see k_kbyu_tiled): [0,24) diagonal kd[a][m],
-// [24,60) off-diagonal ko[a][b][4] for the 6 ordered pairs a != b.
-__constant__ double cKm64[60];
-__constant__ float cKm32[60];
+// Mode-space element stiffness of the isotropic cube (see mode_forces).
+__constant__ double cKm64[9];
+__constant__ float cKm32[9];
 
 template <class T> __device__ __forceinline__ T ke(int i);
 template <> __device__ __forceinline__ double ke<double>(int i) { return cKe64[i]; }
 template <> __device__ __forceinline__ float ke<float>(int i) { return cKe32[i]; }
-template <class T> __device__ __forceinline__ T km(int i);
-template <> __device__ __forceinline__ double km<double>(int i) { return cKm64[i]; }
-template <> __device__ __forceinline__ float km<float>(int i) { return cKm32[i]; }
+template <class T> __device__ __forceinline__ T km9(int i);
+template <> __device__ __forceinline__ double km9<double>(int i) { return cKm64[i]; }
+template <> __device__ __forceinline__ float km9<float>(int i) { return cKm32[i]; }
 
 // Local node j of the corner (x, y, z) bits: node order of FEA_GaussianIntegral.m:2-9.
 __host__ __device__ constexpr int corner_index(int x, int y, int z) {
@@ -118,165 +117,296 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
 }
 
 
+// Mode forces of one hexahedron: uh[a][m] holds the Hadamard modes
+// (m = x + 2y + 4z difference bits) of component a on entry, the force modes
+// E K' uh on exit (in place).
+template <class T>
+__device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
+    // K' is block diagonal: modes couple only within {(x,1),(y,2),(z,4)},
+    // {(x,6),(y,5),(z,3)}, the pairs {(a,e_b),(b,e_a)} and {(a,e_a+e_c),
+    // (b,e_b+e_c)}, and (a,7) alone; mode 0 (translation) carries no force.
+    // For the isotropic cube, axis permutations leave K' invariant, so nine
+    // numbers define it (km9: own-axis linear, cross-axis linear, bilinear
+    // with own axis, bilinear without, trilinear, then the four couplings).
+    const T dLo = km9<T>(0), dLx = km9<T>(1), dBo = km9<T>(2), dBx = km9<T>(3), dT = km9<T>(4);
+    const T c0 = km9<T>(5), c1 = km9<T>(6), c2 = km9<T>(7), c3 = km9<T>(8);
+    {
+        const T x = uh[0][1], yv = uh[1][2], z = uh[2][4];
+        uh[0][1] = dLo * x + c0 * (yv + z);
+        uh[1][2] = dLo * yv + c0 * (x + z);
+        uh[2][4] = dLo * z + c0 * (x + yv);
+    }
+    {
+        const T x = uh[0][6], yv = uh[1][5], z = uh[2][3];
+        uh[0][6] = dBx * x + c3 * (yv + z);
+        uh[1][5] = dBx * yv + c3 * (x + z);
+        uh[2][3] = dBx * z + c3 * (x + yv);
+    }
+    {
+        const T x = uh[0][2], yv = uh[1][1];
+        uh[0][2] = dLx * x + c1 * yv;
+        uh[1][1] = dLx * yv + c1 * x;
+    }
+    {
+        const T x = uh[0][4], z = uh[2][1];
+        uh[0][4] = dLx * x + c1 * z;
+        uh[2][1] = dLx * z + c1 * x;
+    }
+    {
+        const T yv = uh[1][4], z = uh[2][2];
+        uh[1][4] = dLx * yv + c1 * z;
+        uh[2][2] = dLx * z + c1 * yv;
+    }
+    {
+        const T x = uh[0][5], yv = uh[1][6];
+        uh[0][5] = dBo * x + c2 * yv;
+        uh[1][6] = dBo * yv + c2 * x;
+    }
+    {
+        const T x = uh[0][3], z = uh[2][6];
+        uh[0][3] = dBo * x + c2 * z;
+        uh[2][6] = dBo * z + c2 * x;
+    }
+    {
+        const T yv = uh[1][3], z = uh[2][5];
+        uh[1][3] = dBo * yv + c2 * z;
+        uh[2][5] = dBo * z + c2 * yv;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uh[a][7] *= dT;
+        uh[a][0] = T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int m = 1; m < 8; ++m) uh[a][m] *= Ee;
+}
+
 // ----------------------------------------------------------------------------
-// K.u, tiled mode-space form.  A block owns a 15 x 15 (row x col) column of
-// output nodes and marches through a chunk of node layers.  Per element layer
-// it stages the 17 x 17 node layer in LDS (ring of 3 layers), computes each of
-// the 16 x 16 elements once (one per thread) and accumulates the element
-// forces into two node-layer accumulators with LDS atomics; a node layer is
-// written to HBM once both element layers touching it are done.
+// K.u, tiled mode-space form ("element columns").
 //
-// Element product in the per-component Walsh-Hadamard basis: with H the 8x8
-// +-1 transform over the corner bits (x, y, z), Ke = H^T K' H, and K' =
-// H Ke H^T / 64 has 45 nonzeros for the isotropic hexahedron (cube
-// reflection symmetry): the 21 non-rigid diagonal modes and, per component
-// pair (a, b), 4 couplings between modes e_a, e_b, e_a+e_c, e_b+e_c.  Cost per
-// element: 72 add (H), 45 mul/FMA (K'), 21 mul (E), 72 add (H^T), instead of
-// the 576 FMA of the dense product.  The host derives K' from the actual Ke
-// and uses this kernel only if Ke has that structure.
+// A block owns a 15 x 15 (row x col) patch of output nodes and a chunk of
+// node layers; thread (ei, ej) owns the element column under one face of the
+// 16 x 16 element patch around it and marches up through the layers.  The
+// per-element Walsh-Hadamard transform is split separably: the (x, y) part is
+// taken once per face and node layer (F), the z part combines the faces
+// below and above an element; on the way back the z part splits the element
+// forces onto its two faces, the lower face completes (its upper-element part
+// is carried in registers from the previous layer), and only then is the
+// (x, y) part inverted and scattered to the face's four nodes, 12 LDS adds
+// per element into an FP64 node-layer accumulator.  With K' = H Ke H^T / 64
+// block diagonal (mode_forces), an element costs about 175 adds/multiplies
+// instead of the 576 FMA of the dense product.
+//
+// Persistent, XCD-aware grid: block b runs on XCD b % 8 and walks the work
+// items (patch x z-chunk) of that XCD's contiguous share, so patches sharing
+// halo nodes run behind the same L2.  Node layers are prefetched two
+// iterations ahead into registers and staged in a 3-slot LDS ring.
 // ----------------------------------------------------------------------------
-constexpr int kTE = 16;        // elements per tile edge
-constexpr int kTN = kTE + 1;   // staged nodes per tile edge
-constexpr int kTO = kTE - 1;   // output nodes per tile edge
+constexpr int kTE = 16;        // element columns per patch edge
+constexpr int kTN = kTE + 1;   // staged nodes per patch edge
+constexpr int kTO = kTE - 1;   // output nodes per patch edge
+constexpr int kXCD = 8;        // accelerator complex dies (one L2 each)
+
+struct TileMap {
+    int tilesR, tilesC, tilesZ, cz;
+    __host__ __device__ int64_t work() const { return (int64_t)tilesR * tilesC * tilesZ; }
+};
 
 template <class T>
-__global__ void __launch_bounds__(256) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
+__device__ __forceinline__ void face_modes(const T *__restrict__ lay, int ei, int ej, T (&F)[3][4]) {
+    // corners of face (ei, ej): (x0,y0) = node (ei+1, ej), (x1,y0) = (ei+1, ej+1),
+    // (x0,y1) = (ei, ej), (x1,y1) = (ei, ej+1)
+    const T *n00 = lay + 3 * ((ei + 1) + kTN * ej);
+    const T *n10 = lay + 3 * ((ei + 1) + kTN * (ej + 1));
+    const T *n01 = lay + 3 * (ei + kTN * ej);
+    const T *n11 = lay + 3 * (ei + kTN * (ej + 1));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T s0 = n00[a] + n10[a], d0 = n10[a] - n00[a];
+        const T s1 = n01[a] + n11[a], d1 = n11[a] - n01[a];
+        F[a][0] = s0 + s1;
+        F[a][1] = d0 + d1;
+        F[a][2] = s1 - s0;
+        F[a][3] = d1 - d0;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256, 3) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
                                                     const T *__restrict__ E,
                                                     const uint8_t *__restrict__ nflag, Grid gd,
-                                                    int cz, int tilesR, int tilesC,
-                                                    double *__restrict__ partial, const Ctl *ctl) {
+                                                    TileMap tm, double *__restrict__ partial,
+                                                    const Ctl *ctl) {
     if (stopped(ctl)) return;
     __shared__ T us[3][kTN * kTN * 3];
-    __shared__ T acc[2][kTO * kTO * 3];
+    __shared__ double acc[2][kTN * kTN * 3];
     const int t = threadIdx.x;
-    int bid = blockIdx.x;
-    const int tr = bid % tilesR;
-    bid /= tilesR;
-    const int tc = bid % tilesC;
-    const int tz = bid / tilesC;
-    const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * cz;
-    const int64_t NY = gd.NY(), NX = gd.NX(), NZ = gd.NZ();
-    const int64_t pl = NY * NX;
-    const int Zend = (int)min((int64_t)Z0 + cz, NZ);
+    const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
+    const int64_t pl = (int64_t)NY * NX;       // nodes per layer
+    const int64_t ple = (int64_t)gd.ny * gd.nx;  // elements per layer
     const int ei = t % kTE, ej = t / kTE;
-    const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
-    const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
-    for (int k = t; k < 2 * kTO * kTO * 3; k += 256) (&acc[0][0])[k] = T(0);
-    auto load_layer = [&](int l) {
-        T *dst = us[(l + 3) % 3];
-        for (int k = t; k < kTN * kTN; k += 256) {
-            const int li = k % kTN, lj = k / kTN;
-            const int64_t r = R0 - 1 + li, c = C0 - 1 + lj;
-            T v0 = 0, v1 = 0, v2 = 0;
-            if (l >= 0 && l < NZ && r >= 0 && r < NY && c >= 0 && c < NX) {
-                const int64_t g = r + NY * c + pl * l;
-                v0 = u[3 * g];
-                v1 = u[3 * g + 1];
-                v2 = u[3 * g + 2];
-            }
-            dst[3 * k] = v0;
-            dst[3 * k + 1] = v1;
-            dst[3 * k + 2] = v2;
-        }
-    };
-    load_layer(Z0 - 1);
+    const int64_t W = tm.work();
+    const int G = gridDim.x;
+    const int nxcd = G < kXCD ? G : kXCD;  // XCDs that hold at least one block
+    const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
+    const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
+    const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
+    for (int k = t; k < 2 * kTN * kTN * 3; k += 256) (&acc[0][0])[k] = 0.0;
+    const int k1 = t + 256;
+    const bool has1 = k1 < kTN * kTN;
+    const int oi = t % kTO, oj = t / kTO;
+    // LDS offsets (in T) of this face's corners and of this thread's output node
+    const int f00 = 3 * ((ei + 1) + kTN * ej), f10 = 3 * ((ei + 1) + kTN * (ej + 1));
+    const int f01 = 3 * (ei + kTN * ej), f11 = 3 * (ei + kTN * (ej + 1));
+    const int fo = 3 * ((oi + 1) + kTN * (oj + 1));
     double dotacc = 0.0;
-    for (int el = Z0 - 1; el < Zend; ++el) {
-        load_layer(el + 1);
-        T Ee = T(0);
-        if (einside && el >= 0 && el < gd.nz) Ee = E[er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el];
-        __syncthreads();
-        if (Ee != T(0)) {
-            const T *lo = us[(el + 3) % 3], *hi = us[(el + 4) % 3];
-            T uh[3][8];
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int xb = b & 1, yb = (b >> 1) & 1, zb = b >> 2;
-                const T *src = (zb ? hi : lo) + 3 * ((ei + 1 - yb) + kTN * (ej + xb));
-                uh[0][b] = src[0];
-                uh[1][b] = src[1];
-                uh[2][b] = src[2];
+    for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
+        int64_t rem = w;
+        const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
+        rem -= (int64_t)tz * tm.tilesR * tm.tilesC;
+        const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
+        const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * tm.cz;
+        const int Zend = min(Z0 + tm.cz, NZ);
+        const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
+        const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
+        // staged-node sources (k = t and k1), -1 when outside the lattice
+        int64_t g0 = -1, g1 = -1;
+        {
+            const int r0 = R0 - 1 + t % kTN, c0 = C0 - 1 + t / kTN;
+            if (r0 >= 0 && r0 < NY && c0 >= 0 && c0 < NX) g0 = r0 + (int64_t)NY * c0;
+            if (has1) {
+                const int r1 = R0 - 1 + k1 % kTN, c1 = C0 - 1 + k1 / kTN;
+                if (r1 >= 0 && r1 < NY && c1 >= 0 && c1 < NX) g1 = r1 + (int64_t)NY * c1;
             }
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int d = 1; d < 8; d <<= 1)
-#pragma unroll
-                    for (int b = 0; b < 8; ++b)
-                        if (!(b & d)) {
-                            const T l0 = uh[a][b], h0 = uh[a][b | d];
-                            uh[a][b] = l0 + h0;
-                            uh[a][b | d] = h0 - l0;
-                        }
-            T w[3][8];
+        }
+        const bool outNode = t < kTO * kTO && R0 + oi < NY && C0 + oj < NX;
+        // Pointers advanced one layer per iteration (no index arithmetic in the loop).
+        const T *pu0 = u + 3 * (g0 < 0 ? 0 : g0 + pl * (Z0 + 2));   // layer fetched next: el+3
+        const T *pu1 = u + 3 * (g1 < 0 ? 0 : g1 + pl * (Z0 + 2));
+        const T *pE = E + (einside ? er + (int64_t)gd.ny * ec + ple * Z0 : 0);  // layer el+1
+        const int64_t gOut = (R0 + oi) + (int64_t)NY * (C0 + oj);
+        const uint8_t *pF = nflag + (outNode ? gOut + pl * (Z0 - 1) : 0);      // layer el
+        T *py = y + (outNode ? 3 * (gOut + pl * (Z0 - 1)) : 0);
+        T pf[2][3];
+        auto load3 = [](const T *p, bool ok, T (&q)[3]) {
+            if (ok) { q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; }
+            else { q[0] = T(0); q[1] = T(0); q[2] = T(0); }
+        };
+        T *sPrev = us[0], *sCur = us[1], *sNext = us[2];  // layers el, el+1, el+2
+        double *aCur = acc[0], *aNext = acc[1];
+        {   // stage node layers Z0-1 and Z0; layer Z0+1 stays in flight in pf
+            T p0[2][3], p1[2][3];
+            const bool l0ok = Z0 - 1 >= 0, l2ok = Z0 + 1 < NZ;
+            load3(u + 3 * (g0 + pl * (Z0 - 1)), g0 >= 0 && l0ok, p0[0]);
+            load3(u + 3 * (g1 + pl * (Z0 - 1)), g1 >= 0 && l0ok, p0[1]);
+            load3(u + 3 * (g0 + pl * Z0), g0 >= 0, p1[0]);
+            load3(u + 3 * (g1 + pl * Z0), g1 >= 0, p1[1]);
+            load3(u + 3 * (g0 + pl * (Z0 + 1)), g0 >= 0 && l2ok, pf[0]);
+            load3(u + 3 * (g1 + pl * (Z0 + 1)), g1 >= 0 && l2ok, pf[1]);
+            __syncthreads();  // previous work item done with LDS
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                w[a][0] = T(0);
-#pragma unroll
-                for (int m = 1; m < 8; ++m) w[a][m] = km<T>(8 * a + m) * uh[a][m];
-            }
-            constexpr int eb[3] = {1, 2, 4};
-            int pidx = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    if (a == b) continue;
-                    const int c = 3 - a - b;
-                    const int o = 24 + 4 * pidx;
-                    w[a][eb[a]] += km<T>(o + 0) * uh[b][eb[b]];
-                    w[a][eb[b]] += km<T>(o + 1) * uh[b][eb[a]];
-                    w[a][eb[a] + eb[c]] += km<T>(o + 2) * uh[b][eb[b] + eb[c]];
-                    w[a][eb[b] + eb[c]] += km<T>(o + 3) * uh[b][eb[a] + eb[c]];
-                    ++pidx;
-                }
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int m = 1; m < 8; ++m) w[a][m] *= Ee;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int d = 1; d < 8; d <<= 1)
-#pragma unroll
-                    for (int b = 0; b < 8; ++b)
-                        if (!(b & d)) {
-                            const T s0 = w[a][b], d0 = w[a][b | d];
-                            w[a][b] = s0 - d0;
-                            w[a][b | d] = s0 + d0;
-                        }
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int xb = b & 1, yb = (b >> 1) & 1, zb = b >> 2;
-                const int orow = ei - yb, ocol = ej + xb - 1;
-                if (orow < 0 || orow >= kTO || ocol < 0 || ocol >= kTO) continue;
-                T *dst = acc[(el + zb + 2) & 1] + 3 * (orow + kTO * ocol);
-                atomicAdd(dst, w[0][b]);
-                atomicAdd(dst + 1, w[1][b]);
-                atomicAdd(dst + 2, w[2][b]);
+                sPrev[3 * t + a] = p0[0][a];
+                sCur[3 * t + a] = p1[0][a];
+                if (has1) { sPrev[3 * k1 + a] = p0[1][a]; sCur[3 * k1 + a] = p1[1][a]; }
             }
         }
         __syncthreads();
-        if (t < kTO * kTO) {
-            T *a3 = acc[(el + 2) & 1] + 3 * t;
-            if (el >= Z0) {
-                const int oi = t % kTO, oj = t / kTO;
-                const int64_t r = R0 + oi, c = C0 + oj;
-                if (r < NY && c < NX) {
-                    const int64_t g = r + NY * c + pl * el;
-                    const uint8_t f = nflag[g];
-                    const T *un = us[(el + 3) % 3] + 3 * ((oi + 1) + kTN * (oj + 1));
+        T Flo[3][4], Alo[3][4];
+        face_modes<T>(sPrev, ei, ej, Flo);
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const T v = ((f & kSolid) && !(f & (1 << a))) ? a3[a] : T(0);
-                        y[3 * g + a] = v;
-                        dotacc += (double)un[a] * (double)v;
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
+        T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
+        int lfetch = Z0 + 2;  // node layer of the next fetch
+        // One element layer.  On entry sPrev/sCur hold node layers el and el+1,
+        // `ready` holds layer el+2 (loaded during the previous step), Flo are
+        // the face modes of layer el and Alo the part of layer el's face
+        // forces from element layer el-1.  `spare` receives layer el+3 and
+        // stays in flight through the next step; the caller alternates the
+        // two buffers so no register copy waits on a load.
+        auto step = [&](int el, T (&ready)[2][3], T (&spare)[2][3]) {
+            const T Enext = (einside && el + 1 < gd.nz) ? *pE : T(0);
+            const uint8_t fcur = (outNode && el >= Z0) ? *pF : (uint8_t)0;
+            const bool lok = lfetch < NZ;
+            load3(pu0, g0 >= 0 && lok, spare[0]);
+            load3(pu1, g1 >= 0 && lok, spare[1]);
+            T m[3][8];
+            {
+                T Fhi[3][4];
+                face_modes<T>(sCur, ei, ej, Fhi);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        m[a][q] = Flo[a][q] + Fhi[a][q];
+                        m[a][q + 4] = Fhi[a][q] - Flo[a][q];
+                        Flo[a][q] = Fhi[a][q];
                     }
+            }
+            mode_forces<T>(m, Ecur);
+            if (el >= Z0) {
+                // lower face complete: inverse (x, y) transform, scatter 4 nodes
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T A0 = Alo[a][0] + (m[a][0] - m[a][4]), A1 = Alo[a][1] + (m[a][1] - m[a][5]);
+                    const T A2 = Alo[a][2] + (m[a][2] - m[a][6]), A3 = Alo[a][3] + (m[a][3] - m[a][7]);
+                    const T t0 = A0 - A2, t1 = A1 - A3, t2 = A0 + A2, t3 = A1 + A3;
+                    atomicAdd(aCur + f00 + a, (double)(t0 - t1));
+                    atomicAdd(aCur + f10 + a, (double)(t0 + t1));
+                    atomicAdd(aCur + f01 + a, (double)(t2 - t3));
+                    atomicAdd(aCur + f11 + a, (double)(t2 + t3));
                 }
             }
-            a3[0] = T(0);
-            a3[1] = T(0);
-            a3[2] = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Alo[a][q] = m[a][q] + m[a][q + 4];
+            Ecur = Enext;
+            __syncthreads();  // node layer el accumulated
+            if (el >= Z0 && outNode) {
+                const double *a3 = aCur + fo;
+                const T *un = sPrev + fo;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T v = ((fcur & kSolid) && !(fcur & (1 << a))) ? (T)a3[a] : T(0);
+                    py[a] = v;
+                    dotacc += (double)un[a] * (double)v;
+                }
+            }
+            // clear the other accumulator (read during the previous step,
+            // scattered into during the next); halo nodes get discarded sums
+            for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
+            // layer el+2 into the slot of layer el-1 (free since the barrier above)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sNext[3 * t + a] = ready[0][a];
+                if (has1) sNext[3 * k1 + a] = ready[1][a];
+            }
+            {
+                T *tmp = sPrev;
+                sPrev = sCur;
+                sCur = sNext;
+                sNext = tmp;
+                double *ta = aCur;
+                aCur = aNext;
+                aNext = ta;
+            }
+            pu0 += 3 * pl;
+            pu1 += 3 * pl;
+            pE += ple;
+            pF += pl;
+            py += 3 * pl;
+            ++lfetch;
+            __syncthreads();  // layer el+2 staged, accumulator clear
+        };
+        T pn[2][3];
+        for (int el = Z0 - 1; el < Zend; el += 2) {
+            step(el, pf, pn);
+            if (el + 1 < Zend) step(el + 1, pn, pf);
         }
     }
     if (partial) {
@@ -736,7 +866,8 @@ struct sgld_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
-    int tileCz = 16, tilesR = 1, tilesC = 1, tilesZ = 1;
+    TileMap tiles{1, 1, 1, 16};
+    int tiledGrid = 1;
     double stats[8] = {0};
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }
 };
@@ -765,9 +896,9 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     const Level &l0 = C->lv[0];
     int nb;
     if (C->kbyuVariant == 1) {
-        nb = C->tilesR * C->tilesC * C->tilesZ;
+        nb = C->tiledGrid;
         hipLaunchKernelGGL(k_kbyu_tiled<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
-                           l0.nflag.p, l0.g, C->tileCz, C->tilesR, C->tilesC, partial, ctl);
+                           l0.nflag.p, l0.g, C->tiles, partial, ctl);
     } else {
         nb = grid_blocks(l0.g.nodes());
         hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
@@ -778,9 +909,11 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     return SGLD_OK;
 }
 
-// Mode-space coefficients K' = H Ke H^T / 64 (see k_kbyu_tiled); returns false
-// when Ke lacks the isotropic-hexahedron sparsity (the dense kernel is used).
-bool mode_coefficients(const double *Ke, double *km) {
+// Mode-space coefficients K' = H Ke H^T / 64 (see mode_forces).  Returns
+// false, and the dense kernel is used, unless Ke has the block-diagonal mode
+// structure and the axis-permutation symmetry of an isotropic cube; km9
+// receives the nine defining values.
+bool mode_coefficients(const double *Ke, double *km9) {
     double H[8][8];
     for (int m = 0; m < 8; ++m)
         for (int j = 0; j < 8; ++j) {
@@ -790,7 +923,7 @@ bool mode_coefficients(const double *Ke, double *km) {
                 if ((m >> d) & 1) f *= c[d] ? 1.0 : -1.0;
             H[m][j] = f;
         }
-    double Kp[24][24];
+    static double Kp[24][24];
     double maxabs = 0;
     for (int a = 0; a < 3; ++a)
         for (int m = 0; m < 8; ++m)
@@ -803,29 +936,35 @@ bool mode_coefficients(const double *Ke, double *km) {
                     Kp[8 * a + m][8 * b + mp] = acc / 64.0;
                     maxabs = std::max(maxabs, std::fabs(acc / 64.0));
                 }
-    bool used[24][24] = {};
     const int eb[3] = {1, 2, 4};
+    // classes: (row, col) entries that must share one value
+    std::vector<std::vector<std::pair<int, int>>> cls(9);
+    bool used[24][24] = {};
     for (int a = 0; a < 3; ++a) {
-        km[8 * a] = 0.0;
-        used[8 * a][8 * a] = true;
-        for (int m = 1; m < 8; ++m) {
-            km[8 * a + m] = Kp[8 * a + m][8 * a + m];
-            used[8 * a + m][8 * a + m] = true;
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        auto D = [&](int k, int m) { cls[k].push_back({8 * a + m, 8 * a + m}); };
+        D(0, eb[a]);
+        D(1, eb[b]); D(1, eb[c]);
+        D(2, eb[a] + eb[b]); D(2, eb[a] + eb[c]);
+        D(3, eb[b] + eb[c]);
+        D(4, 7);
+        for (int bb = 0; bb < 3; ++bb) {
+            if (bb == a) continue;
+            const int cc = 3 - a - bb;
+            cls[5].push_back({8 * a + eb[a], 8 * bb + eb[bb]});
+            cls[6].push_back({8 * a + eb[bb], 8 * bb + eb[a]});
+            cls[7].push_back({8 * a + eb[a] + eb[cc], 8 * bb + eb[bb] + eb[cc]});
+            cls[8].push_back({8 * a + eb[bb] + eb[cc], 8 * bb + eb[a] + eb[cc]});
         }
     }
-    int pidx = 0;
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            if (a == b) continue;
-            const int c = 3 - a - b;
-            const int pr[4][2] = {{eb[a], eb[b]}, {eb[b], eb[a]}, {eb[a] + eb[c], eb[b] + eb[c]},
-                                  {eb[b] + eb[c], eb[a] + eb[c]}};
-            for (int k = 0; k < 4; ++k) {
-                km[24 + 4 * pidx + k] = Kp[8 * a + pr[k][0]][8 * b + pr[k][1]];
-                used[8 * a + pr[k][0]][8 * b + pr[k][1]] = true;
-            }
-            ++pidx;
+    for (int k = 0; k < 9; ++k) {
+        const double v = Kp[cls[k][0].first][cls[k][0].second];
+        km9[k] = v;
+        for (auto &e : cls[k]) {
+            if (std::fabs(Kp[e.first][e.second] - v) > 1e-13 * maxabs) return false;
+            used[e.first][e.second] = true;
         }
+    }
     for (int i = 0; i < 24; ++i)
         for (int j = 0; j < 24; ++j)
             if (!used[i][j] && std::fabs(Kp[i][j]) > 1e-13 * maxabs) return false;
@@ -1212,23 +1351,30 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     float k32[576];
     for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
     SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
-    double km[60];
+    double km[9];
     C->kbyuVariant = mode_coefficients(C->Ke, km) ? 1 : 0;
     if (const char *env = std::getenv("SGLD_KBYU")) {
         if (!std::strcmp(env, "dense")) C->kbyuVariant = 0;
     }
     if (C->kbyuVariant == 1) {
-        float km32[60];
-        for (int i = 0; i < 60; ++i) km32[i] = (float)km[i];
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 60));
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 60));
+        float km32[9];
+        for (int i = 0; i < 9; ++i) km32[i] = (float)km[i];
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 9));
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 9));
         const Grid &g0 = f0.g;
-        C->tilesR = (int)((g0.NY() + 14) / 15);
-        C->tilesC = (int)((g0.NX() + 14) / 15);
-        C->tileCz = 16;
-        while ((int64_t)C->tilesR * C->tilesC * ((g0.NZ() + C->tileCz - 1) / C->tileCz) > kRedBlocks)
-            C->tileCz *= 2;
-        C->tilesZ = (int)((g0.NZ() + C->tileCz - 1) / C->tileCz);
+        TileMap &tm = C->tiles;
+        tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
+        tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
+        tm.cz = 16;
+        tm.tilesZ = (int)((g0.NZ() + tm.cz - 1) / tm.cz);
+        int dev = 0, ncu = 256, per = 4;
+        SGLD_HIP(hipGetDevice(&dev));
+        SGLD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        int p64 = 0, p32 = 0;
+        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p64, k_kbyu_tiled<double>, 256, 0));
+        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p32, k_kbyu_tiled<float>, 256, 0));
+        per = std::max(1, std::min(p64, p32));
+        C->tiledGrid = (int)std::min<int64_t>(std::min<int64_t>(tm.work(), (int64_t)ncu * per), kRedBlocks);
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
     *out = C.release();

@@ -239,6 +239,11 @@ class FEASolver:
         check(lib().sgld_ctx_unit_compliance(self._h, U, ce))
         return ce
 
+    def kbyu_device(self, du_ptr, dy_ptr, precision=PREC_FP64):
+        """K.u on device vectors (compressed layout); FP32 arithmetic for
+        PREC_MIXED (inputs/outputs stay FP64)."""
+        check(lib().sgld_ctx_kbyu_device(self._h, C.c_void_p(du_ptr), C.c_void_p(dy_ptr), precision))
+
     def bench_kbyu(self, precision=PREC_FP64, reps=20):
         ms = C.c_double(0)
         check(lib().sgld_ctx_bench_kbyu(self._h, precision, reps, C.byref(ms)))

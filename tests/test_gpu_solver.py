@@ -135,3 +135,56 @@ def test_cfg0_fp64_against_oracle():
     print("cfg0 its gpu %d oracle %d relres %.3e c %.12e" % (its, itso, rr, p["F"] @ U), flush=True)
     assert abs(its - itso) <= 1 and rr <= 1e-8
     assert abs(p["F"] @ U - p["F"] @ Uo) <= 1e-9 * abs(p["F"] @ Uo)
+
+
+@pytest.mark.parametrize("variant", ["dense", "tiled"])
+@pytest.mark.parametrize("dims", [(38, 22, 18), (16, 8, 8)])
+def test_kbyu_kernel_variants(variant, dims, monkeypatch):
+    """Both K.u kernels (node-centric dense Ke; tiled mode-space) against the
+    oracle, on lattices that are not multiples of the 15-node tile, with voids
+    and moduli spanning 1e-9..1."""
+    if variant == "dense":
+        monkeypatch.setenv("SGLD_KBYU", "dense")
+    else:
+        monkeypatch.delenv("SGLD_KBYU", raising=False)
+    nx, ny, nz = dims
+    vox = holes(nx, ny, nz, 11, nholes=4, r=3)
+    F, fixed = problems.cantilever3(vox)
+    rng = np.random.default_rng(12)
+    E = 10.0 ** rng.uniform(-9, 0, int(vox.sum()))
+    print("case: kbyu", variant, dims, flush=True)
+    g = solver.FEASolver(vox, 2)
+    g.set_fixed(fixed)
+    g.set_modulus(E)
+    g.assemble(0.6)
+    m = orc.Mesh(vox, 2)
+    m.set_fixed(fixed)
+    m.set_modulus(E)
+    u = rng.standard_normal(g.numDOFs)
+    y, ref = g.kbyu(u), m.kbyu(u)
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_kbyu_fp32_against_fp64():
+    import torch
+    vox = holes(38, 22, 18, 13, nholes=4, r=3)
+    F, fixed = problems.cantilever3(vox)
+    rng = np.random.default_rng(14)
+    E = rng.uniform(0.01, 1.0, int(vox.sum()))
+    g = solver.FEASolver(vox, 2)
+    g.set_fixed(fixed)
+    g.set_modulus(E)
+    u = rng.standard_normal(g.numDOFs)
+    du = torch.from_numpy(u).cuda()
+    y64 = torch.zeros_like(du)
+    y32 = torch.zeros_like(du)
+    g.kbyu_device(du.data_ptr(), y64.data_ptr(), solver.PREC_FP64)
+    g.kbyu_device(du.data_ptr(), y32.data_ptr(), solver.PREC_MIXED)
+    torch.cuda.synchronize()
+    a, b = y64.cpu().numpy(), y32.cpu().numpy()
+    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
+    m = orc.Mesh(vox, 2)
+    m.set_fixed(fixed)
+    m.set_modulus(E)
+    ref = m.kbyu(u)
+    assert np.abs(a - ref).max() <= 1e-13 * np.abs(ref).max()
