@@ -135,7 +135,7 @@ def main():
                  "fp32_GBs": bytes32 / ms32 / 1e6},
         "roofline": {"bound": "hbm", "achieved": bytes64 / ms64 / 1e6, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_kbyu<double>", "bytes_per_launch": bytes64},
+                     "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=a.cpu_its)
