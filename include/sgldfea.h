@@ -125,8 +125,10 @@ int sgld_ctx_vcycle(sgld_ctx *ctx, const double *r, double *z);
 /* [U, its] = Solving_PreconditionedConjugateGradientSolver(@KbyU, @Vcycle, F, tol,
  * maxIT, printP, U) (src/Solving_PreconditionedConjugateGradientSolver.m:1-37),
  * as Solving_CG_GMGS.m:7 calls it.  U: in x0, out solution.  precision:
- * SGLD_PREC_FP64 (the reference algorithm) or SGLD_PREC_MIXED (FP32 inner PCG,
- * V-cycle and K.u with FP64 iterative refinement on the true residual).
+ * SGLD_PREC_FP64 (the reference algorithm) or SGLD_PREC_MIXED (PCG in FP32 -
+ * vectors, K.u, V-cycle - with FP64 dot products and coarsest inverse, and
+ * reliable updates that fold the correction into the FP64 solution and
+ * replace the residual by the FP64 true residual; stops on the latter).
  * relres receives the final FP64 true relative residual ||F - K U|| / ||F||.
  * Returns SGLD_ERR_MAXIT (with U filled) when maxIT is reached. */
 int sgld_ctx_pcg(sgld_ctx *ctx, const double *F, double *U, double tol, int maxIT,
@@ -146,7 +148,8 @@ int sgld_ctx_bench_kbyu(sgld_ctx *ctx, int precision, int reps, double *msPerLau
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
  * last assemble, [5] K.u launches, [6] average K.u kernel milliseconds
- * (hipEvent, FP64), [7] average K.u kernel milliseconds (FP32). */
+ * (hipEvent, FP64), [7] average K.u kernel milliseconds (FP32).  For
+ * SGLD_PREC_MIXED [3] counts reliable updates. */
 int sgld_ctx_stats(const sgld_ctx *ctx, double *out);
 
 #ifdef __cplusplus

@@ -176,7 +176,10 @@ class FEASolver:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().sgld_ctx_destroy(self._h)
+            try:
+                lib().sgld_ctx_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def __del__(self):
