@@ -1423,9 +1423,11 @@ int solve_mixed(sgld_ctx *C, double tol, int maxit, int *itsTot, double *resOut,
         C->hctl->rho = st.rho * f * f;
         C->hctl->tol = inner_tol(scale);
         SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-        // z = M r and beta for the replaced residual, then continue from p
+        // z = M r and beta for the replaced residual, then continue from p.
+        // After a forced update (FP32 stalled mid-stretch) rho already belongs
+        // to the recursive residual, so the direction restarts (beta = 0).
         SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), C->ctl.p, &nb));
-        SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO));
+        SGLD_TRY(fin(C, slot(C, 2), nb, st.done ? FIN_RHO : FIN_RHO0));
     }
     *itsTot = its;
     *resOut = res;
