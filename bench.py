@@ -30,7 +30,38 @@ def parse():
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kbyu-reps", type=int, default=50)
+    ap.add_argument("--kbyu-only", action="store_true",
+                    help="only K.u launches plus a 256 MiB torch copy (for rocprofv3 --pmc runs)")
     return ap.parse_args()
+
+
+TRAFFIC_MD = os.path.join(ROOT, "profiles", "kbyu_traffic_cuboid256x128x128.md")
+
+
+def measured_traffic():
+    """HBM bytes per FP64 K.u launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    try:
+        for line in open(TRAFFIC_MD):
+            if line.startswith("traffic_bytes_per_launch_fp64:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def kbyu_only(a, s, n, ne):
+    import torch
+    from sgldbench_amd import solver
+    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
+    ms32 = s.bench_kbyu(solver.PREC_MIXED, a.kbyu_reps)
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kbyu_only": True, "fp64_ms": ms64, "fp32_ms": ms32, "reps": a.kbyu_reps,
+                      "copy_bytes": src.numel() * 8, "dofs": n, "elements": ne}), flush=True)
 
 
 def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
@@ -75,6 +106,9 @@ def main():
     s = solver.FEASolver(p["voxels"], p["num_levels"])
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
+    if a.kbyu_only:
+        kbyu_only(a, s, n, ne)
+        return
     t0 = time.time()
     s.assemble(p["omega"])
     setup_s = time.time() - t0
@@ -134,7 +168,7 @@ def main():
                  "fp32_gdofs": n / ms32 / 1e6, "fp64_GBs": bytes64 / ms64 / 1e6,
                  "fp32_GBs": bytes32 / ms32 / 1e6},
         "roofline": {"bound": "hbm", "achieved": bytes64 / ms64 / 1e6, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64},
     }
     if not a.no_cpu:
