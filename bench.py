@@ -25,6 +25,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "cfg2"],
+                    help="cfg1: 256x128x128 GRF cuboid (default); cfg2: ~125 M-DOF ellipsoid union in 512^3")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
@@ -88,6 +90,10 @@ def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
             "solve_s_est": t_solve}
 
 
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -97,12 +103,18 @@ def main():
     import torch
     from sgldbench_amd import problems, solver
 
-    nx, ny, nz = (int(v) for v in a.size.split("x"))
-    p = problems.cfg1(nx, ny, nz, a.levels)
+    if a.workload == "cfg2":
+        p = problems.cfg2()
+        ny, nx, nz = p["voxels"].shape
+        a.no_cpu = True  # the oracle's per-element Galerkin storage does not fit this size on the host
+    else:
+        nx, ny, nz = (int(v) for v in a.size.split("x"))
+        p = problems.cfg1(nx, ny, nz, a.levels)
     n = p["F"].size
     ne = int(p["E"].size)
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
+    progress("problem %s: %d DOF, %d elements" % (a.workload, n, ne))
     s = solver.FEASolver(p["voxels"], p["num_levels"])
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
@@ -112,6 +124,7 @@ def main():
     t0 = time.time()
     s.assemble(p["omega"])
     setup_s = time.time() - t0
+    progress("assembled in %.2f s" % setup_s)
     dF = torch.from_numpy(p["F"]).to(dev)
     dU = torch.zeros(n, dtype=torch.float64, device=dev)
 
@@ -119,7 +132,7 @@ def main():
         dU.zero_()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], 2000, prec)
+        its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], 5000, prec)
         torch.cuda.synchronize()
         return time.perf_counter() - t, its, rr, s.stats()
 
@@ -129,7 +142,10 @@ def main():
             solve(prec)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        runs = [solve(prec) for _ in range(a.steps)]
+        runs = []
+        for _ in range(a.steps):
+            runs.append(solve(prec))
+            progress("%s solve: %.3f s, %d its, rel-res %.2e" % (name, runs[-1][0], runs[-1][1], runs[-1][2]))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         res[name] = dict(ms_per_step=1e3 * wall / a.steps, its=runs[-1][1], relres=max(r[2] for r in runs),
@@ -155,8 +171,11 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if best == "mixed" else "f64",
         "data": "synthetic (seeded Gaussian random density field)",
-        "config": {"workload": "cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, "
-                               "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]),
+        "config": {"workload": ("cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, "
+                                "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]))
+                   if a.workload == "cfg1" else
+                   ("ellipsoid union in %dx%dx%d box (fill %.3f), %d DOF, E=1, z<8 fixed, 2000 unit loads, "
+                    "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, float(p["voxels"].mean()), n, p["num_levels"])),
                    "path": best, "dofs": n, "elements": ne},
         "solve": {"fp64": res["fp64"], "mixed": res["mixed"],
                   "speedup_mixed_vs_fp64": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],

@@ -94,5 +94,11 @@ void child_prolongation(int span, int q, double p[8][8]);
 void galerkin_child_add(const double p[8][8], const double *K, double *out);
 
 inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
+// Grid for a grid-stride kernel: the dispatch packet holds the grid size in
+// work-items as 32 bits, so one-item-per-thread launches must stay below 2^32.
+inline int bounded_blocks(int64_t n, int t) {
+    const int64_t b = (n + t - 1) / t, cap = (int64_t)1 << 20;
+    return (int)(b < cap ? b : cap);
+}
 
 }  // namespace sgld

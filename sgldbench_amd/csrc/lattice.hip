@@ -834,20 +834,22 @@ __global__ void k_diagk(const double *__restrict__ Ks, Grid gd, const uint8_t *_
 // H_q = P_q^T Ke P_q (Solving_AssembleCmptStencilFromFinestLevel.cpp:71-169).
 __global__ void k_galerkin_from_E(const double *__restrict__ H, int s, const double *__restrict__ E,
                                   Grid gf, Grid gc, double *__restrict__ Ks) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= gc.elems() * 576) return;
-    const int64_t e = idx / 576;
-    const int t = (int)(idx % 576);
-    const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
-    double acc = 0.0;
+    const int64_t total = gc.elems() * 576;
     const int nq = s * s * s;
-    for (int q = 0; q < nq; ++q) {
-        const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
-        const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
-                          (int64_t)gf.ny * gf.nx * (s * el + ql);
-        acc += E[v] * H[576 * q + t];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / 576;
+        const int t = (int)(idx % 576);
+        const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
+        double acc = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
+            const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
+                              (int64_t)gf.ny * gf.nx * (s * el + ql);
+            acc += E[v] * H[576 * q + t];
+        }
+        Ks[idx] = acc;
     }
-    Ks[idx] = acc;
 }
 
 // Level >= 3: Ks_e = sum_q P_q^T Ks_prev(child q) P_q
@@ -858,41 +860,42 @@ __global__ void __launch_bounds__(576) k_galerkin_from_Ks(const double *__restri
                                                           const double *__restrict__ P,
                                                           double *__restrict__ Ks) {
     __shared__ double sK[576], sT[576], sP[64];
-    const int64_t e = blockIdx.x;
     const int t = threadIdx.x;
     const int row = t % 24, col = t / 24;
-    const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
-    double out = 0.0;
     const int nq = s * s * s;
-    for (int q = 0; q < nq; ++q) {
-        const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
-        const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
-                          (int64_t)gf.ny * gf.nx * (s * el + ql);
-        __syncthreads();
-        sK[t] = Kf[576 * v + t];
-        if (t < 64) sP[t] = P[64 * q + t];
-        __syncthreads();
-        {
-            const int J = col / 3, a = col % 3;
-            double acc = 0.0;
-            for (int j = 0; j < 8; ++j) {
-                const double w = sP[j * 8 + J];
-                if (w != 0.0) acc += w * sK[row + 24 * (3 * j + a)];
+    for (int64_t e = blockIdx.x; e < gc.elems(); e += gridDim.x) {
+        const int er = (int)(e % gc.ny), ec = (int)((e / gc.ny) % gc.nx), el = (int)(e / ((int64_t)gc.ny * gc.nx));
+        double out = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const int qr = q % s, qc = (q / s) % s, ql = q / (s * s);
+            const int64_t v = (int64_t)(s * er + qr) + (int64_t)gf.ny * (s * ec + qc) +
+                              (int64_t)gf.ny * gf.nx * (s * el + ql);
+            __syncthreads();
+            sK[t] = Kf[576 * v + t];
+            if (t < 64) sP[t] = P[64 * q + t];
+            __syncthreads();
+            {
+                const int J = col / 3, a = col % 3;
+                double acc = 0.0;
+                for (int j = 0; j < 8; ++j) {
+                    const double w = sP[j * 8 + J];
+                    if (w != 0.0) acc += w * sK[row + 24 * (3 * j + a)];
+                }
+                sT[t] = acc;
             }
-            sT[t] = acc;
-        }
-        __syncthreads();
-        {
-            const int I = row / 3, a = row % 3;
-            double acc = 0.0;
-            for (int i = 0; i < 8; ++i) {
-                const double w = sP[i * 8 + I];
-                if (w != 0.0) acc += w * sT[3 * i + a + 24 * col];
+            __syncthreads();
+            {
+                const int I = row / 3, a = row % 3;
+                double acc = 0.0;
+                for (int i = 0; i < 8; ++i) {
+                    const double w = sP[i * 8 + I];
+                    if (w != 0.0) acc += w * sT[3 * i + a + 24 * col];
+                }
+                out += acc;
             }
-            out += acc;
         }
+        Ks[576 * e + t] = out;
     }
-    Ks[576 * e + t] = out;
 }
 
 // Compressed <-> lattice.
@@ -1243,8 +1246,9 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Dense Cholesky (lower, col-major, left-looking) in place; 0 on success.
-int host_cholesky(std::vector<double> &A, int64_t n) {
+// Dense Cholesky (lower, col-major, left-looking) in place; 0 on success,
+// else 1 + the failing pivot.
+int64_t host_cholesky(std::vector<double> &A, int64_t n) {
     for (int64_t j = 0; j < n; ++j) {
         double *Aj = A.data() + n * j;
         for (int64_t k = 0; k < j; ++k) {
@@ -1254,7 +1258,7 @@ int host_cholesky(std::vector<double> &A, int64_t n) {
             for (int64_t i = j; i < n; ++i) Aj[i] -= Ak[i] * ljk;
         }
         const double d = Aj[j];
-        if (!(d > 0.0)) return 1;
+        if (!(d > 0.0)) return 1 + j;
         const double sd = std::sqrt(d), inv = 1.0 / sd;
         Aj[j] = sd;
         for (int64_t i = j + 1; i < n; ++i) Aj[i] *= inv;
@@ -1677,7 +1681,7 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
             }
             DevBuf<double> dH;
             SGLD_TRY(dH.upload(H.data(), H.size()));
-            hipLaunchKernelGGL(k_galerkin_from_E, dim3(blocks_for(576 * lc.g.elems(), 256)), dim3(256), 0,
+            hipLaunchKernelGGL(k_galerkin_from_E, dim3(bounded_blocks(576 * lc.g.elems(), 256)), dim3(256), 0,
                                C->s, dH.p, s, C->E.p, lf.g, lc.g, KsCur.p);
             SGLD_HIP(hipGetLastError());
             SGLD_HIP(hipStreamSynchronize(C->s));
@@ -1690,8 +1694,8 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
             }
             DevBuf<double> dP;
             SGLD_TRY(dP.upload(P.data(), P.size()));
-            hipLaunchKernelGGL(k_galerkin_from_Ks, dim3((unsigned)lc.g.elems()), dim3(576), 0, C->s,
-                               KsPrev.p, lf.g, lc.g, s, dP.p, KsCur.p);
+            hipLaunchKernelGGL(k_galerkin_from_Ks, dim3((unsigned)std::min<int64_t>(lc.g.elems(), 1 << 20)),
+                               dim3(576), 0, C->s, KsPrev.p, lf.g, lc.g, s, dP.p, KsCur.p);
             SGLD_HIP(hipGetLastError());
             SGLD_HIP(hipStreamSynchronize(C->s));
         }
@@ -1734,8 +1738,20 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
                 if (dof[r] >= 0) A[dof[r] + nf * dof[c]] += Ks[576 * e + r + 24 * c];
         }
     }
-    if (host_cholesky(A, nf)) {
-        set_error("sgld:notspd: coarsest-level matrix is not positive definite (free DOFs %lld)", (long long)nf);
+    double asym = 0, amax = 0;
+    for (int64_t j = 0; j < nf; ++j)
+        for (int64_t i = 0; i < nf; ++i) {
+            asym = std::max(asym, std::fabs(A[i + nf * j] - A[j + nf * i]));
+            amax = std::max(amax, std::fabs(A[i + nf * j]));
+        }
+    std::vector<double> Adiag(nf);
+    for (int64_t j = 0; j < nf; ++j) Adiag[j] = A[j + nf * j];
+    if (const int64_t bad = host_cholesky(A, nf)) {
+        const int64_t dof = freeIdx[bad - 1], g = dof / 3;
+        set_error("sgld:notspd: coarsest-level matrix is not positive definite (free DOFs %lld; pivot %lld "
+                  "= lattice node (%lld,%lld,%lld) comp %lld, diag %.3e, max |a| %.3e, max asym %.3e)",
+                  (long long)nf, (long long)(bad - 1), (long long)(g % NY), (long long)((g / NY) % NX),
+                  (long long)(g / (NY * NX)), (long long)(dof % 3), Adiag[bad - 1], amax, asym);
         return SGLD_ERR_NOT_SPD;
     }
     std::vector<double> Ainv;

@@ -162,17 +162,18 @@ __global__ void k_assemble_finest(const double *__restrict__ H, const double *__
                                   const int32_t *__restrict__ upMap, int64_t nC, int nq,
                                   double *__restrict__ Ks) {
 #pragma clang fp contract(off)
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nC * 576) return;
-    const int64_t i = idx / 576;
-    const int t = (int)(idx % 576);
-    double acc = 0.0;
-    for (int q = 0; q < nq; ++q) {
-        const int32_t child = upMap[i + q * nC];
-        if (child == 0) continue;
-        acc += E[child - 1] * H[576 * q + t];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nC * 576;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx / 576;
+        const int t = (int)(idx % 576);
+        double acc = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const int32_t child = upMap[i + q * nC];
+            if (child == 0) continue;
+            acc += E[child - 1] * H[576 * q + t];
+        }
+        Ks[idx] = acc;
     }
-    Ks[idx] = acc;
 }
 
 // Solving_AssembleCmptStencilFromNonFinestLevel.cpp:78-143 as
@@ -186,39 +187,40 @@ __global__ void __launch_bounds__(576) k_assemble_nonfinest(const double *__rest
                                                             double *__restrict__ Ks) {
 #pragma clang fp contract(off)
     __shared__ double sK[576], sT[576], sP[64];
-    const int64_t i = blockIdx.x;
     const int t = threadIdx.x;
     const int row = t % 24, col = t / 24;
-    double out = 0.0;
-    for (int q = 0; q < nq; ++q) {
-        const int32_t child = upMap[i + q * nC];
-        if (child <= 0) continue;
-        __syncthreads();
-        sK[t] = KsPrev[576 * (int64_t)(child - 1) + t];
-        if (t < 64) sP[t] = P[64 * q + t];  // p[j][J] at j*8 + J
-        __syncthreads();
-        {   // T[:, 3J+a] = sum_j p[j][J] K[:, 3j+a]
-            const int J = col / 3, a = col % 3;
-            double v = 0.0;
-            for (int j = 0; j < 8; ++j) {
-                const double w = sP[j * 8 + J];
-                if (w == 0.0) continue;
-                v += w * sK[row + 24 * (3 * j + a)];
+    for (int64_t i = blockIdx.x; i < nC; i += gridDim.x) {
+        double out = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const int32_t child = upMap[i + q * nC];
+            if (child <= 0) continue;
+            __syncthreads();
+            sK[t] = KsPrev[576 * (int64_t)(child - 1) + t];
+            if (t < 64) sP[t] = P[64 * q + t];  // p[j][J] at j*8 + J
+            __syncthreads();
+            {   // T[:, 3J+a] = sum_j p[j][J] K[:, 3j+a]
+                const int J = col / 3, a = col % 3;
+                double v = 0.0;
+                for (int j = 0; j < 8; ++j) {
+                    const double w = sP[j * 8 + J];
+                    if (w == 0.0) continue;
+                    v += w * sK[row + 24 * (3 * j + a)];
+                }
+                sT[t] = v;
             }
-            sT[t] = v;
-        }
-        __syncthreads();
-        {   // out[3I+a, c] += sum_i p[i][I] T[3i+a, c]
-            const int I = row / 3, a = row % 3;
-            double acc = 0.0;
-            for (int ii = 0; ii < 8; ++ii) {
-                const double w = sP[ii * 8 + I];
-                if (w != 0.0) acc += w * sT[3 * ii + a + 24 * col];
+            __syncthreads();
+            {   // out[3I+a, c] += sum_i p[i][I] T[3i+a, c]
+                const int I = row / 3, a = row % 3;
+                double acc = 0.0;
+                for (int ii = 0; ii < 8; ++ii) {
+                    const double w = sP[ii * 8 + I];
+                    if (w != 0.0) acc += w * sT[3 * ii + a + 24 * col];
+                }
+                out += acc;
             }
-            out += acc;
         }
+        Ks[576 * i + t] = out;
     }
-    Ks[576 * i + t] = out;
 }
 
 // --- unit compliance (TopOpti_CmptUnitCompliance_mex.cpp:40-107) -----------
@@ -458,7 +460,7 @@ extern "C" int sgld_assemble_finest(const double *Ke, int64_t keRows, int64_t ke
     SGLD_TRY(dE.upload(E, std::max<int64_t>(numE, 1)));
     SGLD_TRY(dU.upload(upMap, nq * nC));
     SGLD_TRY(dKs.alloc(576 * nC));
-    hipLaunchKernelGGL(k_assemble_finest, dim3(sgld::blocks_for(576 * nC, 256)), dim3(256), 0, 0,
+    hipLaunchKernelGGL(k_assemble_finest, dim3(sgld::bounded_blocks(576 * nC, 256)), dim3(256), 0, 0,
                        dH.p, dE.p, dU.p, nC, (int)nq, dKs.p);
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpy(Ks, dKs.p, sizeof(double) * 576 * nC, hipMemcpyDeviceToHost));
@@ -487,7 +489,8 @@ extern "C" int sgld_assemble_nonfinest(const double *KsPrev, int64_t numPrev,
     SGLD_TRY(dP.upload(P.data(), P.size()));
     SGLD_TRY(dU.upload(upMap, nq * nC));
     SGLD_TRY(dKs.alloc(576 * nC));
-    hipLaunchKernelGGL(k_assemble_nonfinest, dim3((unsigned)nC), dim3(576), 0, 0, dKp.p, dU.p, nC,
+    hipLaunchKernelGGL(k_assemble_nonfinest, dim3((unsigned)std::min<int64_t>(nC, 1 << 20)), dim3(576), 0, 0,
+                       dKp.p, dU.p, nC,
                        (int)nq, dP.p, dKs.p);
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpy(Ks, dKs.p, sizeof(double) * 576 * nC, hipMemcpyDeviceToHost));

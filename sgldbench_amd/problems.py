@@ -100,3 +100,79 @@ def cfg1(nx=256, ny=128, nz=128, levels=6, seed=CFG1_SEED):
     E = simp_modulus(random_density_cuboid(nx, ny, nz, seed))
     return dict(name="cuboid%dx%dx%d-grf" % (nx, ny, nz), voxels=vox, E=E, F=F, fixed=fixed,
                 num_levels=levels, omega=0.6, tol=1e-8, max_it=500)
+
+
+CFG2_SEED = 20277  # fill close to the ~30 % the workload describes (seed scan at 1/4 scale)
+
+
+def ellipsoid_union(size, n_ell, axes_range, seed, keep_below=8):
+    """Union of seeded axis-aligned ellipsoids (centres uniform in the box,
+    semi-axes uniform in axes_range) in a size^3 voxel box, restricted to the
+    face-connected components that contain a voxel with layer < keep_below
+    (a floating component would make K singular).  Returns (ny, nx, nz) uint8."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    vox = np.zeros((size, size, size), dtype=bool)
+    c = rng.uniform(0, size, (n_ell, 3))
+    ax = rng.uniform(axes_range[0], axes_range[1], (n_ell, 3))
+    for (cy, cx, cz), (ay, bx, gz) in zip(c, ax):
+        lo = np.maximum(np.floor([cy - ay, cx - bx, cz - gz]).astype(int), 0)
+        hi = np.minimum(np.ceil([cy + ay, cx + bx, cz + gz]).astype(int) + 1, size)
+        yy = (np.arange(lo[0], hi[0]) + 0.5 - cy) / ay
+        xx = (np.arange(lo[1], hi[1]) + 0.5 - cx) / bx
+        zz = (np.arange(lo[2], hi[2]) + 0.5 - cz) / gz
+        m = (yy[:, None, None] ** 2 + xx[None, :, None] ** 2 + zz[None, None, :] ** 2) <= 1.0
+        vox[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] |= m
+    lab, n = ndimage.label(vox)
+    keep = np.unique(lab[:, :, :keep_below])
+    keep = keep[keep > 0]
+    return np.isin(lab, keep).astype(np.uint8)
+
+
+def surface_nodes(voxels):
+    """Positions (in the solid-node list) of nodes with fewer than 8 incident
+    solid voxels, as FEA_VoxelBasedDiscretization.m:92-98 marks boundary nodes."""
+    vox = np.asarray(voxels) != 0
+    ny, nx, nz = vox.shape
+    cnt = np.zeros((ny + 1, nx + 1, nz + 1), dtype=np.uint8)
+    for dr in (0, 1):
+        for dc in (0, 1):
+            for dl in (0, 1):
+                cnt[dr:dr + ny, dc:dc + nx, dl:dl + nz] += vox
+    flat = cnt.ravel(order="F")
+    solid = np.flatnonzero(flat)
+    return np.flatnonzero(flat[solid] < 8), solid
+
+
+def cfg2(size=512, n_ell=24, axes=(40, 160), levels=7, n_loads=2000, z_fix=8, z_load=400,
+         seed=CFG2_SEED):
+    """Ellipsoid-union domain: nodes of active voxels with layer < z_fix fixed
+    in x/y/z; n_loads seeded surface nodes above z_load carry unit forces in
+    seeded random directions; E = 1."""
+    vox = ellipsoid_union(size, n_ell, axes, seed, keep_below=z_fix)
+    ny, nx, nz = vox.shape
+    surf, solid = surface_nodes(vox)
+    lay = solid // ((ny + 1) * (nx + 1))
+    nn = len(solid)
+    # fixed: every node of an active voxel with layer < z_fix
+    fixed_node = np.zeros((ny + 1, nx + 1, nz + 1), dtype=bool)
+    low = vox[:, :, :z_fix] != 0
+    for dr in (0, 1):
+        for dc in (0, 1):
+            for dl in (0, 1):
+                fixed_node[dr:dr + ny, dc:dc + nx, dl:dl + z_fix] |= low
+    fixed_n = fixed_node.ravel(order="F")[solid]
+    rng = np.random.default_rng(seed + 1)
+    cand = surf[(lay[surf] > z_load) & ~fixed_n[surf]]
+    pick = rng.choice(cand, size=min(n_loads, len(cand)), replace=False)
+    d = rng.standard_normal((len(pick), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    F = np.zeros(3 * nn)
+    for a in range(3):
+        F[3 * pick + a] = d[:, a]
+    fixed = np.zeros(3 * nn, dtype=np.uint8)
+    for a in range(3):
+        fixed[3 * np.flatnonzero(fixed_n) + a] = 1
+    E = np.ones(int(vox.sum()))
+    return dict(name="ellipsoids%d^3" % size, voxels=vox, E=E, F=F, fixed=fixed, num_levels=levels,
+                omega=0.6, tol=1e-8, max_it=1000)

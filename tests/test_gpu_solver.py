@@ -23,8 +23,8 @@ def holes(nx, ny, nz, seed, nholes=6, r=3):
     return vox
 
 
-def build_pair(vox, levels, nd=0, seed=5, density=True):
-    F, fixed = problems.cantilever3(vox)
+def build_pair(vox, levels, nd=0, seed=5, density=True, bc=None):
+    F, fixed = problems.cantilever3(vox) if bc is None else bc
     ne = int(vox.sum())
     if density:
         x = problems.gaussian_random_density(vox.shape, seed, sigma=2.0).ravel(order="F")
@@ -44,6 +44,7 @@ def build_pair(vox, levels, nd=0, seed=5, density=True):
 
 CASES = {
     "cuboid32x16x16": (lambda: problems.cuboid(32, 16, 16), 4, 0),
+    "ellipsoids32": (lambda: problems.cfg2(32, 12, (4, 10), 3, 20, 2, 16, seed=7), 3, 0),
     "holes32x16x16": (lambda: holes(32, 16, 16, 7), 3, 0),
     "nondyadic32x16x16": (lambda: problems.cuboid(32, 16, 16), 4, 1),
 }
@@ -53,7 +54,10 @@ CASES = {
 def pair(request):
     mk, lv, nd = CASES[request.param]
     print("case:", request.param, flush=True)
-    return build_pair(mk(), lv, nd)
+    v = mk()
+    if isinstance(v, dict):  # irregular domain with its own supports and loads
+        return build_pair(v["voxels"], lv, nd, bc=(v["F"], v["fixed"]))
+    return build_pair(v, lv, nd)
 
 
 def test_hierarchy_sizes_match(pair):
