@@ -25,8 +25,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "cfg2"],
-                    help="cfg1: 256x128x128 GRF cuboid (default); cfg2: ~125 M-DOF ellipsoid union in 512^3")
+    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "cfg2", "part_r256"],
+                    help="cfg1: 256x128x128 GRF cuboid (default); cfg2: ~125 M-DOF ellipsoid union in 512^3; "
+                         "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
@@ -107,6 +108,12 @@ def main():
         p = problems.cfg2()
         ny, nx, nz = p["voxels"].shape
         a.no_cpu = True  # the oracle's per-element Galerkin storage does not fit this size on the host
+    elif a.workload == "part_r256":
+        from sgldbench_amd import topvoxel
+        here = os.path.dirname(os.path.abspath(__file__))
+        p = topvoxel.setup(topvoxel.load_packed(os.path.join(here, "tests", "golden", "part_R256.npz")),
+                           num_levels=6)
+        ny, nx, nz = p["voxels"].shape
     else:
         nx, ny, nz = (int(v) for v in a.size.split("x"))
         p = problems.cfg1(nx, ny, nz, a.levels)
@@ -170,10 +177,15 @@ def main():
         "ms_per_step": res[best]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if best == "mixed" else "f64",
-        "data": "synthetic (seeded Gaussian random density field)",
+        "data": ("the reference's voxel model data/part_R256.TopVoxel (packed copy)" if a.workload == "part_r256"
+                 else "synthetic (seeded Gaussian random density field)" if a.workload == "cfg1"
+                 else "synthetic (seeded ellipsoid union)"),
         "config": {"workload": ("cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, "
                                 "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]))
                    if a.workload == "cfg1" else
+                   ("voxel model part_R256 (256x160x112, padded %dx%dx%d), %d DOF, E=1, its 13042 fixations "
+                    "and 7788 loads, %d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]))
+                   if a.workload == "part_r256" else
                    ("ellipsoid union in %dx%dx%d box (fill %.3f), %d DOF, E=1, z<8 fixed, 2000 unit loads, "
                     "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, float(p["voxels"].mean()), n, p["num_levels"])),
                    "path": best, "dofs": n, "elements": ne},

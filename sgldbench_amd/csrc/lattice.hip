@@ -898,6 +898,32 @@ __global__ void __launch_bounds__(576) k_galerkin_from_Ks(const double *__restri
     }
 }
 
+// Coarsest-level SPD inverse by Gauss-Jordan elimination without pivoting
+// (the matrix is SPD, so every pivot is positive), one elimination step per
+// launch, ping-pong between two n x n buffers so no entry is overwritten while
+// other threads still read it.  Step k: B = A - a_col a_row / a_kk off the
+// pivot row/col, B_kj = a_kj / a_kk, B_ik = -a_ik / a_kk, B_kk = 1 / a_kk.
+// A non-positive pivot sets *bad (to 1 + k; first one wins).
+__global__ void k_gauss_jordan_step(const double *__restrict__ A, double *__restrict__ B, int64_t n,
+                                    int64_t k, int64_t *bad) {
+    const double piv = A[k * n + k];
+    if (!(piv > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS((unsigned long long *)bad, 0ull,
+                                                           (unsigned long long)(k + 1));
+    }
+    const double inv = 1.0 / piv;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx / n, j = idx % n;
+        double v;
+        if (i == k && j == k) v = inv;
+        else if (i == k) v = A[idx] * inv;
+        else if (j == k) v = -A[idx] * inv;
+        else v = A[idx] - A[i * n + k] * A[k * n + j] * inv;
+        B[idx] = v;
+    }
+}
+
 // Compressed <-> lattice.
 template <class T>
 __global__ void k_scatter_nodes(const T *__restrict__ src, const int32_t *__restrict__ map,
@@ -1244,53 +1270,6 @@ int run_pcg(sgld_ctx *C, const T *b, T *x, T *r, T *z, T *p0, T *p1, T *q, bool 
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-// Dense Cholesky (lower, col-major, left-looking) in place; 0 on success,
-// else 1 + the failing pivot.
-int64_t host_cholesky(std::vector<double> &A, int64_t n) {
-    for (int64_t j = 0; j < n; ++j) {
-        double *Aj = A.data() + n * j;
-        for (int64_t k = 0; k < j; ++k) {
-            const double *Ak = A.data() + n * k;
-            const double ljk = Ak[j];
-            if (ljk == 0.0) continue;
-            for (int64_t i = j; i < n; ++i) Aj[i] -= Ak[i] * ljk;
-        }
-        const double d = Aj[j];
-        if (!(d > 0.0)) return 1 + j;
-        const double sd = std::sqrt(d), inv = 1.0 / sd;
-        Aj[j] = sd;
-        for (int64_t i = j + 1; i < n; ++i) Aj[i] *= inv;
-    }
-    return 0;
-}
-
-// Inverse of SPD A from its Cholesky factor L: A^-1 = W^T W with W = L^-1.
-void host_chol_inverse(const std::vector<double> &Lf, int64_t n, std::vector<double> &Ainv) {
-    std::vector<double> W((size_t)n * n, 0.0);  // lower, col-major
-    for (int64_t c = 0; c < n; ++c) {  // column c of L^-1: L w = e_c
-        double *w = W.data() + n * c;
-        w[c] = 1.0;
-        for (int64_t k = c; k < n; ++k) {
-            const double wk = w[k] / Lf[k + n * k];
-            w[k] = wk;
-            if (wk == 0.0) continue;
-            const double *Lk = Lf.data() + n * k;
-            for (int64_t i = k + 1; i < n; ++i) w[i] -= Lk[i] * wk;
-        }
-    }
-    Ainv.assign((size_t)n * n, 0.0);
-    for (int64_t j = 0; j < n; ++j) {
-        const double *wj = W.data() + n * j;
-        for (int64_t i = 0; i <= j; ++i) {
-            const double *wi = W.data() + n * i;
-            double acc = 0.0;
-            for (int64_t k = j; k < n; ++k) acc += wi[k] * wj[k];
-            Ainv[i + n * j] = acc;
-            Ainv[j + n * i] = acc;
-        }
-    }
 }
 
 int upload_flags(Level &l) {
@@ -1740,24 +1719,32 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
                 if (dof[r] >= 0) A[dof[r] + nf * dof[c]] += Ks[576 * e + r + 24 * c];
         }
     }
-    double asym = 0, amax = 0;
-    for (int64_t j = 0; j < nf; ++j)
-        for (int64_t i = 0; i < nf; ++i) {
-            asym = std::max(asym, std::fabs(A[i + nf * j] - A[j + nf * i]));
-            amax = std::max(amax, std::fabs(A[i + nf * j]));
-        }
-    std::vector<double> Adiag(nf);
-    for (int64_t j = 0; j < nf; ++j) Adiag[j] = A[j + nf * j];
-    if (const int64_t bad = host_cholesky(A, nf)) {
+    // invert on the device (row-major = col-major: A is symmetric)
+    DevBuf<double> dA, dB;
+    DevBuf<int64_t> dBad;
+    SGLD_TRY(dA.upload(A.data(), A.size()));
+    SGLD_TRY(dB.alloc(A.size()));
+    SGLD_TRY(dBad.alloc(1));
+    SGLD_TRY(dBad.zero(C->s));
+    for (int64_t k = 0; k < nf; ++k) {
+        hipLaunchKernelGGL(k_gauss_jordan_step, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s,
+                           dA.p, dB.p, nf, k, dBad.p);
+        std::swap(dA.p, dB.p);
+    }
+    SGLD_HIP(hipGetLastError());
+    int64_t bad = 0;
+    SGLD_HIP(hipMemcpyAsync(&bad, dBad.p, sizeof(int64_t), hipMemcpyDeviceToHost, C->s));
+    std::vector<double> Ainv(A.size());
+    SGLD_HIP(hipMemcpyAsync(Ainv.data(), dA.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    if (bad) {
         const int64_t dof = freeIdx[bad - 1], g = dof / 3;
         set_error("sgld:notspd: coarsest-level matrix is not positive definite (free DOFs %lld; pivot %lld "
-                  "= lattice node (%lld,%lld,%lld) comp %lld, diag %.3e, max |a| %.3e, max asym %.3e)",
+                  "= lattice node (%lld,%lld,%lld) comp %lld, diag %.3e)",
                   (long long)nf, (long long)(bad - 1), (long long)(g % NY), (long long)((g / NY) % NX),
-                  (long long)(g / (NY * NX)), (long long)(dof % 3), Adiag[bad - 1], amax, asym);
+                  (long long)(g / (NY * NX)), (long long)(dof % 3), A[(bad - 1) * (nf + 1)]);
         return SGLD_ERR_NOT_SPD;
     }
-    std::vector<double> Ainv;
-    host_chol_inverse(A, nf, Ainv);
     std::vector<double> M((size_t)nd * nd, 0.0);
     for (int64_t j = 0; j < nf; ++j)
         for (int64_t i = 0; i < nf; ++i) M[freeIdx[i] * nd + freeIdx[j]] = Ainv[i + nf * j];
