@@ -199,7 +199,9 @@ def main():
                  "fp32_gdofs": n / ms32 / 1e6, "fp64_GBs": bytes64 / ms64 / 1e6,
                  "fp32_GBs": bytes32 / ms32 / 1e6},
         "roofline": {"bound": "hbm", "achieved": bytes64 / ms64 / 1e6, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if (a.workload == "cfg1" and (nx, ny, nz) == (256, 128, 128)
+                                                       and p["num_levels"] == 6) else None,
                      "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64},
     }
     if not a.no_cpu:
