@@ -145,6 +145,32 @@ int sgld_ctx_kbyu_device(sgld_ctx *ctx, const double *du, double *dy, int precis
  * (hipEvents on the context stream); msPerLaunch = average kernel time. */
 int sgld_ctx_bench_kbyu(sgld_ctx *ctx, int precision, int reps, double *msPerLaunch);
 
+/* Batched solve of nrhs <= 16 load cases on one mesh hierarchy.  No
+ * counterpart in the reference, which solves load cases one call of
+ * Solving_CG_GMGS.m at a time; each case follows sgld_ctx_pcg's contract.
+ * F, U: numDOF x nrhs, case-major (case j at F + j numDOF); U in: initial
+ * guesses, out: solutions.  its[j], relres[j]: per-case iterations and FP64
+ * true relative residual.  precision SGLD_PREC_MIXED: all cases iterate
+ * together (16 cases interleaved per DOF, K.u on the BF16 matrix cores as
+ * hi/lo split products with FP32 accumulation, FP32 V-cycle, FP64 per-case
+ * scalars and reliable updates); SGLD_PREC_FP64: nrhs sequential FP64 solves.
+ * Returns SGLD_ERR_MAXIT (solutions filled) if any case misses tol. */
+int sgld_ctx_pcg_batched(sgld_ctx *ctx, int nrhs, const double *F, double *U, double tol,
+                         int maxIT, int precision, int *its, double *relres);
+/* Same with F and U in device memory. */
+int sgld_ctx_pcg_batched_device(sgld_ctx *ctx, int nrhs, const double *dF, double *dU,
+                                double tol, int maxIT, int precision, int *its,
+                                double *relres);
+/* Y = K U for nrhs <= 16 device vectors (compressed, case-major, FP64 in and
+ * out) through the batched matrix-core K.u (FP32 vectors; per element the
+ * displacements relative to its first node, scaled by E, enter the BF16
+ * matrix cores as hi/lo pairs with FP32 accumulation; fixed DOFs 0).  No
+ * counterpart in the reference. */
+int sgld_ctx_kbyu_batched_device(sgld_ctx *ctx, int nrhs, const double *dU, double *dY);
+/* Times `reps` launches of the batched (16-case) K.u kernel; msPerLaunch =
+ * average kernel time (hipEvents on the context stream). */
+int sgld_ctx_bench_kbyu_batched(sgld_ctx *ctx, int reps, double *msPerLaunch);
+
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
  * last assemble, [5] K.u launches, [6] average K.u kernel milliseconds

@@ -46,6 +46,10 @@ SIGNATURES = {
     "sgld_ctx_unit_compliance": [_vp, _dp, _dp],
     "sgld_ctx_kbyu_device": [_vp, _vp, _vp, C.c_int],
     "sgld_ctx_bench_kbyu": [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    "sgld_ctx_pcg_batched": [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp],
+    "sgld_ctx_pcg_batched_device": [_vp, C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int, _ip, _dp],
+    "sgld_ctx_kbyu_batched_device": [_vp, C.c_int, _vp, _vp],
+    "sgld_ctx_bench_kbyu_batched": [_vp, C.c_int, C.POINTER(C.c_double)],
     "sgld_ctx_stats": [_vp, _dp],
 }
 _RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}

@@ -1,0 +1,808 @@
+// Batched multi-RHS MG-PCG: up to 16 load cases on one mesh hierarchy, solved
+// together in mixed precision.  The reference solves load cases one at a time
+// (Solving_CG_GMGS.m per F_); here the Krylov vectors of all cases share one
+// lattice array with the 16 cases interleaved per DOF, so every pass over the
+// mesh (K.u, transfers, smoothing, updates) serves all of them.
+//
+// Layout of a batched vector: v[(3 g + a) * 16 + j] (FP32) for lattice node
+// g, component a, case j.  K.u runs on v_mfma_f32_32x32x16_bf16 with BF16
+// hi/lo split operands (x = hi + lo, hi = bf16(x), lo = bf16(x - hi): 16
+// significant bits) and FP32 accumulation:
+//   q_e = Ke_hi d_hi + Ke_hi d_lo + Ke_lo d_hi,  d = E_e (u_e - u_e,node0)
+// Subtracting the element's node-0 displacement (a rigid translation, in the
+// null space of Ke) before the split leaves only strain-sized differences to
+// round, so smooth directions - the usual case in a multigrid-preconditioned
+// CG - do not lose their K.u to cancellation against the displacement
+// magnitude (without it the 16-bit operands cost 3x more iterations).
+//
+// Precision: as the single-RHS mixed solver (lattice.hip), the iteration runs
+// on FP32 residuals scaled to unit norm, with FP64 dot products and
+// alpha/beta per case, and reliable updates: when a case's recursive residual
+// has dropped by delta, its FP32 correction is folded into its FP64 solution,
+// its FP64 true residual f - K u (FP64 K.u) replaces the recursive one, and p
+// and rho are rescaled so the direction carries on.  Each case stops on its
+// own FP64 true residual.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "lattice_ctx.hpp"
+
+namespace sgld {
+namespace {
+
+constexpr int NR = 16;  // cases (right-hand sides) per batched vector
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Device-side per-case PCG state.
+struct BCtl {
+    double rho[NR], pq[NR], rr[NR], alpha[NR], beta[NR], res[NR];
+    int active[NR], restart[NR], its[NR], pad[NR];
+};
+
+// BF16 hi/lo split of 8 values into MFMA operand vectors.
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const __bf16 h = (__bf16)v[i];
+        hi[i] = h;
+        lo[i] = (__bf16)(v[i] - (float)h);
+    }
+}
+
+// Sum v over the threads of a block that share threadIdx.x % 16 (the case
+// lane of grid-stride loops over batched vectors); thread j < 16 writes
+// out[j].
+__device__ __forceinline__ void block_sum16(double v, double *__restrict__ out) {
+    __shared__ double sh[16][NR];
+    v += __shfl_down(v, 32, 64);
+    v += __shfl_down(v, 16, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < NR) sh[w][lane] = v;
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double s = 0.0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) s += sh[i][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// q = K p for 16 cases on the BF16 matrix cores.
+//
+// A block owns a patch of 15 x 15 output node columns and a chunk of cz node
+// layers (the same patch/chunk decomposition and XCD-aware persistent walk as
+// the single-RHS kernel) and marches the element layers of the chunk.  For an
+// element layer the 16 x 16 element columns that touch the patch are 128
+// MFMA column pairs (2 elements x 16 cases = 32 MFMA columns); each of the 8
+// wavefronts computes 16 pairs:
+//   rows i = 8 a + n  (component a, element-local node n; 24..31 zero)
+//   B[k][col] = E_e (p - p_node0) at element-local DOF k of the column's
+//               element and case, split hi/lo,
+//   D = Ke_c B, 32 x 32 FP32, lane l holds rows 8 r/4 + 4 (l/32) + r%4:
+//   lanes 0-31 the 4 nodes of the lower face, lanes 32-63 the upper face.
+// The face forces are scaled by E_e and added with LDS atomics into a ring
+// of two node-layer accumulators covering the 17 x 17 patch nodes; a layer
+// is complete once the element layers below and above it are done, then its
+// 15 x 15 owned nodes are masked (fixed / void DOFs 0, as
+// Solving_KbyU_MatrixFree.m:39), stored, dotted with p for the PCG step
+// length, and the slot is cleared.
+// ----------------------------------------------------------------------------
+constexpr int kBT = 512;                 // threads per block
+constexpr int kBS = kTN * kTN * 3 * NR;  // floats per accumulator layer
+
+__global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, float *__restrict__ q,
+                                                  const float *__restrict__ E,
+                                                  const uint8_t *__restrict__ nflag, Grid gd, TileMap tm,
+                                                  const float *__restrict__ Kc, double *__restrict__ partial) {
+    __shared__ float acc[2][kBS];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int g = lane >> 5;          // 0: lower-face rows / x,z inputs; 1: upper face / y inputs
+    const int col = lane & 31;        // MFMA column: element (col / 16) of the pair, case col % 16
+    const int j = col & 15, e = col >> 4;
+    const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
+    const int64_t pl = (int64_t)NY * NX;
+    const int64_t ple = (int64_t)gd.ny * gd.nx;
+
+    // A operands: Ke_c rows i = lane % 32, columns 16 s + 8 g + 0..7, as hi/lo.
+    bf16x8 Ahi[2], Alo[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kk = 16 * s + 8 * g + i;
+            const float a = (col < 24 && kk < 24) ? Kc[col * 24 + kk] : 0.0f;
+            const __bf16 hi = (__bf16)a;
+            Ahi[s][i] = hi;
+            Alo[s][i] = (__bf16)(a - (float)hi);
+        }
+    for (int i = t; i < 2 * kBS; i += kBT) (&acc[0][0])[i] = 0.0f;
+    __syncthreads();
+
+    const int64_t W = tm.work();
+    const int G = gridDim.x;
+    const int nxcd = G < kXCD ? G : kXCD;
+    const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
+    const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
+    const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
+    double dotacc = 0.0;  // case (t % 16) of this thread in the store phase
+    // element-local node n -> patch-node offset (row + 17 col) from the
+    // element's (row, col) corner, and lattice-node offset within a layer
+    const int nodeOffP[4] = {1, 1 + kTN, kTN, 0};
+    for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
+        int64_t rem = w;
+        const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
+        rem -= (int64_t)tz * tm.tilesR * tm.tilesC;
+        const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
+        const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * tm.cz;
+        const int Zend = min(Z0 + tm.cz, NZ);
+        for (int el = Z0 - 1; el < Zend; ++el) {
+            const bool layerOk = el >= 0 && el < gd.nz;
+            if (layerOk) {
+#pragma unroll 2
+                for (int pp = 0; pp < 16; ++pp) {
+                    const int pair = wave * 16 + pp;
+                    const int ei = 2 * (pair & 7) + e, ej = pair >> 3;  // patch-relative element
+                    const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
+                    const bool ok = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
+                    const float Ee = ok ? E[er + (int64_t)gd.ny * ec + ple * el] : 0.0f;
+                    // inputs: g = 0 loads components x (K-step 0) and z (K-step 1)
+                    // of the 8 nodes, g = 1 component y (K-step 0)
+                    float wa[8], wb[8];
+                    if (Ee != 0.0f) {
+                        const int64_t base = (int64_t)(er + 1) + (int64_t)NY * ec + pl * el;  // node (x0, y0, z0)
+#pragma unroll
+                        for (int n = 0; n < 8; ++n) {
+                            const int xn = (n == 1 || n == 2 || n == 5 || n == 6);
+                            const int yn = (n == 2 || n == 3 || n == 6 || n == 7);
+                            const int zn = n >= 4;
+                            const int64_t gn = base - yn + (int64_t)NY * xn + pl * zn;
+                            const float *pn = p + (3 * gn) * NR + j;
+                            wa[n] = pn[g * NR];            // x (g = 0) or y (g = 1)
+                            wb[n] = g ? 0.0f : pn[2 * NR];  // z
+                        }
+                        const float a0 = wa[0], b0 = wb[0];
+#pragma unroll
+                        for (int n = 0; n < 8; ++n) {
+                            wa[n] = Ee * (wa[n] - a0);
+                            wb[n] = Ee * (wb[n] - b0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int n = 0; n < 8; ++n) { wa[n] = 0.0f; wb[n] = 0.0f; }
+                    }
+                    bf16x8 Bh0, Bl0, Bh1, Bl1;
+                    split8(wa, Bh0, Bl0);
+                    split8(wb, Bh1, Bl1);
+                    f32x16 D = {};
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bh0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bl0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[0], Bh0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bh1, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bl1, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[1], Bh1, D, 0, 0, 0);
+                    const int tl = el + g;  // node layer of this lane's face
+                    if (Ee != 0.0f && tl >= Z0 && tl < Zend) {
+                        float *a = &acc[tl & 1][(ei + kTN * ej) * 3 * NR + j];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int m = 0; m < 4; ++m)
+                                atomicAdd(a + nodeOffP[m] * 3 * NR + c * NR, D[4 * c + m]);
+                    }
+                }
+            }
+            __syncthreads();
+            if (el >= Z0) {  // node layer el is complete
+                float *a = acc[el & 1];
+                const int64_t lay = pl * el;
+                for (int f = t; f < kTO * kTO * 3 * NR; f += kBT) {
+                    const int node = f / (3 * NR), within = f % (3 * NR);
+                    const int oi = node % kTO, oj = node / kTO;
+                    const int r = R0 + oi, c = C0 + oj;
+                    if (r < NY && c < NX) {
+                        const int comp = within / NR;
+                        const int64_t gn = r + (int64_t)NY * c + lay;
+                        const uint8_t fl = nflag[gn];
+                        const float v = ((fl & kSolid) && !(fl & (1 << comp)))
+                                            ? a[((oi + 1) + kTN * (oj + 1)) * 3 * NR + within]
+                                            : 0.0f;
+                        const int64_t idx = 3 * gn * NR + within;
+                        q[idx] = v;
+                        dotacc += (double)v * (double)p[idx];
+                    }
+                }
+                __syncthreads();
+                for (int i = t; i < kBS; i += kBT) a[i] = 0.0f;
+                __syncthreads();
+            }
+        }
+    }
+    block_sum16(dotacc, partial + (int64_t)blockIdx.x * NR);
+}
+
+// ------------------------------ vector kernels ------------------------------
+// Grid-stride over n16 = 16 x lattice DOFs; a thread's case is t % 16.
+__device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
+
+// xc += alpha p; r -= alpha q; partial r.r per case
+__global__ void __launch_bounds__(256) k_bupdate_xr(float *__restrict__ xc, float *__restrict__ r,
+                                                    const float *__restrict__ p, const float *__restrict__ q,
+                                                    int64_t n16, const BCtl *__restrict__ ctl,
+                                                    double *__restrict__ partial) {
+    const float al = (float)ctl->alpha[threadIdx.x & 15];
+    double acc = 0.0;
+    for (int64_t i = gtid(); i < n16; i += gstride()) {
+        xc[i] += al * p[i];
+        const float rn = r[i] - al * q[i];
+        r[i] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    block_sum16(acc, partial + (int64_t)blockIdx.x * NR);
+}
+
+// p_new = z + beta p_old
+__global__ void __launch_bounds__(256) k_bupdate_p(float *__restrict__ pnew, const float *__restrict__ z,
+                                                   const float *__restrict__ pold, int64_t n16,
+                                                   const BCtl *__restrict__ ctl) {
+    const float be = (float)ctl->beta[threadIdx.x & 15];
+    for (int64_t i = gtid(); i < n16; i += gstride())
+        pnew[i] = z[i] + be * pold[i];
+}
+
+enum BFin { BFIN_ALPHA = 0, BFIN_RR, BFIN_RHO };
+
+// Per-case reduction of partial[nb][16] and the scalar step of the PCG
+// recurrence (Solving_PreconditionedConjugateGradientSolver.m:15-28).
+__global__ void __launch_bounds__(256) k_bfin(const double *__restrict__ partial, int nb, int op,
+                                              BCtl *__restrict__ ctl) {
+    __shared__ double sh[16][NR];
+    const int j = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    double s = 0.0;
+    for (int b = grp; b < nb; b += 16) s += partial[(int64_t)b * NR + j];
+    sh[grp][j] = s;
+    __syncthreads();
+    if (threadIdx.x >= NR) return;
+    double tot = 0.0;
+    for (int i = 0; i < 16; ++i) tot += sh[i][j];
+    const bool act = ctl->active[j] != 0;
+    if (op == BFIN_ALPHA) {
+        ctl->pq[j] = tot;
+        ctl->alpha[j] = (act && tot > 0.0) ? ctl->rho[j] / tot : 0.0;
+    } else if (op == BFIN_RR) {
+        ctl->rr[j] = tot;
+        ctl->res[j] = sqrt(tot);
+        if (act) ctl->its[j] += 1;
+    } else {
+        const double rho0 = ctl->rho[j];
+        ctl->beta[j] = (act && !ctl->restart[j] && rho0 != 0.0) ? tot / rho0 : 0.0;
+        ctl->rho[j] = tot;
+        ctl->restart[j] = 0;
+    }
+}
+
+// ------------------------------ batched V-cycle -----------------------------
+template <int S>
+__global__ void __launch_bounds__(256) k_brestrict(const float *__restrict__ rf, Grid gf, float *__restrict__ rc,
+                                                   Grid gc, const uint8_t *__restrict__ cflag) {
+    const int j = threadIdx.x & 15;
+    const int64_t nN = gc.nodes();
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    const int64_t NYf = gf.NY(), NXf = gf.NX(), NZf = gf.NZ();
+    for (int64_t G = gtid() >> 4; G < nN; G += gstride() >> 4) {
+        const int R = (int)(G % NYc), Cc = (int)((G / NYc) % NXc), Lc = (int)(G / (NYc * NXc));
+        float a0 = 0, a1 = 0, a2 = 0;
+        if (cflag[G] & kSolid) {
+#pragma unroll
+            for (int dl = -(S - 1); dl <= S - 1; ++dl) {
+                const int64_t lf = (int64_t)S * Lc + dl;
+                if (lf < 0 || lf >= NZf) continue;
+                const float wz = float(S - (dl < 0 ? -dl : dl)) / float(S);
+#pragma unroll
+                for (int dc = -(S - 1); dc <= S - 1; ++dc) {
+                    const int64_t cf = (int64_t)S * Cc + dc;
+                    if (cf < 0 || cf >= NXf) continue;
+                    const float wx = float(S - (dc < 0 ? -dc : dc)) / float(S);
+#pragma unroll
+                    for (int dr = -(S - 1); dr <= S - 1; ++dr) {
+                        const int64_t rr = (int64_t)S * R + dr;
+                        if (rr < 0 || rr >= NYf) continue;
+                        const float w = wz * wx * (float(S - (dr < 0 ? -dr : dr)) / float(S));
+                        const float *pn = rf + 3 * (rr + NYf * cf + NYf * NXf * lf) * NR + j;
+                        a0 += w * pn[0];
+                        a1 += w * pn[NR];
+                        a2 += w * pn[2 * NR];
+                    }
+                }
+            }
+        }
+        float *po = rc + 3 * G * NR + j;
+        po[0] = a0;
+        po[NR] = a1;
+        po[2 * NR] = a2;
+    }
+}
+
+// x_f = P x_c + dinv_f r_f; finest: fixed DOFs 0 and partial x_f.r_f per case
+template <int S>
+__global__ void __launch_bounds__(256) k_bprolong(const float *__restrict__ xc, Grid gc, float *__restrict__ xf,
+                                                  Grid gf, const float *__restrict__ rf,
+                                                  const float *__restrict__ dinv,
+                                                  const uint8_t *__restrict__ fflag, int finest,
+                                                  double *__restrict__ partial) {
+    const int j = threadIdx.x & 15;
+    const int64_t NYf = gf.NY(), NXf = gf.NX(), nN = gf.nodes();
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    double dotacc = 0.0;
+    for (int64_t n = gtid() >> 4; n < nN; n += gstride() >> 4) {
+        const int r = (int)(n % NYf), c = (int)((n / NYf) % NXf), l = (int)(n / (NYf * NXf));
+        const uint8_t f = fflag[n];
+        float v[3] = {0, 0, 0};
+        if (f & kSolid) {
+            const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+#pragma unroll
+            for (int bl = 0; bl < 2; ++bl) {
+                if (bl && !ml) continue;
+                const float wl = bl ? float(ml) / float(S) : float(S - ml) / float(S);
+#pragma unroll
+                for (int bc = 0; bc < 2; ++bc) {
+                    if (bc && !mc) continue;
+                    const float wc = bc ? float(mc) / float(S) : float(S - mc) / float(S);
+#pragma unroll
+                    for (int br = 0; br < 2; ++br) {
+                        if (br && !mr) continue;
+                        const float w = wl * wc * (br ? float(mr) / float(S) : float(S - mr) / float(S));
+                        const float *pc = xc + 3 * ((qr + br) + NYc * (qc + bc) + NYc * NXc * (ql + bl)) * NR + j;
+                        v[0] += w * pc[0];
+                        v[1] += w * pc[NR];
+                        v[2] += w * pc[2 * NR];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int64_t idx = (3 * n + a) * NR + j;
+            const float rv = rf[idx];
+            float val = v[a] + dinv[3 * n + a] * rv;
+            if (finest && (f & (1 << a))) val = 0.0f;
+            xf[idx] = val;
+            dotacc += (double)val * (double)rv;
+        }
+    }
+    if (partial) block_sum16(dotacc, partial + (int64_t)blockIdx.x * NR);
+}
+
+// Coarsest level: x = Minv r for 16 cases; one block per row, threads split
+// the columns 16 ways.
+__global__ void __launch_bounds__(256) k_bcoarse(const double *__restrict__ Minv, int64_t n,
+                                                 const float *__restrict__ r, float *__restrict__ x) {
+    __shared__ double sh[16][NR];
+    const int j = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int64_t row = blockIdx.x;
+    double acc = 0.0;
+    for (int64_t k = grp; k < n; k += 16) acc += Minv[row * n + k] * (double)r[k * NR + j];
+    sh[grp][j] = acc;
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double s = 0.0;
+        for (int i = 0; i < 16; ++i) s += sh[i][j];
+        x[row * NR + j] = (float)s;
+    }
+}
+
+// ---------------------------- reliable updates ------------------------------
+// For the cases in `mask`: x64_j += scale_j xc_j, xc_j = 0.
+__global__ void __launch_bounds__(256) k_bfold(double *__restrict__ x64, float *__restrict__ xc, int64_t nd,
+                                               const double *__restrict__ scale, uint32_t mask) {
+    const int j = threadIdx.x & 15;
+    if (!((mask >> j) & 1u)) return;
+    const double sc = scale[j];
+    double *xj = x64 + (int64_t)j * nd;
+    for (int64_t i = gtid() >> 4; i < nd; i += gstride() >> 4) {
+        const int64_t idx = i * NR + j;
+        xj[i] += sc * (double)xc[idx];
+        xc[idx] = 0.0f;
+    }
+}
+
+// For the cases in `mask`: r_j = r64_j * inv_j (FP32), p_j *= f_j; for the
+// cases in `zmask`: r_j = p_j = 0.
+__global__ void __launch_bounds__(256) k_breplace(float *__restrict__ r, float *__restrict__ p,
+                                                  const double *__restrict__ r64, int64_t nd,
+                                                  const double *__restrict__ inv, const double *__restrict__ f,
+                                                  uint32_t mask, uint32_t zmask) {
+    const int j = threadIdx.x & 15;
+    const bool m = (mask >> j) & 1u, z = (zmask >> j) & 1u;
+    if (!m && !z) return;
+    const double iv = inv[j];
+    const float fj = (float)f[j];
+    const double *rj = r64 + (int64_t)j * nd;
+    for (int64_t i = gtid() >> 4; i < nd; i += gstride() >> 4) {
+        const int64_t idx = i * NR + j;
+        if (m) {
+            r[idx] = (float)(rj[i] * iv);
+            p[idx] = fj * p[idx];
+        } else {
+            r[idx] = 0.0f;
+            p[idx] = 0.0f;
+        }
+    }
+}
+
+// FP64 case-major lattice vectors <-> batched FP32 (tests and K.u).
+__global__ void __launch_bounds__(256) k_bpack(const double *__restrict__ src, int64_t nd, int nrhs,
+                                               float *__restrict__ dst) {
+    const int j = threadIdx.x & 15;
+    for (int64_t i = gtid() >> 4; i < nd; i += gstride() >> 4)
+        dst[i * NR + j] = j < nrhs ? (float)src[(int64_t)j * nd + i] : 0.0f;
+}
+__global__ void __launch_bounds__(256) k_bunpack(const float *__restrict__ src, int64_t nd, int nrhs,
+                                                 double *__restrict__ dst) {
+    const int j = threadIdx.x & 15;
+    if (j >= nrhs) return;
+    for (int64_t i = gtid() >> 4; i < nd; i += gstride() >> 4) dst[(int64_t)j * nd + i] = (double)src[i * NR + j];
+}
+
+}  // namespace
+
+// ================================ host side =================================
+struct BatchState {
+    int64_t nd = 0;                      // lattice DOFs of level 0
+    DevBuf<float> r, z, q, xc, p0, p1;   // level 0, 16 cases
+    std::vector<DevBuf<float>> lr, lx;   // levels >= 1
+    DevBuf<double> b64, x64, r64;        // 16 case-major FP64 lattice vectors each
+    DevBuf<double> part, scal;           // partial sums [blocks][16]; per-case scalars
+    DevBuf<float> Kc;                    // Ke, component-major 24 x 24
+    DevBuf<BCtl> ctl;
+    BCtl *hctl = nullptr;                // pinned mirror
+    int kGrid = 1;                       // persistent grid of k_bkbyu
+    TileMap tiles{1, 1, 1, 32};
+    ~BatchState() {
+        if (hctl) (void)hipHostFree(hctl);
+    }
+};
+
+void batch_free(BatchState *b) { delete b; }
+
+namespace {
+
+constexpr int kBRedBlocks = 4096;  // partial slots per reduction (>= every reducing grid)
+
+int batch_init(sgld_ctx *C) {
+    if (C->batch) return SGLD_OK;
+    std::unique_ptr<BatchState> B(new BatchState());
+    const int64_t nd = C->nDof();
+    B->nd = nd;
+    const size_t n16 = (size_t)nd * NR;
+    SGLD_TRY(B->r.alloc(n16));
+    SGLD_TRY(B->z.alloc(n16));
+    SGLD_TRY(B->q.alloc(n16));
+    SGLD_TRY(B->xc.alloc(n16));
+    SGLD_TRY(B->p0.alloc(n16));
+    SGLD_TRY(B->p1.alloc(n16));
+    B->lr.resize(C->L);
+    B->lx.resize(C->L);
+    for (int k = 1; k < C->L; ++k) {
+        const size_t nk = (size_t)3 * C->lv[k].g.nodes() * NR;
+        SGLD_TRY(B->lr[k].alloc(nk));
+        SGLD_TRY(B->lx[k].alloc(nk));
+    }
+    SGLD_TRY(B->b64.alloc(n16));
+    SGLD_TRY(B->x64.alloc(n16));
+    SGLD_TRY(B->r64.alloc(n16));
+    SGLD_TRY(B->part.alloc((size_t)kBRedBlocks * NR));
+    SGLD_TRY(B->scal.alloc(4 * NR));
+    SGLD_TRY(B->ctl.alloc(1));
+    SGLD_HIP(hipHostMalloc((void **)&B->hctl, sizeof(BCtl), hipHostMallocDefault));
+    std::vector<float> kc(576);
+    for (int i = 0; i < 24; ++i)
+        for (int k = 0; k < 24; ++k) {
+            const int ni = i % 8, ai = i / 8, nk = k % 8, ak = k / 8;
+            kc[i * 24 + k] = (float)C->Ke[(3 * ni + ai) + 24 * (3 * nk + ak)];
+        }
+    SGLD_TRY(B->Kc.upload(kc.data(), kc.size()));
+    const Grid &g = C->lv[0].g;
+    const int tilesR = (int)((g.NY() + kTO - 1) / kTO), tilesC = (int)((g.NX() + kTO - 1) / kTO);
+    const int cz = 32, tilesZ = (int)((g.NZ() + cz - 1) / cz);
+    B->tiles = TileMap{tilesR, tilesC, tilesZ, cz};
+    int dev = 0, cus = 0;
+    SGLD_HIP(hipGetDevice(&dev));
+    SGLD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    B->kGrid = (int)std::min<int64_t>(std::min<int64_t>(B->tiles.work(), cus), kBRedBlocks);
+    C->batch = B.release();
+    return SGLD_OK;
+}
+
+int bgrid(int64_t n16) { return (int)std::min<int64_t>(kBRedBlocks, std::max<int64_t>(1, (n16 + 255) / 256)); }
+
+int bfin(sgld_ctx *C, int nb, int op) {
+    hipLaunchKernelGGL(k_bfin, dim3(1), dim3(256), 0, C->s, C->batch->part.p, nb, op, C->batch->ctl.p);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int enqueue_bkbyu(sgld_ctx *C, const float *p, float *q, int *nbOut) {
+    BatchState *B = C->batch;
+    const Level &l0 = C->lv[0];
+    hipLaunchKernelGGL(k_bkbyu, dim3(B->kGrid), dim3(kBT), 0, C->s, p, q, C->E32.p, l0.nflag.p, l0.g, B->tiles,
+                       B->Kc.p, B->part.p);
+    SGLD_HIP(hipGetLastError());
+    *nbOut = B->kGrid;
+    return SGLD_OK;
+}
+
+// z = M r for all 16 cases; partial z.r per case.
+int enqueue_bvcycle(sgld_ctx *C, const float *r, float *z, int *nbOut) {
+    BatchState *B = C->batch;
+    const int L = C->L;
+    const float *rf = r;
+    for (int k = 1; k < L; ++k) {
+        const Level &F = C->lv[k - 1], &Cc = C->lv[k];
+        const int nb = bgrid(Cc.g.nodes() * NR);
+        if (Cc.span == 2)
+            hipLaunchKernelGGL(k_brestrict<2>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, Cc.nflag.p);
+        else
+            hipLaunchKernelGGL(k_brestrict<4>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, Cc.nflag.p);
+        SGLD_HIP(hipGetLastError());
+        rf = B->lr[k].p;
+    }
+    hipLaunchKernelGGL(k_bcoarse, dim3((int)C->nCoarseDof), dim3(256), 0, C->s, C->Minv.p, C->nCoarseDof,
+                       B->lr[L - 1].p, B->lx[L - 1].p);
+    SGLD_HIP(hipGetLastError());
+    for (int k = L - 1; k >= 1; --k) {
+        const Level &F = C->lv[k - 1], &Cc = C->lv[k];
+        const bool finest = k == 1;
+        float *xf = finest ? z : B->lx[k - 1].p;
+        const float *rfk = finest ? r : B->lr[k - 1].p;
+        const int nb = bgrid(F.g.nodes() * NR);
+        double *part = finest ? B->part.p : nullptr;
+        if (Cc.span == 2)
+            hipLaunchKernelGGL(k_bprolong<2>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, rfk,
+                               F.dinv32.p, F.nflag.p, (int)finest, part);
+        else
+            hipLaunchKernelGGL(k_bprolong<4>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, rfk,
+                               F.dinv32.p, F.nflag.p, (int)finest, part);
+        SGLD_HIP(hipGetLastError());
+        if (finest) *nbOut = nb;
+    }
+    return SGLD_OK;
+}
+
+int upload_ctl(sgld_ctx *C) {
+    SGLD_HIP(hipMemcpyAsync(C->batch->ctl.p, C->batch->hctl, sizeof(BCtl), hipMemcpyHostToDevice, C->s));
+    return SGLD_OK;
+}
+int download_ctl(sgld_ctx *C) {
+    SGLD_HIP(hipMemcpyAsync(C->batch->hctl, C->batch->ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+// Batched mixed solve of nrhs cases; dF, dU compressed FP64 (numDOF x nrhs,
+// case-major), dU holds the initial guesses.
+int solve_batched(sgld_ctx *C, int nrhs, const double *dF, double *dU, double tol, int maxit, int *itsOut,
+                  double *relres, int *updatesOut) {
+    SGLD_TRY(batch_init(C));
+    BatchState *B = C->batch;
+    const int64_t nd = B->nd, n16 = nd * NR;
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    double *b64 = B->b64.p, *x64 = B->x64.p, *r64 = B->r64.p;
+    SGLD_HIP(hipMemsetAsync(B->xc.p, 0, sizeof(float) * n16, C->s));
+    SGLD_HIP(hipMemsetAsync(B->p0.p, 0, sizeof(float) * n16, C->s));
+    SGLD_HIP(hipMemsetAsync(B->r.p, 0, sizeof(float) * n16, C->s));
+    std::vector<double> scale(NR, 0.0), normb(NR, 0.0), itol(NR, 0.0), fct(NR, 1.0), inv(NR, 0.0);
+    std::vector<int> done(NR, 1), itsAtUpdate(NR, 0);
+    double delta = 0.1;
+    if (const char *env = std::getenv("SGLD_MIXED_DELTA")) delta = std::atof(env);
+    const int maxStretch = 96;
+    BCtl &h = *B->hctl;
+    std::memset(&h, 0, sizeof(BCtl));
+    int updates = 0;
+    uint32_t mask = 0;
+    for (int j = 0; j < nrhs; ++j) {
+        SGLD_TRY(to_lattice(C, dF + j * nc, b64 + j * nd));
+        SGLD_TRY(to_lattice(C, dU + j * nc, x64 + j * nd));
+        double rr = 0, bb = 0;
+        SGLD_TRY(residual64(C, b64 + j * nd, x64 + j * nd, r64 + j * nd, &rr, &bb));
+        normb[j] = std::sqrt(bb);
+        relres[j] = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        itsOut[j] = 0;
+        if (!(bb > 0) || relres[j] < tol || !(rr > 0)) continue;
+        done[j] = 0;
+        scale[j] = std::sqrt(rr);
+        inv[j] = 1.0 / scale[j];
+        itol[j] = std::max(delta, 0.5 * tol * normb[j] / scale[j]);
+        h.active[j] = 1;
+        h.restart[j] = 1;
+        mask |= 1u << j;
+    }
+    if (!mask) return SGLD_OK;
+    const int nbv = bgrid(n16);
+    SGLD_HIP(hipMemcpyAsync(B->scal.p, inv.data(), sizeof(double) * NR, hipMemcpyHostToDevice, C->s));
+    SGLD_HIP(hipMemcpyAsync(B->scal.p + NR, fct.data(), sizeof(double) * NR, hipMemcpyHostToDevice, C->s));
+    hipLaunchKernelGGL(k_breplace, dim3(nbv), dim3(256), 0, C->s, B->r.p, B->p0.p, r64, nd, B->scal.p,
+                       B->scal.p + NR, mask, 0u);
+    SGLD_HIP(hipGetLastError());
+    SGLD_TRY(upload_ctl(C));
+    float *P[2] = {B->p0.p, B->p1.p};
+    int cur = 0, nb = 0;
+    // z = M r, rho = z.r, p = z
+    SGLD_TRY(enqueue_bvcycle(C, B->r.p, B->z.p, &nb));
+    SGLD_TRY(bfin(C, nb, BFIN_RHO));
+    hipLaunchKernelGGL(k_bupdate_p, dim3(nbv), dim3(256), 0, C->s, P[1], B->z.p, P[0], n16, B->ctl.p);
+    SGLD_HIP(hipGetLastError());
+    cur = 1;
+    while (true) {
+        SGLD_TRY(enqueue_bkbyu(C, P[cur], B->q.p, &nb));
+        SGLD_TRY(bfin(C, nb, BFIN_ALPHA));
+        hipLaunchKernelGGL(k_bupdate_xr, dim3(nbv), dim3(256), 0, C->s, B->xc.p, B->r.p, P[cur], B->q.p, n16,
+                           B->ctl.p, B->part.p);
+        SGLD_HIP(hipGetLastError());
+        SGLD_TRY(bfin(C, nbv, BFIN_RR));
+        SGLD_TRY(download_ctl(C));
+        uint32_t flag = 0, forced = 0;
+        for (int j = 0; j < NR; ++j) {
+            if (done[j] || !h.active[j]) continue;
+            const bool conv = h.res[j] < itol[j];
+            const bool stretch = h.its[j] - itsAtUpdate[j] >= maxStretch;
+            if (conv || stretch || h.its[j] >= maxit || !(h.res[j] == h.res[j])) flag |= 1u << j;
+            if (!conv && stretch) forced |= 1u << j;
+        }
+        if (flag) {
+            std::vector<double> sc(NR, 0.0);
+            for (int j = 0; j < NR; ++j) sc[j] = scale[j];
+            SGLD_HIP(hipMemcpyAsync(B->scal.p + 2 * NR, sc.data(), sizeof(double) * NR, hipMemcpyHostToDevice,
+                                    C->s));
+            hipLaunchKernelGGL(k_bfold, dim3(nbv), dim3(256), 0, C->s, x64, B->xc.p, nd, B->scal.p + 2 * NR, flag);
+            SGLD_HIP(hipGetLastError());
+            uint32_t keep = 0, zero = 0;
+            for (int j = 0; j < NR; ++j) {
+                if (!((flag >> j) & 1u)) continue;
+                double rr = 0, bb = 0;
+                SGLD_TRY(residual64(C, b64 + j * nd, x64 + j * nd, r64 + j * nd, &rr, &bb));
+                ++updates;
+                relres[j] = std::sqrt(rr / bb);
+                itsOut[j] = h.its[j];
+                if (relres[j] < tol || h.its[j] >= maxit || !(rr > 0)) {
+                    done[j] = 1;
+                    h.active[j] = 0;
+                    zero |= 1u << j;
+                    continue;
+                }
+                fct[j] = scale[j] / std::sqrt(rr);
+                scale[j] = std::sqrt(rr);
+                inv[j] = 1.0 / scale[j];
+                itol[j] = std::max(delta, 0.5 * tol * normb[j] / scale[j]);
+                h.rho[j] *= fct[j] * fct[j];
+                h.restart[j] = (forced >> j) & 1u;
+                itsAtUpdate[j] = h.its[j];
+                keep |= 1u << j;
+            }
+            SGLD_HIP(hipMemcpyAsync(B->scal.p, inv.data(), sizeof(double) * NR, hipMemcpyHostToDevice, C->s));
+            SGLD_HIP(hipMemcpyAsync(B->scal.p + NR, fct.data(), sizeof(double) * NR, hipMemcpyHostToDevice,
+                                    C->s));
+            hipLaunchKernelGGL(k_breplace, dim3(nbv), dim3(256), 0, C->s, B->r.p, P[cur], r64, nd, B->scal.p,
+                               B->scal.p + NR, keep, zero);
+            SGLD_HIP(hipGetLastError());
+            SGLD_TRY(upload_ctl(C));
+        }
+        bool any = false;
+        for (int j = 0; j < NR; ++j) any |= !done[j];
+        if (!any) break;
+        SGLD_TRY(enqueue_bvcycle(C, B->r.p, B->z.p, &nb));
+        SGLD_TRY(bfin(C, nb, BFIN_RHO));
+        hipLaunchKernelGGL(k_bupdate_p, dim3(nbv), dim3(256), 0, C->s, P[1 - cur], B->z.p, P[cur], n16, B->ctl.p);
+        SGLD_HIP(hipGetLastError());
+        cur = 1 - cur;
+    }
+    for (int j = 0; j < nrhs; ++j) SGLD_TRY(from_lattice(C, x64 + j * nd, dU + j * nc));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    *updatesOut = updates;
+    return SGLD_OK;
+}
+
+}  // namespace
+}  // namespace sgld
+
+using namespace sgld;
+
+// ================================= C ABI ====================================
+extern "C" int sgld_ctx_pcg_batched_device(sgld_ctx *C, int nrhs, const double *dF, double *dU, double tol,
+                                           int maxIT, int precision, int *its, double *relres) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(nrhs >= 1 && nrhs <= NR, "sgld:arg: nrhs must be in 1..16");
+    SGLD_ARG(dF && dU && its && relres, "sgld:arg: null pointer");
+    SGLD_ARG(tol > 0 && maxIT > 0, "sgld:arg: need tol > 0 and maxIT > 0");
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    const double t0 = now_s();
+    int allConv = 1;
+    if (precision == SGLD_PREC_FP64) {
+        // sequential single-case FP64 solves (the reference recurrence)
+        for (int j = 0; j < nrhs; ++j) {
+            double rr = 0;
+            const int rc = sgld_ctx_pcg_device(C, dF + j * nc, dU + j * nc, tol, maxIT, SGLD_PREC_FP64, its + j, &rr);
+            if (rc != SGLD_OK && rc != SGLD_ERR_MAXIT) return rc;
+            relres[j] = rr;
+            if (rc == SGLD_ERR_MAXIT) allConv = 0;
+        }
+        C->stats[0] = now_s() - t0;
+        return allConv ? SGLD_OK : SGLD_ERR_MAXIT;
+    }
+    SGLD_ARG(precision == SGLD_PREC_MIXED, "sgld:arg: precision must be SGLD_PREC_FP64 or SGLD_PREC_MIXED");
+    int updates = 0;
+    SGLD_TRY(solve_batched(C, nrhs, dF, dU, tol, maxIT, its, relres, &updates));
+    C->stats[0] = now_s() - t0;
+    C->stats[3] = updates;
+    int maxits = 0;
+    for (int j = 0; j < nrhs; ++j) {
+        maxits = std::max(maxits, its[j]);
+        if (!(relres[j] < tol)) allConv = 0;
+    }
+    C->stats[1] = maxits;
+    if (!allConv) {
+        set_error("sgld:maxit: %d of %d cases did not reach tol %.3e within %d iterations", nrhs, nrhs, tol, maxIT);
+        return SGLD_ERR_MAXIT;
+    }
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_pcg_batched(sgld_ctx *C, int nrhs, const double *F, double *U, double tol, int maxIT,
+                                    int precision, int *its, double *relres) {
+    SGLD_TRY(check_ready(C));
+    SGLD_ARG(nrhs >= 1 && nrhs <= NR, "sgld:arg: nrhs must be in 1..16");
+    SGLD_ARG(F && U, "sgld:arg: null pointer");
+    const int64_t nc = 3 * C->lv[0].numNodes;
+    DevBuf<double> dF, dU;
+    SGLD_TRY(dF.upload(F, (size_t)nc * nrhs));
+    SGLD_TRY(dU.upload(U, (size_t)nc * nrhs));
+    const int rc = sgld_ctx_pcg_batched_device(C, nrhs, dF.p, dU.p, tol, maxIT, precision, its, relres);
+    if (rc != SGLD_OK && rc != SGLD_ERR_MAXIT) return rc;
+    SGLD_HIP(hipMemcpy(U, dU.p, sizeof(double) * nc * nrhs, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+extern "C" int sgld_ctx_kbyu_batched_device(sgld_ctx *C, int nrhs, const double *dU, double *dY) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
+    SGLD_ARG(nrhs >= 1 && nrhs <= NR, "sgld:arg: nrhs must be in 1..16");
+    SGLD_ARG(dU && dY, "sgld:arg: null pointer");
+    SGLD_TRY(batch_init(C));
+    BatchState *B = C->batch;
+    const int64_t nd = B->nd, nc = 3 * C->lv[0].numNodes;
+    SGLD_HIP(hipMemsetAsync(B->r64.p, 0, sizeof(double) * nd * NR, C->s));
+    for (int j = 0; j < nrhs; ++j) SGLD_TRY(to_lattice(C, dU + j * nc, B->r64.p + j * nd));
+    const int nbv = bgrid(nd * NR);
+    hipLaunchKernelGGL(k_bpack, dim3(nbv), dim3(256), 0, C->s, B->r64.p, nd, nrhs, B->p0.p);
+    SGLD_HIP(hipGetLastError());
+    int nb = 0;
+    SGLD_TRY(enqueue_bkbyu(C, B->p0.p, B->q.p, &nb));
+    hipLaunchKernelGGL(k_bunpack, dim3(nbv), dim3(256), 0, C->s, B->q.p, nd, nrhs, B->r64.p);
+    SGLD_HIP(hipGetLastError());
+    for (int j = 0; j < nrhs; ++j) SGLD_TRY(from_lattice(C, B->r64.p + j * nd, dY + j * nc));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_bench_kbyu_batched(sgld_ctx *C, int reps, double *msPerLaunch) {
+    SGLD_ARG(C && msPerLaunch && reps > 0, "sgld:arg: bad arguments");
+    SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
+    SGLD_TRY(batch_init(C));
+    BatchState *B = C->batch;
+    int nb = 0;
+    SGLD_TRY(enqueue_bkbyu(C, B->p0.p, B->q.p, &nb));  // warm-up
+    SGLD_HIP(hipEventRecord(C->e0, C->s));
+    for (int i = 0; i < reps; ++i) SGLD_TRY(enqueue_bkbyu(C, B->p0.p, B->q.p, &nb));
+    SGLD_HIP(hipEventRecord(C->e1, C->s));
+    SGLD_HIP(hipEventSynchronize(C->e1));
+    float ms = 0;
+    SGLD_HIP(hipEventElapsedTime(&ms, C->e0, C->e1));
+    *msPerLaunch = ms / reps;
+    return SGLD_OK;
+}

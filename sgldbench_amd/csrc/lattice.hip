@@ -9,7 +9,7 @@
 #include <memory>
 #include <vector>
 
-#include "lattice_kernels.hpp"
+#include "lattice_ctx.hpp"
 
 namespace sgld {
 namespace {
@@ -204,15 +204,6 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
 // halo nodes run behind the same L2.  Node layers are prefetched two
 // iterations ahead into registers and staged in a 3-slot LDS ring.
 // ----------------------------------------------------------------------------
-constexpr int kTE = 16;        // element columns per patch edge
-constexpr int kTN = kTE + 1;   // staged nodes per patch edge
-constexpr int kTO = kTE - 1;   // output nodes per patch edge
-constexpr int kXCD = 8;        // accelerator complex dies (one L2 each)
-
-struct TileMap {
-    int tilesR, tilesC, tilesZ, cz;
-    __host__ __device__ int64_t work() const { return (int64_t)tilesR * tilesC * tilesZ; }
-};
 
 template <class T>
 __device__ __forceinline__ void face_modes(const T *__restrict__ lay, int ei, int ej, T (&F)[3][4]) {
@@ -993,48 +984,6 @@ __global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u
 using namespace sgld;
 
 namespace {
-struct Level {
-    Grid g{};
-    int span = 1;
-    int64_t numElements = 0, numNodes = 0;
-    std::vector<uint8_t> esolid;  // host, lattice elements
-    std::vector<uint8_t> hflag;   // host, lattice nodes
-    DevBuf<uint8_t> nflag;
-    DevBuf<double> dinv, r, x;
-    DevBuf<float> dinv32, r32, x32;
-};
-}  // namespace
-
-struct sgld_ctx {
-    int L = 0, nonDyadic = 0;
-    double nu = 0.3, h = 1.0, omega = 0.6;
-    double Ke[576];
-    std::vector<Level> lv;
-    bool full = true;
-    std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
-    DevBuf<int32_t> nodMap, eleMap;
-    DevBuf<double> E;
-    DevBuf<float> E32;
-    DevBuf<double> Minv;
-    int64_t nCoarseDof = 0;
-    DevBuf<double> b, x, r, z, p, p2, q;
-    DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
-    DevBuf<double> partial;
-    DevBuf<Ctl> ctl;
-    Ctl *hctl = nullptr;  // pinned host mirror
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool haveFixed = false, haveModulus = false, assembled = false;
-    int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
-    TileMap tiles{1, 1, 1, 16};
-    int tiledGrid = 1;
-    double stats[8] = {0};
-    int64_t nDof() const { return 3 * lv[0].g.nodes(); }
-};
-
-namespace {
-
-int grid_blocks(int64_t n) { return (int)std::min<int64_t>(kVecBlocks, std::max<int64_t>(1, (n + 255) / 256)); }
 
 template <class T> T *lvl_r(Level &l);
 template <> double *lvl_r<double>(Level &l) { return l.r.p; }
@@ -1268,48 +1217,9 @@ int run_pcg(sgld_ctx *C, const T *b, T *x, T *r, T *z, T *p0, T *p1, T *q, bool 
     return SGLD_OK;
 }
 
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 int upload_flags(Level &l) {
     if (!l.nflag.p) SGLD_TRY(l.nflag.alloc(l.g.nodes()));
     SGLD_HIP(hipMemcpy(l.nflag.p, l.hflag.data(), l.hflag.size(), hipMemcpyHostToDevice));
-    return SGLD_OK;
-}
-
-int to_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
-    const int64_t n = C->nDof();
-    if (C->full) {
-        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * n, hipMemcpyDeviceToDevice, C->s));
-        return SGLD_OK;
-    }
-    SGLD_HIP(hipMemsetAsync(ddst, 0, sizeof(double) * n, C->s));
-    const int64_t nn = C->lv[0].numNodes;
-    hipLaunchKernelGGL(k_scatter_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
-                       C->nodMap.p, nn, ddst);
-    SGLD_HIP(hipGetLastError());
-    return SGLD_OK;
-}
-
-int from_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
-    if (C->full) {
-        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * C->nDof(), hipMemcpyDeviceToDevice, C->s));
-        return SGLD_OK;
-    }
-    const int64_t nn = C->lv[0].numNodes;
-    hipLaunchKernelGGL(k_gather_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
-                       C->nodMap.p, nn, ddst);
-    SGLD_HIP(hipGetLastError());
-    return SGLD_OK;
-}
-
-int check_ready(sgld_ctx *C) {
-    SGLD_ARG(C, "sgld:arg: null context");
-    if (!C->assembled) {
-        set_error("sgld:state: call sgld_ctx_set_fixed, sgld_ctx_set_modulus and sgld_ctx_assemble first");
-        return SGLD_ERR_STATE;
-    }
     return SGLD_OK;
 }
 
@@ -1321,20 +1231,8 @@ int solve_fp64(sgld_ctx *C, double tol, int maxit, int *its, double *res, int *d
 
 // True FP64 relative residual ||b - K x|| / ||b||; leaves b - Kx in C->r.
 int true_residual(sgld_ctx *C, double *res, double *normr) {
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
-    SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, n);
-    SGLD_HIP(hipGetLastError());
-    std::vector<double> part(2 * kRedBlocks);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->r.p, C->r.p, n, slot(C, 0), nullptr);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->b.p, n, slot(C, 1), nullptr);
-    SGLD_HIP(hipGetLastError());
-    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
-                            hipMemcpyDeviceToHost, C->s));
-    SGLD_HIP(hipStreamSynchronize(C->s));
     double rr = 0, bb = 0;
-    for (int i = 0; i < nbv; ++i) { rr += part[i]; bb += part[kRedBlocks + i]; }
+    SGLD_TRY(residual64(C, C->b.p, C->x.p, C->r.p, &rr, &bb));
     *normr = std::sqrt(rr);
     *res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     return SGLD_OK;
@@ -1420,6 +1318,69 @@ int solve_mixed(sgld_ctx *C, double tol, int maxit, int *itsTot, double *resOut,
 }
 
 }  // namespace
+
+namespace sgld {
+
+int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb) {
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    SGLD_TRY(enqueue_kbyu<double>(C, x, C->q.p, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, n);
+    SGLD_HIP(hipGetLastError());
+    std::vector<double> part(2 * kRedBlocks);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, r, r, n, slot(C, 0), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, b, b, n, slot(C, 1), nullptr);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
+                            hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    double a = 0, c = 0;
+    for (int i = 0; i < nbv; ++i) { a += part[i]; c += part[kRedBlocks + i]; }
+    *rr = a;
+    *bb = c;
+    return SGLD_OK;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int to_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
+    const int64_t n = C->nDof();
+    if (C->full) {
+        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * n, hipMemcpyDeviceToDevice, C->s));
+        return SGLD_OK;
+    }
+    SGLD_HIP(hipMemsetAsync(ddst, 0, sizeof(double) * n, C->s));
+    const int64_t nn = C->lv[0].numNodes;
+    hipLaunchKernelGGL(k_scatter_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
+                       C->nodMap.p, nn, ddst);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int from_lattice(sgld_ctx *C, const double *dsrc, double *ddst) {
+    if (C->full) {
+        SGLD_HIP(hipMemcpyAsync(ddst, dsrc, sizeof(double) * C->nDof(), hipMemcpyDeviceToDevice, C->s));
+        return SGLD_OK;
+    }
+    const int64_t nn = C->lv[0].numNodes;
+    hipLaunchKernelGGL(k_gather_nodes<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, C->s, dsrc,
+                       C->nodMap.p, nn, ddst);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int check_ready(sgld_ctx *C) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    if (!C->assembled) {
+        set_error("sgld:state: call sgld_ctx_set_fixed, sgld_ctx_set_modulus and sgld_ctx_assemble first");
+        return SGLD_ERR_STATE;
+    }
+    return SGLD_OK;
+}
+
+}  // namespace sgld
 
 // ================================= C ABI ====================================
 extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, const uint8_t *voxels,
@@ -1565,6 +1526,7 @@ extern "C" int sgld_ctx_destroy(sgld_ctx *C) {
     if (C->hctl) (void)hipHostFree(C->hctl);
     if (C->e0) (void)hipEventDestroy(C->e0);
     if (C->e1) (void)hipEventDestroy(C->e1);
+    batch_free(C->batch);
     hipStream_t s = C->s;
     delete C;
     if (s) (void)hipStreamDestroy(s);

@@ -1,0 +1,82 @@
+// Host-side state of one solver context (the device-resident mesh hierarchy,
+// the meshHierarchy_ of src/Data_CartesianMeshStruct.m) shared by the
+// single-RHS solver (lattice.hip) and the batched multi-RHS solver
+// (batched.hip).
+#pragma once
+#include <algorithm>
+#include <vector>
+
+#include "lattice_kernels.hpp"
+
+namespace sgld {
+
+constexpr int kTE = 16;        // element columns per K.u patch edge
+constexpr int kTN = kTE + 1;   // staged nodes per patch edge
+constexpr int kTO = kTE - 1;   // output nodes per patch edge
+constexpr int kXCD = 8;        // accelerator complex dies (one L2 each)
+
+// Work decomposition of the patch-tiled K.u: tilesR x tilesC patches of
+// kTO x kTO output node columns, each cut into z-chunks of cz node layers.
+struct TileMap {
+    int tilesR, tilesC, tilesZ, cz;
+    __host__ __device__ int64_t work() const { return (int64_t)tilesR * tilesC * tilesZ; }
+};
+
+struct Level {
+    Grid g{};
+    int span = 1;
+    int64_t numElements = 0, numNodes = 0;
+    std::vector<uint8_t> esolid;  // host, lattice elements
+    std::vector<uint8_t> hflag;   // host, lattice nodes
+    DevBuf<uint8_t> nflag;
+    DevBuf<double> dinv, r, x;
+    DevBuf<float> dinv32, r32, x32;
+};
+
+struct BatchState;             // batched.hip
+void batch_free(BatchState *b);
+
+}  // namespace sgld
+
+struct sgld_ctx {
+    int L = 0, nonDyadic = 0;
+    double nu = 0.3, h = 1.0, omega = 0.6;
+    double Ke[576];
+    std::vector<sgld::Level> lv;
+    bool full = true;
+    std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
+    sgld::DevBuf<int32_t> nodMap, eleMap;
+    sgld::DevBuf<double> E;
+    sgld::DevBuf<float> E32;
+    sgld::DevBuf<double> Minv;
+    int64_t nCoarseDof = 0;
+    sgld::DevBuf<double> b, x, r, z, p, p2, q;
+    sgld::DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
+    sgld::DevBuf<double> partial;
+    sgld::DevBuf<sgld::Ctl> ctl;
+    sgld::Ctl *hctl = nullptr;  // pinned host mirror
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool haveFixed = false, haveModulus = false, assembled = false;
+    int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
+    sgld::TileMap tiles{1, 1, 1, 16};
+    int tiledGrid = 1;
+    double stats[8] = {0};
+    sgld::BatchState *batch = nullptr;  // multi-RHS buffers, allocated on first use
+    int64_t nDof() const { return 3 * lv[0].g.nodes(); }
+};
+
+namespace sgld {
+
+inline int grid_blocks(int64_t n) {
+    return (int)std::min<int64_t>(kVecBlocks, std::max<int64_t>(1, (n + 255) / 256));
+}
+double now_s();
+int to_lattice(sgld_ctx *C, const double *dsrc, double *ddst);    // compressed -> lattice DOFs
+int from_lattice(sgld_ctx *C, const double *dsrc, double *ddst);  // lattice -> compressed DOFs
+int check_ready(sgld_ctx *C);
+// r = b - K x on lattice FP64 vectors (fixed and void DOFs 0), with
+// rr = r.r and bb = b.b; host-synchronous.
+int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb);
+
+}  // namespace sgld

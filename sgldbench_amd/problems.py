@@ -102,6 +102,40 @@ def cfg1(nx=256, ny=128, nz=128, levels=6, seed=CFG1_SEED):
                 num_levels=levels, omega=0.6, tol=1e-8, max_it=500)
 
 
+def face_load_cases(voxels, nrhs, seed):
+    """Load cases of the batched workload: each a unit total force spread
+    uniformly over the solid nodes of the x = nelx face, in a direction drawn
+    uniformly on the sphere (seeded); supports as Cantilever 3 (x = 0 face
+    fixed in x/y/z).  Returns (F (nrhs, numDOF), fixed uint8, directions)."""
+    nodes = solid_nodes(voxels)
+    xyz = node_coords(voxels, nodes)
+    nx = np.asarray(voxels).shape[1]
+    face = np.flatnonzero(xyz[:, 0] == nx)
+    fixed_n = np.flatnonzero(xyz[:, 0] == xyz[:, 0].min())
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((nrhs, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    n = len(nodes)
+    F = np.zeros((nrhs, 3 * n))
+    for a in range(3):
+        F[:, 3 * face + a] = d[:, a:a + 1] / len(face)
+    fixed = np.zeros(3 * n, dtype=np.uint8)
+    for a in range(3):
+        fixed[3 * fixed_n + a] = 1
+    return F, fixed, d
+
+
+def cfg3(n=256, nrhs=16, levels=6, seed=CFG3_SEED):
+    """n^3 cuboid, SIMP E from the GRF recipe of the 256x128x128 workload at
+    n^3, 16 load cases on the x = n face (unit force, random direction),
+    x = 0 face fixed."""
+    vox = cuboid(n, n, n)
+    F, fixed, d = face_load_cases(vox, nrhs, seed)
+    E = simp_modulus(random_density_cuboid(n, n, n, seed))
+    return dict(name="cuboid%d^3-grf-%drhs" % (n, nrhs), voxels=vox, E=E, F=F, fixed=fixed,
+                directions=d, num_levels=levels, omega=0.6, tol=1e-8, max_it=500)
+
+
 CFG2_SEED = 20277  # fill close to the ~30 % the workload describes (seed scan at 1/4 scale)
 
 

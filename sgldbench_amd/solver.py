@@ -252,6 +252,40 @@ class FEASolver:
         check(lib().sgld_ctx_bench_kbyu(self._h, precision, reps, C.byref(ms)))
         return ms.value
 
+    def pcg_batched(self, F, tol=1e-8, maxIT=500, U0=None, precision=PREC_MIXED):
+        """Up to 16 load cases at once: F is (nrhs, numDOFs) (one case per row),
+        each case solved as Solving_CG_GMGS would solve it alone.  Returns
+        (U (nrhs, numDOFs), its[nrhs], relres[nrhs]); relres is the FP64 true
+        relative residual of each case."""
+        F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+        k = F.shape[0]
+        assert F.shape[1] == self.numDOFs and 1 <= k <= 16
+        U = np.zeros_like(F) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).reshape(F.shape).copy()
+        its = np.zeros(k, dtype=np.int32)
+        rr = np.zeros(k)
+        rc = check(lib().sgld_ctx_pcg_batched(self._h, k, F, U, tol, maxIT, precision, its, rr),
+                   allow=(ERR_MAXIT,))
+        self.converged = rc == 0
+        return U, its, rr
+
+    def pcg_batched_device(self, nrhs, dF_ptr, dU_ptr, tol=1e-8, maxIT=500, precision=PREC_MIXED):
+        its = np.zeros(nrhs, dtype=np.int32)
+        rr = np.zeros(nrhs)
+        rc = check(lib().sgld_ctx_pcg_batched_device(self._h, nrhs, C.c_void_p(dF_ptr), C.c_void_p(dU_ptr), tol,
+                                                     maxIT, precision, its, rr), allow=(ERR_MAXIT,))
+        self.converged = rc == 0
+        return its, rr
+
+    def kbyu_batched_device(self, nrhs, dU_ptr, dY_ptr):
+        """K U for nrhs <= 16 device vectors (case-major, compressed) through the
+        batched matrix-core kernel."""
+        check(lib().sgld_ctx_kbyu_batched_device(self._h, nrhs, C.c_void_p(dU_ptr), C.c_void_p(dY_ptr)))
+
+    def bench_kbyu_batched(self, reps=20):
+        ms = C.c_double(0)
+        check(lib().sgld_ctx_bench_kbyu_batched(self._h, reps, C.byref(ms)))
+        return ms.value
+
     def stats(self):
         out = np.zeros(8)
         check(lib().sgld_ctx_stats(self._h, out))
