@@ -25,9 +25,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "cfg2", "part_r256"],
+    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "cfg2", "cfg3", "part_r256"],
                     help="cfg1: 256x128x128 GRF cuboid (default); cfg2: ~125 M-DOF ellipsoid union in 512^3; "
-                         "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
+                         "cfg3: 16 load cases on a 256^3 GRF cuboid, batched mixed vs 16 sequential FP64 "
+                         "solves; part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
+    ap.add_argument("--n3", type=int, default=256, help="cube edge of the batched workload")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
@@ -95,12 +97,107 @@ def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+BF16_PEAK_TFLOPS = 2500.0  # dense
+
+
+def run_batched(a):
+    """16 load cases on an n^3 GRF cuboid: one batched mixed-precision solve
+    (BF16 matrix-core K.u) against 16 sequential FP64 solves of the same
+    cases.  One step = all 16 cases solved to FP64 true rel-res 1e-8."""
+    import torch
+    from sgldbench_amd import problems, solver
+    n3 = a.n3
+    p = problems.cfg3(n3, 16)
+    k = p["F"].shape[0]
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s.set_fixed(p["fixed"])
+    s.set_modulus(p["E"])
+    t0 = time.time()
+    s.assemble(p["omega"])
+    setup_s = time.time() - t0
+    n, ne = s.numDOFs, s.numElements
+    progress("batched workload: %d^3, %d DOF x %d cases, assembled in %.2f s" % (n3, n, k, setup_s))
+    dF = torch.from_numpy(p["F"]).to(dev)
+    dU = torch.zeros_like(dF)
+
+    def solve(prec):
+        dU.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its, rr = s.pcg_batched_device(k, dF.data_ptr(), dU.data_ptr(), p["tol"], 5000, prec)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, its, rr
+
+    res, U = {}, {}
+    for prec, name in ((solver.PREC_FP64, "fp64_sequential"), (solver.PREC_MIXED, "batched_mixed")):
+        for _ in range(a.warmup):
+            solve(prec)
+        runs = []
+        for _ in range(a.steps):
+            runs.append(solve(prec))
+            progress("%s: %.3f s, its %s, max rel-res %.2e" % (name, runs[-1][0], runs[-1][1].tolist(),
+                                                              runs[-1][2].max()))
+        ts = [r[0] for r in runs]
+        res[name] = dict(s_per_step=float(np.mean(ts)), its=runs[-1][1].tolist(),
+                         max_relres=float(max(r[2].max() for r in runs)),
+                         converged=all(bool(np.all(r[2] <= p["tol"])) for r in runs))
+        U[name] = dU.cpu().numpy()
+    F = p["F"]
+    c64 = np.einsum("ij,ij->i", F, U["fp64_sequential"])
+    cmx = np.einsum("ij,ij->i", F, U["batched_mixed"])
+    urel = np.linalg.norm(U["batched_mixed"] - U["fp64_sequential"], axis=1) / np.linalg.norm(U["fp64_sequential"], axis=1)
+    ms_b = s.bench_kbyu_batched(a.kbyu_reps)
+    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
+    useful = 1152.0 * k * ne                       # flops of 16 dense 24x24 element products
+    executed = 6 * 2 * 32 * 32 * 16 * (ne / 2.0) * 1.0  # 6 MFMA 32x32x16 per element pair (hi/lo, padding)
+    bytes_b = 2 * n * k * 4 + ne * 4
+    t_b = res["batched_mixed"]["s_per_step"]
+    t_s = res["fp64_sequential"]["s_per_step"]
+    out = {
+        "metric": "MG-PCG time to rel-res 1e-8 & K.u GDOF/s, FP64 vs mixed/batched, 1 MI355X",
+        "value": k * n / t_b / 1e9,
+        "unit": "GDOF/s (DOF x load cases solved to rel-res 1e-8 per second)",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_b, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32+f64", "data": "synthetic (seeded Gaussian random density field, seeded load directions)",
+        "config": {"workload": ("cuboid %d^3 voxels, %d DOF, SIMP E from GRF, x=0 fixed, %d load cases "
+                                "(unit force on the x=%d face, random directions), 6 levels, omega 0.6, tol 1e-8"
+                                % (n3, n, k, n3)), "path": "batched mixed", "dofs": n, "elements": ne,
+                   "cases": k},
+        "batched": {"batched_mixed": res["batched_mixed"], "fp64_sequential": res["fp64_sequential"],
+                    "per_case_speedup": t_s / t_b,
+                    "compliance_rel_diff_max": float(np.max(np.abs(cmx - c64) / np.abs(c64))),
+                    "u_rel_diff_max": float(urel.max()), "setup_s": setup_s},
+        "kbyu": {"batched_ms": ms_b, "fp64_single_ms": ms64,
+                 "batched_per_case_ms": ms_b / k, "per_case_speedup_vs_fp64": ms64 * k / ms_b,
+                 "batched_GBs": bytes_b / ms_b / 1e6,
+                 "useful_TFLOPs": useful / ms_b / 1e9, "executed_mfma_TFLOPs": executed / ms_b / 1e9},
+        "roofline": {"bound": "mfma", "achieved": useful / ms_b / 1e9, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": useful / ms_b / 1e9 / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_bkbyu", "flops_per_launch": useful,
+                     "hbm_frac": bytes_b / ms_b / 1e6 / HBM_PEAK_GBS},
+    }
+    if not a.no_cpu:
+        p1 = dict(p)
+        p1["F"] = p["F"][0]
+        cb = cpu_baseline(p1, int(np.mean(res["fp64_sequential"]["its"])), sample_its=a.cpu_its)
+        cb["sample"] += "; 16 cases = 16 sequential solves"
+        cb["solve_s_est"] *= k
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
               "multi-GPU sharding is described in DESIGN.md, not run." % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == "cfg3":
+        run_batched(a)
+        return
     import torch
     from sgldbench_amd import problems, solver
 

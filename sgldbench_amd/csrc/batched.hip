@@ -75,54 +75,77 @@ __device__ __forceinline__ void block_sum16(double v, double *__restrict__ out) 
 // ----------------------------------------------------------------------------
 // q = K p for 16 cases on the BF16 matrix cores.
 //
-// A block owns a patch of 15 x 15 output node columns and a chunk of cz node
-// layers (the same patch/chunk decomposition and XCD-aware persistent walk as
-// the single-RHS kernel) and marches the element layers of the chunk.  For an
-// element layer the 16 x 16 element columns that touch the patch are 128
-// MFMA column pairs (2 elements x 16 cases = 32 MFMA columns); each of the 8
-// wavefronts computes 16 pairs:
-//   rows i = 8 a + n  (component a, element-local node n; 24..31 zero)
-//   B[k][col] = E_e (p - p_node0) at element-local DOF k of the column's
-//               element and case, split hi/lo,
-//   D = Ke_c B, 32 x 32 FP32, lane l holds rows 8 r/4 + 4 (l/32) + r%4:
-//   lanes 0-31 the 4 nodes of the lower face, lanes 32-63 the upper face.
-// The face forces are scaled by E_e and added with LDS atomics into a ring
-// of two node-layer accumulators covering the 17 x 17 patch nodes; a layer
-// is complete once the element layers below and above it are done, then its
-// 15 x 15 owned nodes are masked (fixed / void DOFs 0, as
-// Solving_KbyU_MatrixFree.m:39), stored, dotted with p for the PCG step
-// length, and the slot is cleared.
+// A block owns a patch of kPO_R x kPO_C output node columns (15 x 7) and a
+// chunk of cz node layers, walked persistently and XCD-aware like the
+// single-RHS kernel, and marches the element layers of the chunk.  Node
+// layers of p are staged in LDS (coalesced float4 loads, prefetched one layer
+// ahead in registers) for the 17 x 9 patch nodes; the 16 x 8 element columns
+// touching the patch are MFMA column pairs (2 elements x 16 cases = 32
+// columns), one pair per wavefront per colour pass:
+//   B[k][col]: k 0-7 x of nodes 0-7, 8-15 y of nodes 0-7, 16-19 z of nodes
+//              0-3, 24-27 z of nodes 4-7 (rest 0), as E_e (p - p_node0)
+//              split hi/lo;
+//   A = Ke with rows i = 8 a + n (component a, element-local node n) and the
+//       same column order;
+//   D = A B, lane l holds rows 8 (r/4) + 4 (l/32) + r%4: lanes 0-31 the
+//       forces of the lower face (nodes 0-3), lanes 32-63 the upper face.
+// The 4 colour passes (columns with (ei % 2, ej % 2) fixed) touch disjoint
+// nodes, so face forces are added into a 2-layer LDS accumulator ring with
+// plain read-add-write (FP32 LDS atomics are several times slower than FP64
+// ones on this part).  A node layer is complete after the element layers
+// below and above it; its owned nodes are then masked (fixed / void DOFs 0,
+// as Solving_KbyU_MatrixFree.m:39), stored, dotted with p for the PCG step
+// length, and the slot cleared.
 // ----------------------------------------------------------------------------
-constexpr int kBT = 512;                 // threads per block
-constexpr int kBS = kTN * kTN * 3 * NR;  // floats per accumulator layer
+constexpr int kBT = 1024;                 // threads per block (16 wavefronts)
+constexpr int kPE_R = 16, kPE_C = 8;      // element columns per patch (rows, cols)
+constexpr int kPN_R = kPE_R + 1, kPN_C = kPE_C + 1;  // staged patch nodes
+constexpr int kPO_R = kPE_R - 1, kPO_C = kPE_C - 1;  // owned output nodes
+constexpr int kPN = kPN_R * kPN_C;        // 153 staged nodes
+constexpr int kNV = 3 * NR;               // floats per node (3 components x 16 cases)
+constexpr int kLay = kPN * kNV;           // floats per staged / accumulated layer
+constexpr int kLd4 = kLay / 4;            // float4 per layer
+constexpr int kLdPer = (kLd4 + kBT - 1) / kBT;
+constexpr int kStPer = (kPO_R * kPO_C * kNV + kBT - 1) / kBT;  // stored floats per thread and layer
+static_assert(kPE_R * kPE_C <= kBT && kPN <= kBT, "staging threads");
+
+__device__ __forceinline__ int bcol_k(int k) {  // B/A column k -> element DOF (3 node + comp), -1 = zero
+    if (k < 8) return 3 * k;
+    if (k < 16) return 3 * (k - 8) + 1;
+    if (k < 20) return 3 * (k - 16) + 2;
+    if (k >= 24 && k < 28) return 3 * (k - 20) + 2;
+    return -1;
+}
 
 __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, float *__restrict__ q,
                                                   const float *__restrict__ E,
                                                   const uint8_t *__restrict__ nflag, Grid gd, TileMap tm,
-                                                  const float *__restrict__ Kc, double *__restrict__ partial) {
-    __shared__ float acc[2][kBS];
+                                                  const float *__restrict__ Ke, double *__restrict__ partial) {
+    __shared__ float stage[2][kLay];
+    __shared__ float acc[2][kLay];
+    __shared__ float Es[2][kPE_R * kPE_C];  // moduli of an element layer (ring)
+    __shared__ uint8_t Fs[2][kPN];          // node flags of a staged layer (ring, as stage)
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int g = lane >> 5;          // 0: lower-face rows / x,z inputs; 1: upper face / y inputs
-    const int col = lane & 31;        // MFMA column: element (col / 16) of the pair, case col % 16
+    const int g = lane >> 5;   // 0: lower-face output rows; 1: upper face
+    const int col = lane & 31;
     const int j = col & 15, e = col >> 4;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
     const int64_t pl = (int64_t)NY * NX;
     const int64_t ple = (int64_t)gd.ny * gd.nx;
 
-    // A operands: Ke_c rows i = lane % 32, columns 16 s + 8 g + 0..7, as hi/lo.
+    // A operands: rows i = lane % 32 (8 comp + node), columns 16 s + 8 g + 0..7
     bf16x8 Ahi[2], Alo[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int kk = 16 * s + 8 * g + i;
-            const float a = (col < 24 && kk < 24) ? Kc[col * 24 + kk] : 0.0f;
+            const int kc = bcol_k(16 * s + 8 * g + i);
+            const float a = (col < 24 && kc >= 0) ? Ke[(3 * (col & 7) + (col >> 3)) + 24 * kc] : 0.0f;
             const __bf16 hi = (__bf16)a;
             Ahi[s][i] = hi;
             Alo[s][i] = (__bf16)(a - (float)hi);
         }
-    for (int i = t; i < 2 * kBS; i += kBT) (&acc[0][0])[i] = 0.0f;
-    __syncthreads();
+    for (int i = t; i < 2 * kLay; i += kBT) (&acc[0][0])[i] = 0.0f;
 
     const int64_t W = tm.work();
     const int G = gridDim.x;
@@ -130,97 +153,160 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
     const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
     const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
     const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
-    double dotacc = 0.0;  // case (t % 16) of this thread in the store phase
-    // element-local node n -> patch-node offset (row + 17 col) from the
-    // element's (row, col) corner, and lattice-node offset within a layer
-    const int nodeOffP[4] = {1, 1 + kTN, kTN, 0};
+    double dotacc = 0.0;  // case t % 16 in the store phase
+    const float4 *p4 = reinterpret_cast<const float4 *>(p);
+
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
         int64_t rem = w;
         const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
         rem -= (int64_t)tz * tm.tilesR * tm.tilesC;
         const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
-        const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * tm.cz;
+        const int R0 = tr * kPO_R, C0 = tc * kPO_C, Z0 = tz * tm.cz;
         const int Zend = min(Z0 + tm.cz, NZ);
+        // this thread's staged float4s: patch node, float4 within the node, lattice node column
+        int64_t src[kLdPer];
+        int dst[kLdPer];
+#pragma unroll
+        for (int i = 0; i < kLdPer; ++i) {
+            const int f = t + i * kBT;
+            src[i] = -1;
+            dst[i] = -1;
+            if (f < kLd4) {
+                const int node = f / (kNV / 4), w4 = f % (kNV / 4);
+                const int rp = node % kPN_R, cp = node / kPN_R;
+                const int r = R0 - 1 + rp, c = C0 - 1 + cp;
+                dst[i] = node * (kNV / 4) + w4;
+                if (r >= 0 && r < NY && c >= 0 && c < NX) src[i] = ((int64_t)r + (int64_t)NY * c) * (kNV / 4) + w4;
+            }
+        }
+        auto load_layer = [&](int lay, float4 (&v)[kLdPer]) {
+#pragma unroll
+            for (int i = 0; i < kLdPer; ++i)
+                v[i] = (src[i] >= 0 && lay >= 0 && lay < NZ) ? p4[src[i] + pl * lay * (kNV / 4)]
+                                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        auto put_layer = [&](float *sl, const float4 (&v)[kLdPer]) {
+#pragma unroll
+            for (int i = 0; i < kLdPer; ++i)
+                if (dst[i] >= 0) reinterpret_cast<float4 *>(sl)[dst[i]] = v[i];
+        };
+        float4 pf[kLdPer];
+        // thread t < 128 stages the modulus of patch element column t, t < 153
+        // the flags of patch node t, one layer ahead of use (as the node layers)
+        int64_t eSrc = -1, fSrc = -1;
+        if (t < kPE_R * kPE_C) {
+            const int er = R0 - 1 + t % kPE_R, ec = C0 - 1 + t / kPE_R;
+            if (er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx) eSrc = er + (int64_t)gd.ny * ec;
+        }
+        if (t < kPN) {
+            const int r = R0 - 1 + t % kPN_R, c = C0 - 1 + t / kPN_R;
+            if (r >= 0 && r < NY && c >= 0 && c < NX) fSrc = r + (int64_t)NY * c;
+        }
+        auto load_E = [&](int el) {
+            return (eSrc >= 0 && el >= 0 && el < gd.nz) ? E[eSrc + ple * el] : 0.0f;
+        };
+        auto load_F = [&](int lay) {
+            return (fSrc >= 0 && lay >= 0 && lay < NZ) ? nflag[fSrc + pl * lay] : (uint8_t)0;
+        };
+        __syncthreads();  // previous work item is done with the stage
+        load_layer(Z0 - 1, pf);
+        put_layer(stage[(Z0 - 1) & 1], pf);
+        load_layer(Z0, pf);
+        put_layer(stage[Z0 & 1], pf);
+        load_layer(Z0 + 1, pf);  // in flight
+        if (t < kPE_R * kPE_C) Es[(Z0 - 1) & 1][t] = load_E(Z0 - 1);
+        if (t < kPN) {
+            Fs[(Z0 - 1) & 1][t] = load_F(Z0 - 1);
+            Fs[Z0 & 1][t] = load_F(Z0);
+        }
+        float eNext = load_E(Z0);
+        uint8_t fNext = load_F(Z0 + 1);
+        __syncthreads();
         for (int el = Z0 - 1; el < Zend; ++el) {
             const bool layerOk = el >= 0 && el < gd.nz;
-            if (layerOk) {
-#pragma unroll 2
-                for (int pp = 0; pp < 16; ++pp) {
-                    const int pair = wave * 16 + pp;
-                    const int ei = 2 * (pair & 7) + e, ej = pair >> 3;  // patch-relative element
-                    const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
-                    const bool ok = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
-                    const float Ee = ok ? E[er + (int64_t)gd.ny * ec + ple * el] : 0.0f;
-                    // inputs: g = 0 loads components x (K-step 0) and z (K-step 1)
-                    // of the 8 nodes, g = 1 component y (K-step 0)
-                    float wa[8], wb[8];
-                    if (Ee != 0.0f) {
-                        const int64_t base = (int64_t)(er + 1) + (int64_t)NY * ec + pl * el;  // node (x0, y0, z0)
+            const float *sb = stage[el & 1], *st = stage[(el + 1) & 1];
+#pragma unroll 1
+            for (int colour = 0; colour < 4; ++colour) {
+                if (!layerOk) break;
+                const int ci = colour & 1, cj = colour >> 1;
+                // 32 columns of the colour = 16 pairs, one per wavefront
+                const int ei = 2 * ((wave & 3) + 4 * e) + ci, ej = 2 * (wave >> 2) + cj;
+                const float Ee = Es[el & 1][ei + kPE_R * ej];
+                // element-local node n at patch node (ei + 1 - y_n) + 17 (ej + x_n)
+                const int nb = (ei + kPN_R * ej) * kNV + j;
+                float va[8], vb[4];
+                float ref_a, ref_b;
+                {
+                    const int ca = g;  // component of va: x (g = 0) or y (g = 1)
+                    // nodes 0-3 lower face (stage sb), 4-7 upper face (stage st)
+                    const int offn[4] = {1, 1 + kPN_R, kPN_R, 0};
 #pragma unroll
-                        for (int n = 0; n < 8; ++n) {
-                            const int xn = (n == 1 || n == 2 || n == 5 || n == 6);
-                            const int yn = (n == 2 || n == 3 || n == 6 || n == 7);
-                            const int zn = n >= 4;
-                            const int64_t gn = base - yn + (int64_t)NY * xn + pl * zn;
-                            const float *pn = p + (3 * gn) * NR + j;
-                            wa[n] = pn[g * NR];            // x (g = 0) or y (g = 1)
-                            wb[n] = g ? 0.0f : pn[2 * NR];  // z
-                        }
-                        const float a0 = wa[0], b0 = wb[0];
-#pragma unroll
-                        for (int n = 0; n < 8; ++n) {
-                            wa[n] = Ee * (wa[n] - a0);
-                            wb[n] = Ee * (wb[n] - b0);
-                        }
-                    } else {
-#pragma unroll
-                        for (int n = 0; n < 8; ++n) { wa[n] = 0.0f; wb[n] = 0.0f; }
+                    for (int n = 0; n < 4; ++n) {
+                        va[n] = sb[nb + offn[n] * kNV + ca * NR];
+                        va[n + 4] = st[nb + offn[n] * kNV + ca * NR];
+                        vb[n] = (g ? st : sb)[nb + offn[n] * kNV + 2 * NR];
                     }
-                    bf16x8 Bh0, Bl0, Bh1, Bl1;
-                    split8(wa, Bh0, Bl0);
-                    split8(wb, Bh1, Bl1);
-                    f32x16 D = {};
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bh0, D, 0, 0, 0);
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bl0, D, 0, 0, 0);
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[0], Bh0, D, 0, 0, 0);
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bh1, D, 0, 0, 0);
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bl1, D, 0, 0, 0);
-                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[1], Bh1, D, 0, 0, 0);
-                    const int tl = el + g;  // node layer of this lane's face
-                    if (Ee != 0.0f && tl >= Z0 && tl < Zend) {
-                        float *a = &acc[tl & 1][(ei + kTN * ej) * 3 * NR + j];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-#pragma unroll
-                            for (int m = 0; m < 4; ++m)
-                                atomicAdd(a + nodeOffP[m] * 3 * NR + c * NR, D[4 * c + m]);
-                    }
+                    ref_a = va[0];
+                    ref_b = g ? sb[nb + offn[0] * kNV + 2 * NR] : vb[0];
                 }
+                float da[8], db[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) da[n] = Ee * (va[n] - ref_a);
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    db[n] = Ee * (vb[n] - ref_b);
+                    db[n + 4] = 0.0f;
+                }
+                bf16x8 Bh0, Bl0, Bh1, Bl1;
+                split8(da, Bh0, Bl0);
+                split8(db, Bh1, Bl1);
+                f32x16 D = {};
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bh0, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bl0, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[0], Bh0, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bh1, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bl1, D, 0, 0, 0);
+                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[1], Bh1, D, 0, 0, 0);
+                const int tl = el + g;  // node layer of this lane's face
+                if (Ee != 0.0f && tl >= Z0 && tl < Zend) {
+                    float *a = &acc[tl & 1][nb];
+                    const int offn[4] = {1, 1 + kPN_R, kPN_R, 0};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) a[offn[m] * kNV + c * NR] += D[4 * c + m];
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if (el >= Z0) {  // node layer el is complete
-                float *a = acc[el & 1];
+                const float *a = acc[el & 1];
                 const int64_t lay = pl * el;
-                for (int f = t; f < kTO * kTO * 3 * NR; f += kBT) {
-                    const int node = f / (3 * NR), within = f % (3 * NR);
-                    const int oi = node % kTO, oj = node / kTO;
+#pragma unroll 1
+                for (int i = 0; i < kStPer; ++i) {
+                    const int f = t + i * kBT;
+                    if (f >= kPO_R * kPO_C * kNV) break;
+                    const int node = f / kNV, within = f % kNV;
+                    const int oi = node % kPO_R, oj = node / kPO_R;
                     const int r = R0 + oi, c = C0 + oj;
                     if (r < NY && c < NX) {
                         const int comp = within / NR;
-                        const int64_t gn = r + (int64_t)NY * c + lay;
-                        const uint8_t fl = nflag[gn];
-                        const float v = ((fl & kSolid) && !(fl & (1 << comp)))
-                                            ? a[((oi + 1) + kTN * (oj + 1)) * 3 * NR + within]
-                                            : 0.0f;
-                        const int64_t idx = 3 * gn * NR + within;
-                        q[idx] = v;
-                        dotacc += (double)v * (double)p[idx];
+                        const int pn = (oi + 1) + kPN_R * (oj + 1), sidx = pn * kNV + within;
+                        const uint8_t fl = Fs[el & 1][pn];
+                        const float v = ((fl & kSolid) && !(fl & (1 << comp))) ? a[sidx] : 0.0f;
+                        q[(r + (int64_t)NY * c + lay) * kNV + within] = v;
+                        dotacc += (double)v * (double)sb[sidx];
                     }
                 }
-                __syncthreads();
-                for (int i = t; i < kBS; i += kBT) a[i] = 0.0f;
-                __syncthreads();
             }
+            __syncthreads();  // stores read the slot; colour passes done with stage[el & 1]
+            for (int i = t; i < kLay; i += kBT) acc[el & 1][i] = 0.0f;
+            put_layer(stage[el & 1], pf);  // node layer el + 2
+            load_layer(el + 3, pf);
+            if (t < kPN) Fs[el & 1][t] = fNext;
+            fNext = load_F(el + 3);
+            if (t < kPE_R * kPE_C) Es[(el + 1) & 1][t] = eNext;  // element layer el + 1
+            eNext = load_E(el + 2);
+            __syncthreads();
         }
     }
     block_sum16(dotacc, partial + (int64_t)blockIdx.x * NR);
@@ -459,7 +545,7 @@ struct BatchState {
     std::vector<DevBuf<float>> lr, lx;   // levels >= 1
     DevBuf<double> b64, x64, r64;        // 16 case-major FP64 lattice vectors each
     DevBuf<double> part, scal;           // partial sums [blocks][16]; per-case scalars
-    DevBuf<float> Kc;                    // Ke, component-major 24 x 24
+    DevBuf<float> Kc;                    // Ke (24 x 24, column-major) in FP32
     DevBuf<BCtl> ctl;
     BCtl *hctl = nullptr;                // pinned mirror
     int kGrid = 1;                       // persistent grid of k_bkbyu
@@ -501,15 +587,10 @@ int batch_init(sgld_ctx *C) {
     SGLD_TRY(B->scal.alloc(4 * NR));
     SGLD_TRY(B->ctl.alloc(1));
     SGLD_HIP(hipHostMalloc((void **)&B->hctl, sizeof(BCtl), hipHostMallocDefault));
-    std::vector<float> kc(576);
-    for (int i = 0; i < 24; ++i)
-        for (int k = 0; k < 24; ++k) {
-            const int ni = i % 8, ai = i / 8, nk = k % 8, ak = k / 8;
-            kc[i * 24 + k] = (float)C->Ke[(3 * ni + ai) + 24 * (3 * nk + ak)];
-        }
+    std::vector<float> kc(C->Ke, C->Ke + 576);
     SGLD_TRY(B->Kc.upload(kc.data(), kc.size()));
     const Grid &g = C->lv[0].g;
-    const int tilesR = (int)((g.NY() + kTO - 1) / kTO), tilesC = (int)((g.NX() + kTO - 1) / kTO);
+    const int tilesR = (int)((g.NY() + kPO_R - 1) / kPO_R), tilesC = (int)((g.NX() + kPO_C - 1) / kPO_C);
     const int cz = 32, tilesZ = (int)((g.NZ() + cz - 1) / cz);
     B->tiles = TileMap{tilesR, tilesC, tilesZ, cz};
     int dev = 0, cus = 0;
