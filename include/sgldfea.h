@@ -171,6 +171,48 @@ int sgld_ctx_kbyu_batched_device(sgld_ctx *ctx, int nrhs, const double *dU, doub
  * average kernel time (hipEvents on the context stream). */
 int sgld_ctx_bench_kbyu_batched(sgld_ctx *ctx, int reps, double *msPerLaunch);
 
+/* Compliance sensitivities after a solve (TopOpti_GlobalVolumeConstraint.m:
+ * 113-118 with TopOpti_DeMaterialInterpolation.m:5), fused with the unit
+ * compliance of TopOpti_ComputeUnitCompliance.m: per element
+ *   ce_e = u_e^T Ke u_e,  dc_e = -penal (E0 - Emin) x_e^(penal-1) ce_e scale,
+ * and compliance = scale sum_e E_e ce_e (E_e as set by sgld_ctx_set_modulus).
+ * The reference normalises with scale = 1 / complianceSolid_.  U: numDOF,
+ * x, dc: numElements (compressed order). */
+int sgld_ctx_sensitivity(sgld_ctx *ctx, const double *U, const double *x, double penal, double E0,
+                         double Emin, double scale, double *dc, double *compliance);
+/* Same with U, x and dc in device memory. */
+int sgld_ctx_sensitivity_device(sgld_ctx *ctx, const double *dU, const double *dx, double penal,
+                                double E0, double Emin, double scale, double *ddc,
+                                double *compliance);
+/* Autotuner (no counterpart in the reference): times the K.u variants of this
+ * mesh (patch-tiled mode-space with z-chunks of 8, 16 or 32 node layers, and
+ * the node-centric dense kernel; `reps` launches each, FP64 and FP32) and the
+ * two span-2 restrictions (on FP32 V-cycles, once assembled), and keeps the
+ * fastest for the mixed solver (cost = FP32 K.u + FP64 K.u / 8).  report[8]:
+ * [0] K.u variant (1 tiled, 0 dense), [1] z-chunk, [2] FP64 K.u ms, [3] FP32
+ * K.u ms, [4] restriction (1 tiled, 0 node-centric), [5], [6] V-cycle ms of
+ * the two restrictions. */
+int sgld_ctx_autotune(sgld_ctx *ctx, int reps, double *report);
+/* Pins the schedule the autotuner chooses: kbyuVariant 1 = patch-tiled
+ * mode-space K.u with z-chunks of zChunk node layers, 0 = node-centric dense
+ * K.u; restrictTiled -1 = by level size, 0 = node-centric, 1 = LDS-tiled
+ * span-2 restriction.  No counterpart in the reference. */
+int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restrictTiled);
+/* n independent problems (one context each, distinct) solved in one replayed
+ * graph (no counterpart in the reference, whose topology-optimisation loop
+ * solves one design at a time).  Each instance's PCG iterations are captured
+ * once, `batch` (even) iterations long, as a child graph; the parent graph
+ * of all instances is launched until every instance has converged, with the
+ * host steps (convergence check; for SGLD_PREC_MIXED the reliable updates)
+ * between launches; converged instances' kernels exit at once.  dF[i], dU[i]:
+ * device vectors of instance i (U in: initial guess).  its, relres: per
+ * instance (FP64 true relative residual); latency[i] (optional): seconds from
+ * the call until instance i was seen converged.  Returns SGLD_ERR_MAXIT if
+ * any instance missed tol. */
+int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF, double *const *dU,
+                          double tol, int maxIT, int precision, int batch, int *its,
+                          double *relres, double *latency);
+
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
  * last assemble, [5] K.u launches, [6] average K.u kernel milliseconds

@@ -720,6 +720,13 @@ __global__ void k_convert(To *__restrict__ dst, const Ti *__restrict__ src, doub
 
 // v *= f (in place)
 template <class T>
+__global__ void k_fill(T *__restrict__ v, T val, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = val;
+}
+
+template <class T>
 __global__ void k_scale(T *v, T f, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -977,6 +984,46 @@ __global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u
     ce[i] = q;
 }
 
+// Compliance sensitivity (TopOpti_GlobalVolumeConstraint.m:113-118 with
+// TopOpti_DeMaterialInterpolation.m:5): ce = u_e^T Ke u_e,
+// dc_e = -penal (E0 - Emin) x_e^(penal-1) ce * scale, and the partial of the
+// compliance sum_e E_e ce (E_e the modulus on the lattice).
+__global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ u, Grid gd,
+                                                     const int32_t *__restrict__ emap, int64_t ne,
+                                                     const double *__restrict__ x, const double *__restrict__ E,
+                                                     double penal, double dE, double scale,
+                                                     double *__restrict__ dc, double *__restrict__ partial) {
+    double cacc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = emap ? emap[i] : i;
+        const int er = (int)(v % gd.ny), ec = (int)((v / gd.ny) % gd.nx), el = (int)(v / ((int64_t)gd.ny * gd.nx));
+        const int64_t NY = gd.NY(), pl = gd.NY() * gd.NX();
+        double ue[24];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+            const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+            const int zk = (k >= 4);
+            const int64_t g = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
+            ue[3 * k] = u[3 * g];
+            ue[3 * k + 1] = u[3 * g + 1];
+            ue[3 * k + 2] = u[3 * g + 2];
+        }
+        double q = 0.0;
+#pragma unroll
+        for (int r = 0; r < 24; ++r) {
+            double y = 0.0;
+#pragma unroll
+            for (int c = 0; c < 24; ++c) y += cKe64[r + 24 * c] * ue[c];
+            q += ue[r] * y;
+        }
+        dc[i] = -penal * dE * pow(x[i], penal - 1.0) * q * scale;
+        cacc += E[v] * q;
+    }
+    const double s = block_sum(cacc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s * scale;
+}
+
 }  // namespace
 }  // namespace sgld
 
@@ -1094,7 +1141,8 @@ template <class T>
 int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int nb = blocks_for(Cc.g.nodes(), 256);
-    if (Cc.span == 2 && F.g.nodes() >= (int64_t)1 << 16) {
+    const bool tiled = C->restrictTiled < 0 ? F.g.nodes() >= (int64_t)1 << 16 : C->restrictTiled != 0;
+    if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
         const int czc = 4, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
         hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(tilesR * tilesC * tilesZ), dim3(256), 0, C->s, rf,
@@ -1174,46 +1222,168 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *z, const T *pold, T *pnew, T *
     return SGLD_OK;
 }
 
-// PCG on lattice vectors.  b, x in T; x0zero skips the initial K.u.
-template <class T>
-int run_pcg(sgld_ctx *C, const T *b, T *x, T *r, T *z, T *p0, T *p1, T *q, bool x0zero, double tol,
-            int maxit, int *itsOut, double *resOut, int *doneOut) {
+int true_residual(sgld_ctx *C, double *res, double *normr);
+
+// A solve as a resumable state machine: pcg_begin enqueues and runs the set-up,
+// pcg_enqueue_batch queues `batch` iterations plus the copy of the device
+// control block to its host mirror (stream-capturable: kernels and one
+// memcpy, fixed buffers), pcg_after_batch runs the host decisions once the
+// batch has completed.  A single solve loops enqueue / sync / after; the
+// multi-instance driver captures each instance's batch once as a graph.
+struct SolveState {
+    int precision = SGLD_PREC_FP64;
+    double tol = 1e-8;
+    int maxit = 500, batch = 8;
+    bool finished = false;
+    int its = 0, done = 0, updates = 0;
+    double res = 0.0;
+    // mixed precision
+    double scale = 1.0, normb = 0.0, delta = 0.1;
+    int itsAtUpdate = 0;
+    static constexpr int kMaxStretch = 96;  // force a reliable update if FP32 stalls
+};
+
+int fp64_begin(sgld_ctx *C, SolveState &S) {
+    // Solving_PreconditionedConjugateGradientSolver.m:7-12 on lattice vectors (x holds x0)
     const int64_t n = C->nDof();
     const int nbv = grid_blocks(n);
     *C->hctl = Ctl{};
-    C->hctl->tol = tol;
-    C->hctl->maxit = maxit;
+    C->hctl->tol = S.tol;
+    C->hctl->maxit = S.maxit;
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    hipLaunchKernelGGL(k_dot<T>, dim3(nbv), dim3(256), 0, C->s, b, b, n, slot(C, 3), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->b.p, n, slot(C, 3), nullptr);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 3), nbv, FIN_NORMB));
-    if (!x0zero) SGLD_TRY(enqueue_kbyu<T>(C, x, q, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<T>, dim3(nbv), dim3(256), 0, C->s, b, x0zero ? nullptr : q, r, n);
+    SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, n);
     SGLD_HIP(hipGetLastError());
     int nb = 0;
-    SGLD_TRY(enqueue_vcycle<T>(C, r, z, slot(C, 2), nullptr, &nb));
+    SGLD_TRY(enqueue_vcycle<double>(C, C->r.p, C->z.p, slot(C, 2), nullptr, &nb));
     SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));  // rho = z.r, beta = 0: first p = z
+    hipLaunchKernelGGL(k_fill<double>, dim3(nbv), dim3(256), 0, C->s, C->p.p, 0.0, n);  // beta 0 x p
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
     SGLD_HIP(hipStreamSynchronize(C->s));
     if (!(C->hctl->normb > 0.0)) {  // F = 0: solution 0, nothing to iterate
-        *itsOut = 0;
-        *resOut = 0.0;
-        *doneOut = 1;
+        S.finished = true;
+        S.its = 0;
+        S.res = 0.0;
+        S.done = 1;
+    }
+    return SGLD_OK;
+}
+
+int mixed_begin(sgld_ctx *C, SolveState &S) {
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    double res = 0, normr = 0;
+    SGLD_TRY(true_residual(C, &res, &normr));
+    S.its = 0;
+    S.updates = 0;
+    S.res = res;
+    if (!(normr > 0.0) || res < S.tol) {
+        S.finished = true;
+        S.done = 1;
         return SGLD_OK;
     }
-    const int batch = 8;  // even: the p buffers alternate per iteration
-    while (true) {
-        for (int i = 0; i < batch; i += 2) {
-            SGLD_TRY(enqueue_iteration<T>(C, x, r, z, p0, p1, q));
-            SGLD_TRY(enqueue_iteration<T>(C, x, r, z, p1, p0, q));
+    S.normb = normr / res;
+    // 0.1: on the 125 M-DOF ellipsoid domain 1e-2 let the FP32 recurrence drift (681 vs 397
+    // iterations); on the 12.8 M-DOF cuboid it costs ~8 % over 1e-2.  1e-4 stagnates.
+    S.delta = 0.1;
+    if (const char *env = std::getenv("SGLD_MIXED_DELTA")) S.delta = std::atof(env);
+    S.scale = normr;
+    S.itsAtUpdate = 0;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
+                       1.0 / S.scale, n);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, n);
+    SGLD_HIP(hipGetLastError());
+    *C->hctl = Ctl{};
+    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
+    C->hctl->maxit = S.maxit;
+    C->hctl->normb = 1.0;
+    SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
+    int nb = 0;
+    SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
+    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, n);  // beta 0 x p
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+int pcg_begin(sgld_ctx *C, SolveState &S) {
+    S.finished = false;
+    return S.precision == SGLD_PREC_FP64 ? fp64_begin(C, S) : mixed_begin(C, S);
+}
+
+// `batch` (even) iterations alternating the two direction buffers, starting
+// from the first (the latest direction is kept there between batches).
+int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
+    for (int i = 0; i < S.batch; i += 2) {
+        if (S.precision == SGLD_PREC_FP64) {
+            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->z.p, C->p.p, C->p2.p, C->q.p));
+            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->z.p, C->p2.p, C->p.p, C->q.p));
+        } else {
+            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->z32.p, C->p32.p, C->p32b.p, C->q32.p));
+            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->z32.p, C->p32b.p, C->p32.p, C->q32.p));
         }
-        SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
-        SGLD_HIP(hipStreamSynchronize(C->s));
-        if (C->hctl->done) break;
     }
-    *itsOut = C->hctl->its;
-    *resOut = C->hctl->res;
-    *doneOut = C->hctl->done;
+    SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
+    return SGLD_OK;
+}
+
+// Host step after a completed batch (the control block is in C->hctl).
+int pcg_after_batch(sgld_ctx *C, SolveState &S) {
+    const Ctl st = *C->hctl;
+    if (S.precision == SGLD_PREC_FP64) {
+        if (st.done) {
+            S.finished = true;
+            S.its = st.its;
+            S.res = st.res;
+            S.done = st.done;
+        }
+        return SGLD_OK;
+    }
+    // Mixed: the FP32 Krylov iteration on the residual scaled to unit norm;
+    // when its recursive residual has dropped by delta (or after kMaxStretch
+    // iterations), fold the FP32 correction into the FP64 solution and
+    // replace the residual by the FP64 true residual (reliable update).
+    const int64_t n = C->nDof();
+    const int nbv = grid_blocks(n);
+    // the batch alternates p32 -> p32b -> p32 ...: after an odd number of its
+    // iterations the latest direction is in p32b; the next batch starts from p32
+    if ((st.its - S.its) % 2)
+        hipLaunchKernelGGL(k_copy<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (const float *)C->p32b.p, n);
+    S.its = st.its;
+    if (!st.done && S.its - S.itsAtUpdate < SolveState::kMaxStretch) return SGLD_OK;
+    S.itsAtUpdate = S.its;
+    hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, S.scale, n);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, n);
+    SGLD_HIP(hipGetLastError());
+    double res = 0, normr = 0;
+    SGLD_TRY(true_residual(C, &res, &normr));
+    ++S.updates;
+    S.res = res;
+    if (res < S.tol || st.done == 2 || !(normr > 0.0)) {
+        S.finished = true;
+        S.done = res < S.tol ? 1 : 2;
+        return SGLD_OK;
+    }
+    const double f = S.scale / normr;
+    S.scale = normr;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
+                       1.0 / S.scale, n);
+    hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (float)f, n);
+    SGLD_HIP(hipGetLastError());
+    *C->hctl = st;
+    C->hctl->done = 0;
+    C->hctl->rho = st.rho * f * f;
+    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
+    SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
+    // z = M r and beta for the replaced residual, then continue from p.  After a
+    // forced update (FP32 stalled mid-stretch) rho already belongs to the
+    // recursive residual, so the direction restarts (beta = 0).
+    int nb = 0;
+    SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), C->ctl.p, &nb));
+    SGLD_TRY(fin(C, slot(C, 2), nb, st.done ? FIN_RHO : FIN_RHO0));
     return SGLD_OK;
 }
 
@@ -1223,10 +1393,15 @@ int upload_flags(Level &l) {
     return SGLD_OK;
 }
 
-// FP64 solve on lattice vectors b -> x (x holds x0).
-int solve_fp64(sgld_ctx *C, double tol, int maxit, int *its, double *res, int *done) {
-    return run_pcg<double>(C, C->b.p, C->x.p, C->r.p, C->z.p, C->p.p, C->p2.p, C->q.p, false, tol,
-                           maxit, its, res, done);
+// One solve on lattice vectors b -> x (x holds x0).
+int pcg_solve(sgld_ctx *C, SolveState &S) {
+    SGLD_TRY(pcg_begin(C, S));
+    while (!S.finished) {
+        SGLD_TRY(pcg_enqueue_batch(C, S));
+        SGLD_HIP(hipStreamSynchronize(C->s));
+        SGLD_TRY(pcg_after_batch(C, S));
+    }
+    return SGLD_OK;
 }
 
 // True FP64 relative residual ||b - K x|| / ||b||; leaves b - Kx in C->r.
@@ -1235,85 +1410,6 @@ int true_residual(sgld_ctx *C, double *res, double *normr) {
     SGLD_TRY(residual64(C, C->b.p, C->x.p, C->r.p, &rr, &bb));
     *normr = std::sqrt(rr);
     *res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-    return SGLD_OK;
-}
-
-// Mixed precision with reliable updates.  The Krylov iteration runs entirely
-// in FP32 (vectors, K.u, V-cycle) with FP64 dot products, alpha/beta and
-// coarsest inverse, on the residual scaled to unit norm.  Whenever the FP32
-// recursive residual has dropped by `delta` since the last update, or
-// predicts the target, the FP32 correction is folded into the FP64 solution,
-// the FP64 true residual r = f - K u replaces the recursive one (rescaled),
-// and the iteration continues with the same search direction (p and rho are
-// rescaled with r, then z = M r and beta are recomputed), so the Krylov
-// space is not restarted.  The solve stops on the FP64 true residual.
-int solve_mixed(sgld_ctx *C, double tol, int maxit, int *itsTot, double *resOut, int *updates) {
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
-    double res = 0, normr = 0;
-    SGLD_TRY(true_residual(C, &res, &normr));
-    *itsTot = 0;
-    *updates = 0;
-    *resOut = res;
-    if (!(normr > 0.0) || res < tol) return SGLD_OK;
-    const double normb = normr / res;
-    // 0.1: on the 125 M-DOF ellipsoid domain 1e-2 let the FP32 recurrence drift (681 vs 397
-    // iterations); on the 12.8 M-DOF cuboid it costs ~8 % over 1e-2.  1e-4 stagnates.
-    double delta = 0.1;
-    if (const char *env = std::getenv("SGLD_MIXED_DELTA")) delta = std::atof(env);
-    auto inner_tol = [&](double scale) { return std::max(delta, 0.5 * tol * normb / scale); };
-    double scale = normr;
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
-                       1.0 / scale, n);
-    SGLD_HIP(hipMemsetAsync(C->x32.p, 0, sizeof(float) * n, C->s));
-    *C->hctl = Ctl{};
-    C->hctl->tol = inner_tol(scale);
-    C->hctl->maxit = maxit;
-    C->hctl->normb = 1.0;
-    SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    int nb = 0;
-    SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
-    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));
-    float *P[2] = {C->p32.p, C->p32b.p};
-    int its = 0;  // completed iterations; the latest direction is in P[its % 2]
-    int itsAtUpdate = 0;
-    const int batch = 8, maxStretch = 96;  // force an update if FP32 stalls
-    while (true) {
-        for (int i = 0; i < batch; ++i)
-            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->z32.p, P[(its + i) % 2],
-                                              P[(its + i + 1) % 2], C->q32.p));
-        SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
-        SGLD_HIP(hipStreamSynchronize(C->s));
-        const Ctl st = *C->hctl;
-        its = st.its;
-        if (!st.done && its - itsAtUpdate < maxStretch) continue;
-        itsAtUpdate = its;
-        // reliable update: fold the FP32 correction into the FP64 solution
-        hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, scale, n);
-        SGLD_HIP(hipGetLastError());
-        SGLD_HIP(hipMemsetAsync(C->x32.p, 0, sizeof(float) * n, C->s));
-        SGLD_TRY(true_residual(C, &res, &normr));
-        ++*updates;
-        if (res < tol || st.done == 2 || !(normr > 0.0)) break;
-        const double f = scale / normr;
-        scale = normr;
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
-                           1.0 / scale, n);
-        hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, P[its % 2], (float)f, n);
-        SGLD_HIP(hipGetLastError());
-        *C->hctl = st;
-        C->hctl->done = 0;
-        C->hctl->rho = st.rho * f * f;
-        C->hctl->tol = inner_tol(scale);
-        SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-        // z = M r and beta for the replaced residual, then continue from p.
-        // After a forced update (FP32 stalled mid-stretch) rho already belongs
-        // to the recursive residual, so the direction restarts (beta = 0).
-        SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), C->ctl.p, &nb));
-        SGLD_TRY(fin(C, slot(C, 2), nb, st.done ? FIN_RHO : FIN_RHO0));
-    }
-    *itsTot = its;
-    *resOut = res;
     return SGLD_OK;
 }
 
@@ -1513,7 +1609,8 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p64, k_kbyu_tiled<double>, 256, 0));
         SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p32, k_kbyu_tiled<float>, 256, 0));
         per = std::max(1, std::min(p64, p32));
-        C->tiledGrid = (int)std::min<int64_t>(std::min<int64_t>(tm.work(), (int64_t)ncu * per), kRedBlocks);
+        C->tiledCap = (int)std::min<int64_t>((int64_t)ncu * per, kRedBlocks);
+        C->tiledGrid = (int)std::min<int64_t>(tm.work(), C->tiledCap);
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
     *out = C.release();
@@ -1758,14 +1855,13 @@ extern "C" int sgld_ctx_pcg_device(sgld_ctx *C, const double *dF, double *dU, do
     const double t0 = now_s();
     SGLD_TRY(to_lattice(C, dF, C->b.p));
     SGLD_TRY(to_lattice(C, dU, C->x.p));
-    int its = 0, done = 0, outer = 0;
-    double res = 0, tres = 0, normr = 0;
-    if (precision == SGLD_PREC_FP64) {
-        SGLD_TRY(solve_fp64(C, tol, maxIT, &its, &res, &done));
-    } else {
-        SGLD_TRY(solve_mixed(C, tol, maxIT, &its, &res, &outer));
-        done = res < tol ? 1 : 2;
-    }
+    SolveState S;
+    S.precision = precision;
+    S.tol = tol;
+    S.maxit = maxIT;
+    SGLD_TRY(pcg_solve(C, S));
+    const int its = S.its, done = S.done, outer = S.updates;
+    double tres = 0, normr = 0;
     SGLD_TRY(from_lattice(C, C->x.p, dU));
     SGLD_HIP(hipStreamSynchronize(C->s));
     C->stats[0] = now_s() - t0;
@@ -1860,5 +1956,233 @@ extern "C" int sgld_ctx_bench_kbyu(sgld_ctx *C, int precision, int reps, double 
 extern "C" int sgld_ctx_stats(const sgld_ctx *C, double *out) {
     SGLD_ARG(C && out, "sgld:arg: null argument");
     for (int i = 0; i < 8; ++i) out[i] = C->stats[i];
+    return SGLD_OK;
+}
+
+// ------------------------- sensitivities, autotuning ------------------------
+extern "C" int sgld_ctx_sensitivity_device(sgld_ctx *C, const double *dU, const double *dx, double penal,
+                                           double E0, double Emin, double scale, double *ddc, double *compliance) {
+    SGLD_ARG(C && dU && dx && ddc, "sgld:arg: null argument");
+    SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
+    SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
+    const int64_t ne = C->lv[0].numElements;
+    SGLD_TRY(to_lattice(C, dU, C->z.p));
+    const int nb = grid_blocks(ne);
+    hipLaunchKernelGGL(k_sensitivity, dim3(nb), dim3(256), 0, C->s, C->z.p, C->lv[0].g,
+                       C->full ? nullptr : C->eleMap.p, ne, dx, C->E.p, penal, E0 - Emin, scale, ddc, slot(C, 3));
+    SGLD_HIP(hipGetLastError());
+    std::vector<double> part(nb);
+    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 3), sizeof(double) * nb, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    double c = 0.0;
+    for (double v : part) c += v;
+    if (compliance) *compliance = c;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_sensitivity(sgld_ctx *C, const double *U, const double *x, double penal, double E0,
+                                    double Emin, double scale, double *dc, double *compliance) {
+    SGLD_ARG(C && U && x && dc, "sgld:arg: null argument");
+    const int64_t nc = 3 * C->lv[0].numNodes, ne = C->lv[0].numElements;
+    DevBuf<double> dU, dx, ddc;
+    SGLD_TRY(dU.upload(U, nc));
+    SGLD_TRY(dx.upload(x, ne));
+    SGLD_TRY(ddc.alloc(ne));
+    SGLD_TRY(sgld_ctx_sensitivity_device(C, dU.p, dx.p, penal, E0, Emin, scale, ddc.p, compliance));
+    SGLD_HIP(hipMemcpy(dc, ddc.p, sizeof(double) * ne, hipMemcpyDeviceToHost));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
+    SGLD_ARG(C && report && reps > 0, "sgld:arg: bad arguments");
+    SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
+    double best = 1e300, best64 = 0, best32 = 0;
+    int bestCz = C->tiles.cz, bestVariant = C->kbyuVariant;
+    const int variant0 = C->kbyuVariant;
+    const TileMap tm0 = C->tiles;
+    const int czs[3] = {8, 16, 32};
+    for (int v = 0; v < (variant0 == 1 ? 4 : 1); ++v) {
+        if (variant0 == 1) {
+            if (v < 3) {
+                C->kbyuVariant = 1;
+                C->tiles = tm0;
+                C->tiles.cz = czs[v];
+                C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + czs[v] - 1) / czs[v]);
+                C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
+            } else {
+                C->kbyuVariant = 0;
+            }
+        }
+        double t64 = 0, t32 = 0;
+        SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_FP64, reps, &t64));
+        SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_MIXED, reps, &t32));
+        // the mixed solver spends its iterations in FP32 K.u; FP64 K.u runs once per reliable update
+        const double cost = t32 + t64 / 8.0;
+        if (cost < best) {
+            best = cost;
+            best64 = t64;
+            best32 = t32;
+            bestVariant = C->kbyuVariant;
+            bestCz = C->tiles.cz;
+        }
+    }
+    C->kbyuVariant = bestVariant;
+    if (bestVariant == 1) {
+        C->tiles = tm0;
+        C->tiles.cz = bestCz;
+        C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + bestCz - 1) / bestCz);
+        C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
+    }
+    // span-2 restriction variant, timed on FP32 V-cycles (needs the assembled hierarchy)
+    int bestR = C->restrictTiled;
+    double tv[2] = {0, 0};
+    if (C->assembled) {
+        for (int rt = 0; rt < 2; ++rt) {
+            C->restrictTiled = rt;
+            int nb = 0;
+            SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
+            SGLD_HIP(hipEventRecord(C->e0, C->s));
+            for (int i = 0; i < reps; ++i) SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
+            SGLD_HIP(hipEventRecord(C->e1, C->s));
+            SGLD_HIP(hipEventSynchronize(C->e1));
+            float ms = 0;
+            SGLD_HIP(hipEventElapsedTime(&ms, C->e0, C->e1));
+            tv[rt] = ms / reps;
+        }
+        bestR = tv[1] < tv[0] ? 1 : 0;
+    }
+    C->restrictTiled = bestR;
+    report[0] = bestVariant;
+    report[1] = bestVariant == 1 ? bestCz : 0;
+    report[2] = best64;
+    report[3] = best32;
+    report[4] = bestR;
+    report[5] = tv[0];
+    report[6] = tv[1];
+    report[7] = 0;
+    return SGLD_OK;
+}
+
+// --------------------- many instances, one replayed graph --------------------
+extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF, double *const *dU,
+                                     double tol, int maxIT, int precision, int batch, int *its, double *relres,
+                                     double *latency) {
+    SGLD_ARG(ctxs && dF && dU && its && relres && n >= 1, "sgld:arg: null argument or n < 1");
+    SGLD_ARG(tol > 0 && maxIT >= 1, "sgld:arg: need tol > 0 and maxIT >= 1");
+    SGLD_ARG(precision == SGLD_PREC_FP64 || precision == SGLD_PREC_MIXED, "sgld:arg: unknown precision mode");
+    SGLD_ARG(batch >= 2 && batch <= 64 && batch % 2 == 0, "sgld:arg: batch must be even, 2..64");
+    for (int i = 0; i < n; ++i) {
+        SGLD_TRY(check_ready(ctxs[i]));
+        SGLD_ARG(dF[i] && dU[i], "sgld:arg: null vector of instance %d", i);
+        for (int k = 0; k < i; ++k) SGLD_ARG(ctxs[k] != ctxs[i], "sgld:arg: instance %d repeats a context", i);
+    }
+    SGLD_HIP(hipDeviceSynchronize());
+    const double t0 = now_s();
+    std::vector<SolveState> S(n);
+    for (int i = 0; i < n; ++i) {
+        sgld_ctx *C = ctxs[i];
+        S[i].precision = precision;
+        S[i].tol = tol;
+        S[i].maxit = maxIT;
+        S[i].batch = batch;
+        SGLD_TRY(to_lattice(C, dF[i], C->b.p));
+        SGLD_TRY(to_lattice(C, dU[i], C->x.p));
+        SGLD_TRY(pcg_begin(C, S[i]));
+    }
+    for (int i = 0; i < n; ++i) {
+        SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
+        if (latency) latency[i] = S[i].finished ? now_s() - t0 : -1.0;
+    }
+    // each unfinished instance's batch of iterations becomes a child of one graph
+    struct Graphs {
+        hipGraph_t parent = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraph_t> sub;
+        hipStream_t stream = nullptr;
+        ~Graphs() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (parent) (void)hipGraphDestroy(parent);
+            for (auto g : sub)
+                if (g) (void)hipGraphDestroy(g);
+            if (stream) (void)hipStreamDestroy(stream);
+        }
+    } G;
+    G.sub.assign(n, nullptr);
+    SGLD_HIP(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+    SGLD_HIP(hipGraphCreate(&G.parent, 0));
+    int live = 0;
+    for (int i = 0; i < n; ++i) {
+        if (S[i].finished) continue;
+        sgld_ctx *C = ctxs[i];
+        SGLD_HIP(hipStreamBeginCapture(C->s, hipStreamCaptureModeThreadLocal));
+        const int rc = pcg_enqueue_batch(C, S[i]);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(C->s, &g);
+        if (rc != SGLD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SGLD_HIP(e);
+        G.sub[i] = g;
+        hipGraphNode_t node;
+        SGLD_HIP(hipGraphAddChildGraphNode(&node, G.parent, nullptr, 0, g));
+        ++live;
+    }
+    int replays = 0;
+    if (live) {
+        SGLD_HIP(hipGraphInstantiate(&G.exec, G.parent, nullptr, nullptr, 0));
+        while (true) {
+            SGLD_HIP(hipGraphLaunch(G.exec, G.stream));
+            SGLD_HIP(hipStreamSynchronize(G.stream));
+            ++replays;
+            bool any = false;
+            for (int i = 0; i < n; ++i) {
+                if (S[i].finished) continue;
+                SGLD_TRY(pcg_after_batch(ctxs[i], S[i]));
+                SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
+                if (S[i].finished) {
+                    if (latency) latency[i] = now_s() - t0;
+                } else {
+                    any = true;
+                }
+            }
+            if (!any) break;
+        }
+    }
+    int allConv = 1;
+    for (int i = 0; i < n; ++i) {
+        sgld_ctx *C = ctxs[i];
+        SGLD_TRY(from_lattice(C, C->x.p, dU[i]));
+        double tres = 0, normr = 0;
+        SGLD_TRY(true_residual(C, &tres, &normr));
+        its[i] = S[i].its;
+        relres[i] = tres;
+        C->stats[1] = S[i].its;
+        C->stats[2] = tres;
+        C->stats[3] = S[i].updates;
+        if (S[i].done != 1) allConv = 0;
+    }
+    for (int i = 0; i < n; ++i) ctxs[i]->stats[0] = now_s() - t0;
+    if (!allConv) {
+        set_error("sgld:maxit: not every instance reached tol %.3e within %d iterations", tol, maxIT);
+        return SGLD_ERR_MAXIT;
+    }
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, int restrictTiled) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    SGLD_ARG(kbyuVariant == 0 || kbyuVariant == 1, "sgld:arg: kbyuVariant must be 0 (dense) or 1 (tiled)");
+    SGLD_ARG(restrictTiled >= -1 && restrictTiled <= 1, "sgld:arg: restrictTiled must be -1, 0 or 1");
+    if (kbyuVariant == 1) {
+        double km[9];
+        SGLD_ARG(mode_coefficients(C->Ke, km), "sgld:arg: the tiled K.u needs the isotropic-cube Ke structure");
+        SGLD_ARG(zChunk >= 2 && zChunk <= 256, "sgld:arg: zChunk must be in 2..256");
+        C->tiles.cz = zChunk;
+        C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + zChunk - 1) / zChunk);
+        C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
+    }
+    C->kbyuVariant = kbyuVariant;
+    C->restrictTiled = restrictTiled;
     return SGLD_OK;
 }

@@ -59,8 +59,10 @@ struct sgld_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
+    int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     sgld::TileMap tiles{1, 1, 1, 16};
-    int tiledGrid = 1;
+    int tiledGrid = 1;    // persistent grid of the tiled K.u = min(work items, tiledCap)
+    int tiledCap = 1;     // resident blocks of the tiled K.u on the device
     double stats[8] = {0};
     sgld::BatchState *batch = nullptr;  // multi-RHS buffers, allocated on first use
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }

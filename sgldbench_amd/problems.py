@@ -136,6 +136,38 @@ def cfg3(n=256, nrhs=16, levels=6, seed=CFG3_SEED):
                 directions=d, num_levels=levels, omega=0.6, tol=1e-8, max_it=500)
 
 
+CFG4_SEED = 20244
+CFG4_DIMS = (32, 48, 64, 96, 128)
+
+
+def levels_for(dims, max_levels=5):
+    """Multigrid levels of an instance: every axis divisible by 2^(L-1),
+    at most max_levels (48 and 96 = 3 2^k allow 5)."""
+    L = 1
+    while L < max_levels and all(d % (2 ** L) == 0 for d in dims):
+        L += 1
+    return L
+
+
+def cfg4(n_inst=32, seed=CFG4_SEED, dims_set=CFG4_DIMS, penal=3.0):
+    """Uneven batch: n_inst cuboids with per-axis sizes drawn from dims_set
+    (seeded), Cantilever 3 supports and load, each with its own seeded
+    Gaussian random density in [0.001, 1] (sigma = 6 voxels scaled by the
+    instance's mean edge / 256, at least 1), SIMP E = 1e-9 + x^penal."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        nx, ny, nz = (int(v) for v in rng.choice(dims_set, 3))
+        vox = cuboid(nx, ny, nz)
+        F, fixed = cantilever3(vox)
+        sigma = max(1.0, 6.0 * (nx + ny + nz) / 3.0 / 256.0)
+        x = gaussian_random_density((ny, nx, nz), seed * 1000 + i, sigma=sigma).ravel(order="F")
+        out.append(dict(name="cuboid%dx%dx%d" % (nx, ny, nz), voxels=vox, x=x, E=simp_modulus(x, penal),
+                        F=F, fixed=fixed, num_levels=levels_for((nx, ny, nz)), omega=0.6, tol=1e-8,
+                        max_it=500, penal=penal))
+    return out
+
+
 CFG2_SEED = 20277  # fill close to the ~30 % the workload describes (seed scan at 1/4 scale)
 
 

@@ -286,8 +286,54 @@ class FEASolver:
         check(lib().sgld_ctx_bench_kbyu_batched(self._h, reps, C.byref(ms)))
         return ms.value
 
+    def sensitivity(self, U, x, penal=3.0, E0=1.0, Emin=1e-9, scale=1.0):
+        """dc = -TopOpti_DeMaterialInterpolation(x) .* ceList * scale and the
+        compliance scale * E . ceList (TopOpti_GlobalVolumeConstraint.m:113-118)."""
+        U = _f64(U).ravel()
+        x = _f64(x).ravel()
+        assert U.size == self.numDOFs and x.size == self.numElements
+        dc = np.zeros(self.numElements)
+        c = C.c_double(0)
+        check(lib().sgld_ctx_sensitivity(self._h, U, x, penal, E0, Emin, scale, dc, C.byref(c)))
+        return dc, c.value
+
+    def sensitivity_device(self, dU_ptr, dx_ptr, ddc_ptr, penal=3.0, E0=1.0, Emin=1e-9, scale=1.0):
+        c = C.c_double(0)
+        check(lib().sgld_ctx_sensitivity_device(self._h, C.c_void_p(dU_ptr), C.c_void_p(dx_ptr), penal, E0,
+                                                Emin, scale, C.c_void_p(ddc_ptr), C.byref(c)))
+        return c.value
+
+    def autotune(self, reps=10):
+        """Picks the fastest K.u variant / z-chunk and span-2 restriction for
+        this mesh (kept in the context)."""
+        out = np.zeros(8)
+        check(lib().sgld_ctx_autotune(self._h, reps, out))
+        return dict(kbyu_variant="tiled" if out[0] == 1 else "dense", z_chunk=int(out[1]),
+                    kbyu_ms_fp64=out[2], kbyu_ms_fp32=out[3],
+                    restriction="tiled" if out[4] == 1 else "node-centric", vcycle_ms=(out[5], out[6]))
+
+    def set_schedule(self, kbyu_variant="tiled", z_chunk=16, restriction=None):
+        """Pins a K.u / restriction schedule (what autotune picks)."""
+        rt = {None: -1, "node-centric": 0, "tiled": 1}[restriction]
+        check(lib().sgld_ctx_set_schedule(self._h, 1 if kbyu_variant == "tiled" else 0, z_chunk, rt))
+
     def stats(self):
         out = np.zeros(8)
         check(lib().sgld_ctx_stats(self._h, out))
         return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "kbyu_launches",
                          "kbyu_ms_fp64", "kbyu_ms_fp32"], out.tolist()))
+
+
+def multi_pcg_device(solvers, dF_ptrs, dU_ptrs, tol=1e-8, maxIT=500, precision=PREC_MIXED, batch=8):
+    """Solves len(solvers) independent problems in one replayed graph
+    (sgld_multi_pcg_device).  Returns (its, relres, latency_s, converged)."""
+    n = len(solvers)
+    ctxs = (C.c_void_p * n)(*[s._h.value for s in solvers])
+    fp = (C.c_void_p * n)(*dF_ptrs)
+    up = (C.c_void_p * n)(*dU_ptrs)
+    its = np.zeros(n, dtype=np.int32)
+    rr = np.zeros(n)
+    lat = np.zeros(n)
+    rc = check(lib().sgld_multi_pcg_device(ctxs, n, fp, up, tol, maxIT, precision, batch, its, rr, lat),
+               allow=(ERR_MAXIT,))
+    return its, rr, lat, rc == 0

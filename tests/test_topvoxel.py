@@ -192,3 +192,18 @@ def test_bc_file(tmp_path):
     path = tmp_path / "x.bc"
     path.write_text("12 1 1 1\n15 0 1 1\n")
     np.testing.assert_array_equal(tv.load_bc(path), [[12, 1, 1, 1], [15, 0, 1, 1]])
+
+
+def test_cfg4_generator():
+    ins = problems.cfg4()
+    assert len(ins) == 32
+    for d in ins:
+        ny, nx, nz = d["voxels"].shape
+        assert {nx, ny, nz} <= set(problems.CFG4_DIMS)
+        L = d["num_levels"]
+        assert 2 <= L <= 5 and all(v % 2 ** (L - 1) == 0 for v in (nx, ny, nz))
+        assert d["x"].min() >= 0.001 - 1e-12 and d["x"].max() <= 1.0 + 1e-12
+        np.testing.assert_allclose(d["E"], 1e-9 + d["x"] ** 3 * (1 - 1e-9))
+    again = problems.cfg4()
+    assert all(a["name"] == b["name"] and np.array_equal(a["x"], b["x"]) for a, b in zip(ins, again))
+    assert problems.levels_for((48, 96, 32)) == 5 and problems.levels_for((40, 8, 24)) == 4
