@@ -198,13 +198,14 @@ int sgld_ctx_autotune(sgld_ctx *ctx, int reps, double *report);
  * K.u; restrictTiled -1 = by level size, 0 = node-centric, 1 = LDS-tiled
  * span-2 restriction.  No counterpart in the reference. */
 int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restrictTiled);
-/* n independent problems (one context each, distinct) solved in one replayed
- * graph (no counterpart in the reference, whose topology-optimisation loop
- * solves one design at a time).  Each instance's PCG iterations are captured
- * once, `batch` (even) iterations long, as a child graph; the parent graph
- * of all instances is launched until every instance has converged, with the
- * host steps (convergence check; for SGLD_PREC_MIXED the reliable updates)
- * between launches; converged instances' kernels exit at once.  dF[i], dU[i]:
+/* n independent problems (one context each, distinct) solved together (no
+ * counterpart in the reference, whose topology-optimisation loop solves one
+ * design at a time).  Each instance's PCG iterations are captured once,
+ * `batch` (even) iterations long, as a graph, which is replayed on the
+ * instance's own stream until it has converged; a host loop re-launches each
+ * instance as soon as its previous batch completed, after the host step of
+ * that instance (convergence check; for SGLD_PREC_MIXED the reliable
+ * updates), so instances of different sizes overlap.  dF[i], dU[i]:
  * device vectors of instance i (U in: initial guess).  its, relres: per
  * instance (FP64 true relative residual); latency[i] (optional): seconds from
  * the call until instance i was seen converged.  Returns SGLD_ERR_MAXIT if

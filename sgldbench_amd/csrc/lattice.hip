@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "lattice_ctx.hpp"
@@ -2093,62 +2094,57 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
         if (latency) latency[i] = S[i].finished ? now_s() - t0 : -1.0;
     }
-    // each unfinished instance's batch of iterations becomes a child of one graph
+    // each unfinished instance's batch of iterations is captured once as a
+    // graph; the host re-launches it on the instance's own stream as soon as
+    // the previous batch of that instance has completed (the streams map onto
+    // the device's hardware queues, so instances overlap)
     struct Graphs {
-        hipGraph_t parent = nullptr;
-        hipGraphExec_t exec = nullptr;
-        std::vector<hipGraph_t> sub;
-        hipStream_t stream = nullptr;
+        std::vector<hipGraph_t> g;
+        std::vector<hipGraphExec_t> x;
         ~Graphs() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (parent) (void)hipGraphDestroy(parent);
-            for (auto g : sub)
-                if (g) (void)hipGraphDestroy(g);
-            if (stream) (void)hipStreamDestroy(stream);
+            for (auto e : x)
+                if (e) (void)hipGraphExecDestroy(e);
+            for (auto h : g)
+                if (h) (void)hipGraphDestroy(h);
         }
     } G;
-    G.sub.assign(n, nullptr);
-    SGLD_HIP(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
-    SGLD_HIP(hipGraphCreate(&G.parent, 0));
-    int live = 0;
+    G.g.assign(n, nullptr);
+    G.x.assign(n, nullptr);
     for (int i = 0; i < n; ++i) {
         if (S[i].finished) continue;
         sgld_ctx *C = ctxs[i];
         SGLD_HIP(hipStreamBeginCapture(C->s, hipStreamCaptureModeThreadLocal));
         const int rc = pcg_enqueue_batch(C, S[i]);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(C->s, &g);
-        if (rc != SGLD_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
+        const hipError_t e = hipStreamEndCapture(C->s, &G.g[i]);
+        if (rc != SGLD_OK) return rc;
         SGLD_HIP(e);
-        G.sub[i] = g;
-        hipGraphNode_t node;
-        SGLD_HIP(hipGraphAddChildGraphNode(&node, G.parent, nullptr, 0, g));
+        SGLD_HIP(hipGraphInstantiate(&G.x[i], G.g[i], nullptr, nullptr, 0));
+    }
+    int live = 0;
+    for (int i = 0; i < n; ++i) {
+        if (S[i].finished) continue;
+        SGLD_HIP(hipGraphLaunch(G.x[i], ctxs[i]->s));
         ++live;
     }
-    int replays = 0;
-    if (live) {
-        SGLD_HIP(hipGraphInstantiate(&G.exec, G.parent, nullptr, nullptr, 0));
-        while (true) {
-            SGLD_HIP(hipGraphLaunch(G.exec, G.stream));
-            SGLD_HIP(hipStreamSynchronize(G.stream));
-            ++replays;
-            bool any = false;
-            for (int i = 0; i < n; ++i) {
-                if (S[i].finished) continue;
-                SGLD_TRY(pcg_after_batch(ctxs[i], S[i]));
-                SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
-                if (S[i].finished) {
-                    if (latency) latency[i] = now_s() - t0;
-                } else {
-                    any = true;
-                }
+    while (live) {
+        bool progressed = false;
+        for (int i = 0; i < n; ++i) {
+            if (S[i].finished) continue;
+            const hipError_t q = hipStreamQuery(ctxs[i]->s);
+            if (q == hipErrorNotReady) continue;
+            SGLD_HIP(q);
+            progressed = true;
+            SGLD_TRY(pcg_after_batch(ctxs[i], S[i]));
+            if (S[i].finished) {
+                if (latency) latency[i] = now_s() - t0;
+                --live;
+            } else {
+                SGLD_HIP(hipGraphLaunch(G.x[i], ctxs[i]->s));  // ordered after any update work
             }
-            if (!any) break;
         }
+        if (!progressed) std::this_thread::yield();
     }
+    for (int i = 0; i < n; ++i) SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
     int allConv = 1;
     for (int i = 0; i < n; ++i) {
         sgld_ctx *C = ctxs[i];

@@ -95,15 +95,16 @@ def test_sensitivity_matches_oracle(fleet):
 
 @pytest.mark.parametrize("sched", [("tiled", 8, "node-centric"), ("tiled", 32, "tiled"), ("dense", 16, None)])
 def test_schedules_match_oracle(sched):
-    vox = holes(38, 22, 34, 5, nholes=4, r=3)
+    # 37 node layers: z-chunks of 8 and 32 leave a partial last chunk
+    vox = holes(40, 24, 36, 5, nholes=4, r=3)
     F, fixed = problems.cantilever3(vox)
     E = problems.simp_modulus(problems.gaussian_random_density(vox.shape, 3, sigma=2.0).ravel(order="F")[vox.ravel(order="F") != 0])
-    g = solver.FEASolver(vox, 2)
+    g = solver.FEASolver(vox, 3)
     g.set_fixed(fixed)
     g.set_modulus(E)
     g.assemble(0.6)
     g.set_schedule(*sched)
-    m = orc.Mesh(vox, 2)
+    m = orc.Mesh(vox, 3)
     m.set_fixed(fixed)
     m.set_modulus(E)
     m.assemble()
