@@ -31,6 +31,8 @@ def parse():
                          "solves; cfg4: 32 uneven cuboids in one replayed graph plus sensitivities; "
                          "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
     ap.add_argument("--n3", type=int, default=256, help="cube edge of the batched workload")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="hardware queues for the multi-instance workload (GPU_MAX_HW_QUEUES, <= 32)")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
@@ -107,6 +109,9 @@ def run_multi(a):
     rel-res 1e-8 (mixed precision) in one replayed graph, then the fused
     compliance sensitivities -p x^(p-1) u_e^T Ke u_e of every instance.
     Compared with the same work done instance by instance."""
+    # instances replay on their own streams: give them more hardware queues
+    # than HIP's default of 4 (read when the runtime initialises)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(1, a.hw_queues))))
     import torch
     from sgldbench_amd import problems, solver
     ins = problems.cfg4()
@@ -133,13 +138,13 @@ def run_multi(a):
         return [s.sensitivity_device(u.data_ptr(), x.data_ptr(), c.data_ptr(), penal=d["penal"])
                 for s, u, x, c, d in zip(S, dU, dx, ddc, ins)]
 
-    def step_multi(batch):
+    def step_multi(batch, prec=solver.PREC_MIXED):
         for u in dU:
             u.zero_()
         torch.cuda.synchronize()
         t = time.perf_counter()
         its, rr, lat, ok = solver.multi_pcg_device(S, [f.data_ptr() for f in dF], [u.data_ptr() for u in dU],
-                                                   tol=1e-8, maxIT=2000, precision=solver.PREC_MIXED, batch=batch)
+                                                   tol=1e-8, maxIT=2000, precision=prec, batch=batch)
         comp = sens_all()
         torch.cuda.synchronize()
         return time.perf_counter() - t, its, rr, lat, ok, comp
@@ -171,6 +176,9 @@ def run_multi(a):
         runs.append(step_multi(batch))
         progress("multi step %d: %.3f s, max its %d, max rel-res %.2e" % (k, runs[-1][0], runs[-1][1].max(),
                                                                         runs[-1][2].max()))
+    multi64 = [step_multi(batch, solver.PREC_FP64) for _ in range(max(1, a.steps // 2))]
+    t_multi64 = float(np.mean([r[0] for r in multi64]))
+    progress("multi FP64: %.3f s" % t_multi64)
     seq = {}
     for prec, name in ((solver.PREC_MIXED, "sequential_mixed"), (solver.PREC_FP64, "sequential_fp64")):
         ts = []
@@ -196,7 +204,8 @@ def run_multi(a):
                                 "cantilever BCs, SIMP p=3 densities, 4-5 levels, mixed precision, tol 1e-8, "
                                 "one replayed graph, then sensitivities" % (len(ins), n_tot)),
                    "path": "multi-instance graph, mixed", "graph_batch": batch, "instances": len(ins)},
-        "multi": {"s_per_step": t_multi, "graph_batch_trials_s": trial,
+        "multi": {"s_per_step": t_multi, "graph_batch_trials_s": trial, "fp64_s_per_step": t_multi64,
+                  "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                   "speedup_vs_sequential_mixed": seq["sequential_mixed"]["s_per_step"] / t_multi,
                   "speedup_vs_sequential_fp64": seq["sequential_fp64"]["s_per_step"] / t_multi,
                   "sequential_mixed": seq["sequential_mixed"], "sequential_fp64": seq["sequential_fp64"],
