@@ -1388,6 +1388,32 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     return SGLD_OK;
 }
 
+// The element matrix (and its mode-space coefficients) live in constant
+// memory, shared by every context of the process.  A context whose Ke differs
+// from the uploaded one (another Poisson ratio or cell size) re-uploads them
+// before its kernels run, after the device has drained work that read them.
+double g_constKe[576];
+bool g_constValid = false;
+
+int use_constants(const sgld_ctx *C) {
+    if (g_constValid && !std::memcmp(g_constKe, C->Ke, sizeof(g_constKe))) return SGLD_OK;
+    if (g_constValid) SGLD_HIP(hipDeviceSynchronize());
+    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe64), C->Ke, sizeof(double) * 576));
+    float k32[576];
+    for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
+    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
+    double km[9];
+    if (mode_coefficients(C->Ke, km)) {
+        float km32[9];
+        for (int i = 0; i < 9; ++i) km32[i] = (float)km[i];
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 9));
+        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 9));
+    }
+    std::memcpy(g_constKe, C->Ke, sizeof(g_constKe));
+    g_constValid = true;
+    return SGLD_OK;
+}
+
 int upload_flags(Level &l) {
     if (!l.nflag.p) SGLD_TRY(l.nflag.alloc(l.g.nodes()));
     SGLD_HIP(hipMemcpy(l.nflag.p, l.hflag.data(), l.hflag.size(), hipMemcpyHostToDevice));
@@ -1474,6 +1500,7 @@ int check_ready(sgld_ctx *C) {
         set_error("sgld:state: call sgld_ctx_set_fixed, sgld_ctx_set_modulus and sgld_ctx_assemble first");
         return SGLD_ERR_STATE;
     }
+    SGLD_TRY(use_constants(C));
     return SGLD_OK;
 }
 
@@ -1583,20 +1610,13 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     }
     SGLD_TRY(C->partial.alloc((int64_t)kRedSlots * kRedBlocks));
     SGLD_TRY(C->ctl.alloc(1));
-    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe64), C->Ke, sizeof(double) * 576));
-    float k32[576];
-    for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
-    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
+    SGLD_TRY(use_constants(C.get()));
     double km[9];
     C->kbyuVariant = mode_coefficients(C->Ke, km) ? 1 : 0;
     if (const char *env = std::getenv("SGLD_KBYU")) {
         if (!std::strcmp(env, "dense")) C->kbyuVariant = 0;
     }
     if (C->kbyuVariant == 1) {
-        float km32[9];
-        for (int i = 0; i < 9; ++i) km32[i] = (float)km[i];
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 9));
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 9));
         const Grid &g0 = f0.g;
         TileMap &tm = C->tiles;
         tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
@@ -1698,6 +1718,7 @@ extern "C" int sgld_ctx_set_modulus(sgld_ctx *C, const double *E) {
 extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
     SGLD_ARG(C, "sgld:arg: null context");
     SGLD_ARG(omega > 0 && omega < 2, "sgld:arg: omega must be in (0, 2)");
+    SGLD_TRY(use_constants(C));  // the finest diagonal reads Ke
     if (!C->haveFixed || !C->haveModulus) {
         set_error("sgld:state: set fixed DOFs and modulus before assembling");
         return SGLD_ERR_STATE;
@@ -1895,6 +1916,7 @@ extern "C" int sgld_ctx_pcg(sgld_ctx *C, const double *F, double *U, double tol,
 
 extern "C" int sgld_ctx_unit_compliance(sgld_ctx *C, const double *U, double *ce) {
     SGLD_ARG(C && U && ce, "sgld:arg: null argument");
+    SGLD_TRY(use_constants(C));
     const int64_t nc = 3 * C->lv[0].numNodes, ne = C->lv[0].numElements;
     DevBuf<double> dU, dce;
     SGLD_TRY(dU.upload(U, nc));
@@ -1910,6 +1932,7 @@ extern "C" int sgld_ctx_unit_compliance(sgld_ctx *C, const double *U, double *ce
 
 extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, int precision) {
     SGLD_ARG(C && du && dy, "sgld:arg: null argument");
+    SGLD_TRY(use_constants(C));
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
     SGLD_TRY(to_lattice(C, du, C->z.p));
     if (precision == SGLD_PREC_FP64) {
@@ -1938,6 +1961,7 @@ extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, i
 extern "C" int sgld_ctx_bench_kbyu(sgld_ctx *C, int precision, int reps, double *msPerLaunch) {
     SGLD_ARG(C && msPerLaunch && reps > 0, "sgld:arg: bad argument");
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
+    SGLD_TRY(use_constants(C));
     SGLD_HIP(hipStreamSynchronize(C->s));
     SGLD_HIP(hipEventRecord(C->e0, C->s));
     for (int i = 0; i < reps; ++i) {
@@ -1964,6 +1988,7 @@ extern "C" int sgld_ctx_stats(const sgld_ctx *C, double *out) {
 extern "C" int sgld_ctx_sensitivity_device(sgld_ctx *C, const double *dU, const double *dx, double penal,
                                            double E0, double Emin, double scale, double *ddc, double *compliance) {
     SGLD_ARG(C && dU && dx && ddc, "sgld:arg: null argument");
+    SGLD_TRY(use_constants(C));
     SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
     SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
     const int64_t ne = C->lv[0].numElements;
@@ -2076,6 +2101,8 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         SGLD_TRY(check_ready(ctxs[i]));
         SGLD_ARG(dF[i] && dU[i], "sgld:arg: null vector of instance %d", i);
         for (int k = 0; k < i; ++k) SGLD_ARG(ctxs[k] != ctxs[i], "sgld:arg: instance %d repeats a context", i);
+        SGLD_ARG(!std::memcmp(ctxs[i]->Ke, ctxs[0]->Ke, sizeof(ctxs[0]->Ke)),
+                 "sgld:arg: instance %d has another element matrix (Poisson ratio / cell size) than instance 0", i);
     }
     SGLD_HIP(hipDeviceSynchronize());
     const double t0 = now_s();

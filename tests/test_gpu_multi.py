@@ -131,3 +131,25 @@ def test_autotune_keeps_parity():
     U, its, rr = g.pcg(F, tol=1e-8, maxIT=500)
     ref, its_o, hist, rc = m.pcg(F, 0.6, 1e-8, 500)
     assert abs(its - its_o) <= 1 and abs(F @ U - F @ ref) <= 1e-9 * abs(F @ ref)
+
+
+def test_contexts_with_different_poisson_ratio():
+    """Each context's element matrix is used even when another context with a
+    different one ran in between (constant memory is re-uploaded)."""
+    vox = problems.cuboid(16, 8, 8)
+    F, fixed = problems.cantilever3(vox)
+    E = np.ones(int(vox.sum()))
+    out = {}
+    for nu in (0.3, 0.2):
+        g = solver.FEASolver(vox, 3, poissonRatio=nu)
+        g.set_fixed(fixed)
+        g.set_modulus(E)
+        g.assemble(0.6)
+        out[nu] = g
+    u = np.random.default_rng(2).standard_normal(out[0.3].numDOFs)
+    for nu in (0.3, 0.2, 0.3):
+        m = orc.Mesh(vox, 3, nu=nu)
+        m.set_fixed(fixed)
+        m.set_modulus(E)
+        ref = m.kbyu(u)
+        assert np.abs(out[nu].kbyu(u) - ref).max() <= 1e-13 * np.abs(ref).max(), nu
