@@ -188,16 +188,20 @@ int sgld_ctx_sensitivity_device(sgld_ctx *ctx, const double *dU, const double *d
  * mesh (patch-tiled mode-space with z-chunks of 8, 16 or 32 node layers, and
  * the node-centric dense kernel; `reps` launches each, FP64 and FP32) and the
  * two span-2 restrictions (on FP32 V-cycles, once assembled), and keeps the
- * fastest for the mixed solver (cost = FP32 K.u + FP64 K.u / 8).  report[8]:
- * [0] K.u variant (1 tiled, 0 dense), [1] z-chunk, [2] FP64 K.u ms, [3] FP32
- * K.u ms, [4] restriction (1 tiled, 0 node-centric), [5], [6] V-cycle ms of
- * the two restrictions. */
+ * fastest: the z-chunk per precision, then tiled against dense for the
+ * mixed solver (cost = FP32 K.u + FP64 K.u / 8).  report[8]: [0] K.u variant
+ * (1 tiled, 0 dense), [1] FP64 z-chunk, [2] FP64 K.u ms, [3] FP32 K.u ms,
+ * [4] restriction (1 tiled, 0 node-centric), [5], [6] V-cycle ms of the two
+ * restrictions, [7] FP32 z-chunk. */
 int sgld_ctx_autotune(sgld_ctx *ctx, int reps, double *report);
 /* Pins the schedule the autotuner chooses: kbyuVariant 1 = patch-tiled
- * mode-space K.u with z-chunks of zChunk node layers, 0 = node-centric dense
- * K.u; restrictTiled -1 = by level size, 0 = node-centric, 1 = LDS-tiled
- * span-2 restriction.  No counterpart in the reference. */
-int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restrictTiled);
+ * mode-space K.u with z-chunks of zChunk node layers and a persistent grid of
+ * blocksPerCU blocks per compute unit (0 = as many as fit), both precisions;
+ * 0 = node-centric
+ * dense K.u; restrictTiled -1 = by level size, 0 = node-centric, 1 =
+ * LDS-tiled span-2 restriction.  No counterpart in the reference. */
+int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restrictTiled,
+                          int blocksPerCU);
 /* n independent problems (one context each, distinct) solved together (no
  * counterpart in the reference, whose topology-optimisation loop solves one
  * design at a time).  Each instance's PCG iterations are captured once,
@@ -216,7 +220,7 @@ int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF,
 
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
- * last assemble, [5] K.u launches, [6] average K.u kernel milliseconds
+ * last assemble, [5] reserved (0), [6] average K.u kernel milliseconds
  * (hipEvent, FP64), [7] average K.u kernel milliseconds (FP32).  For
  * SGLD_PREC_MIXED [3] counts reliable updates. */
 int sgld_ctx_stats(const sgld_ctx *ctx, double *out);

@@ -55,7 +55,7 @@ SIGNATURES = {
     "sgld_ctx_sensitivity_device": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
                                     C.POINTER(C.c_double)],
     "sgld_ctx_autotune": [_vp, C.c_int, _dp],
-    "sgld_ctx_set_schedule": [_vp, C.c_int, C.c_int, C.c_int],
+    "sgld_ctx_set_schedule": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "sgld_multi_pcg_device": [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.c_int,
                               C.c_int, C.c_int, _ip, _dp, _dp],
     "sgld_ctx_stats": [_vp, _dp],

@@ -226,7 +226,7 @@ __device__ __forceinline__ void face_modes(const T *__restrict__ lay, int ei, in
 }
 
 template <class T>
-__global__ void __launch_bounds__(256, 3) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
                                                        TileMap tm, double *__restrict__ partial,
@@ -1063,9 +1063,10 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
         u = pout;
     }
     if (C->kbyuVariant == 1) {
-        nb = C->tiledGrid;
+        const TiledSchedule &ts = C->ksched[sizeof(T) == 8 ? 0 : 1];
+        nb = ts.grid;
         hipLaunchKernelGGL(k_kbyu_tiled<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
-                           l0.nflag.p, l0.g, C->tiles, partial, ctl);
+                           l0.nflag.p, l0.g, ts.tm, partial, ctl);
     } else {
         nb = grid_blocks(l0.g.nodes());
         hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
@@ -1618,20 +1619,26 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     }
     if (C->kbyuVariant == 1) {
         const Grid &g0 = f0.g;
-        TileMap &tm = C->tiles;
-        tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
-        tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
-        tm.cz = 16;
-        tm.tilesZ = (int)((g0.NZ() + tm.cz - 1) / tm.cz);
-        int dev = 0, ncu = 256, per = 4;
+        int dev = 0, ncu = 256;
         SGLD_HIP(hipGetDevice(&dev));
         SGLD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        int p64 = 0, p32 = 0;
-        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p64, k_kbyu_tiled<double>, 256, 0));
-        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p32, k_kbyu_tiled<float>, 256, 0));
-        per = std::max(1, std::min(p64, p32));
-        C->tiledCap = (int)std::min<int64_t>((int64_t)ncu * per, kRedBlocks);
-        C->tiledGrid = (int)std::min<int64_t>(tm.work(), C->tiledCap);
+        int per[2] = {0, 0};
+        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double>, 256, 0));
+        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float>, 256, 0));
+        C->numCU = ncu;
+        // Blocks per compute unit of the persistent grid.  A sweep of 2..6 on the
+        // 256x128x128 cuboid measured 3 (FP64: 168 VGPRs, 3 waves per SIMD) and
+        // 4 (FP32: 114 VGPRs) fastest, one more than the occupancy query gives.
+        const int measured[2] = {3, 4};
+        for (int q = 0; q < 2; ++q) {
+            C->kOcc[q] = std::max(per[q], measured[q]);
+            TiledSchedule &ts = C->ksched[q];
+            ts.tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
+            ts.tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
+            ts.cap = (int)std::min<int64_t>((int64_t)ncu * std::max(1, per[q]), kRedBlocks);
+            // z-chunks of 8 node layers: measured best for both precisions on 256x128x128
+            ts.set_chunk((int)g0.NZ(), 8);
+        }
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
     *out = C.release();
@@ -2022,43 +2029,34 @@ extern "C" int sgld_ctx_sensitivity(sgld_ctx *C, const double *U, const double *
 extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
     SGLD_ARG(C && report && reps > 0, "sgld:arg: bad arguments");
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
-    double best = 1e300, best64 = 0, best32 = 0;
-    int bestCz = C->tiles.cz, bestVariant = C->kbyuVariant;
+    SGLD_TRY(use_constants(C));
+    const int NZ = (int)C->lv[0].g.NZ();
     const int variant0 = C->kbyuVariant;
-    const TileMap tm0 = C->tiles;
     const int czs[3] = {8, 16, 32};
-    for (int v = 0; v < (variant0 == 1 ? 4 : 1); ++v) {
-        if (variant0 == 1) {
-            if (v < 3) {
-                C->kbyuVariant = 1;
-                C->tiles = tm0;
-                C->tiles.cz = czs[v];
-                C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + czs[v] - 1) / czs[v]);
-                C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
-            } else {
-                C->kbyuVariant = 0;
+    double tiled[2] = {1e300, 1e300};
+    int bestCz[2] = {C->ksched[0].tm.cz, C->ksched[1].tm.cz};
+    if (variant0 == 1) {
+        // each precision's z-chunk on its own (persistent grid = resident blocks)
+        for (int q = 0; q < 2; ++q)
+            for (int cz : czs) {
+                C->ksched[q].set_chunk(NZ, cz);
+                double t = 0;
+                SGLD_TRY(sgld_ctx_bench_kbyu(C, q == 0 ? SGLD_PREC_FP64 : SGLD_PREC_MIXED, reps, &t));
+                if (t < tiled[q]) {
+                    tiled[q] = t;
+                    bestCz[q] = cz;
+                }
             }
-        }
-        double t64 = 0, t32 = 0;
-        SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_FP64, reps, &t64));
-        SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_MIXED, reps, &t32));
-        // the mixed solver spends its iterations in FP32 K.u; FP64 K.u runs once per reliable update
-        const double cost = t32 + t64 / 8.0;
-        if (cost < best) {
-            best = cost;
-            best64 = t64;
-            best32 = t32;
-            bestVariant = C->kbyuVariant;
-            bestCz = C->tiles.cz;
-        }
+        for (int q = 0; q < 2; ++q) C->ksched[q].set_chunk(NZ, bestCz[q]);
     }
-    C->kbyuVariant = bestVariant;
-    if (bestVariant == 1) {
-        C->tiles = tm0;
-        C->tiles.cz = bestCz;
-        C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + bestCz - 1) / bestCz);
-        C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
-    }
+    // against the node-centric dense K.u; the mixed solver spends its
+    // iterations in FP32 K.u and one FP64 K.u per reliable update
+    C->kbyuVariant = 0;
+    double dense64 = 0, dense32 = 0;
+    SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_FP64, reps, &dense64));
+    SGLD_TRY(sgld_ctx_bench_kbyu(C, SGLD_PREC_MIXED, reps, &dense32));
+    const bool useTiled = variant0 == 1 && tiled[1] + tiled[0] / 8.0 <= dense32 + dense64 / 8.0;
+    C->kbyuVariant = useTiled ? 1 : 0;
     // span-2 restriction variant, timed on FP32 V-cycles (needs the assembled hierarchy)
     int bestR = C->restrictTiled;
     double tv[2] = {0, 0};
@@ -2078,14 +2076,14 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
         bestR = tv[1] < tv[0] ? 1 : 0;
     }
     C->restrictTiled = bestR;
-    report[0] = bestVariant;
-    report[1] = bestVariant == 1 ? bestCz : 0;
-    report[2] = best64;
-    report[3] = best32;
+    report[0] = useTiled ? 1 : 0;
+    report[1] = useTiled ? bestCz[0] : 0;
+    report[2] = useTiled ? tiled[0] : dense64;
+    report[3] = useTiled ? tiled[1] : dense32;
     report[4] = bestR;
     report[5] = tv[0];
     report[6] = tv[1];
-    report[7] = 0;
+    report[7] = useTiled ? bestCz[1] : 0;
     return SGLD_OK;
 }
 
@@ -2193,17 +2191,21 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
     return SGLD_OK;
 }
 
-extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, int restrictTiled) {
+extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, int restrictTiled,
+                                     int blocksPerCU) {
     SGLD_ARG(C, "sgld:arg: null context");
+    SGLD_ARG(blocksPerCU >= 0 && blocksPerCU <= 8, "sgld:arg: blocksPerCU must be in 0..8 (0 = occupancy)");
     SGLD_ARG(kbyuVariant == 0 || kbyuVariant == 1, "sgld:arg: kbyuVariant must be 0 (dense) or 1 (tiled)");
     SGLD_ARG(restrictTiled >= -1 && restrictTiled <= 1, "sgld:arg: restrictTiled must be -1, 0 or 1");
     if (kbyuVariant == 1) {
         double km[9];
         SGLD_ARG(mode_coefficients(C->Ke, km), "sgld:arg: the tiled K.u needs the isotropic-cube Ke structure");
         SGLD_ARG(zChunk >= 2 && zChunk <= 256, "sgld:arg: zChunk must be in 2..256");
-        C->tiles.cz = zChunk;
-        C->tiles.tilesZ = (int)((C->lv[0].g.NZ() + zChunk - 1) / zChunk);
-        C->tiledGrid = (int)std::min<int64_t>(C->tiles.work(), C->tiledCap);
+        for (int q = 0; q < 2; ++q) {
+            const int per = blocksPerCU > 0 ? blocksPerCU : std::max(1, C->kOcc[q]);
+            C->ksched[q].cap = (int)std::min<int64_t>((int64_t)C->numCU * per, kRedBlocks);
+            C->ksched[q].set_chunk((int)C->lv[0].g.NZ(), zChunk);
+        }
     }
     C->kbyuVariant = kbyuVariant;
     C->restrictTiled = restrictTiled;

@@ -22,6 +22,20 @@ struct TileMap {
     __host__ __device__ int64_t work() const { return (int64_t)tilesR * tilesC * tilesZ; }
 };
 
+// Schedule of the patch-tiled K.u for one precision: work decomposition and
+// persistent grid = min(work items, cap), cap = resident blocks on the device.
+struct TiledSchedule {
+    TileMap tm{1, 1, 1, 8};
+    int grid = 1, cap = 1;
+    void set_chunk(int NZ, int cz) {
+        tm.cz = cz;
+        tm.tilesZ = (NZ + cz - 1) / cz;
+        const int64_t w = tm.work();
+        grid = (int)(w < cap ? w : cap);
+        if (grid < 1) grid = 1;
+    }
+};
+
 struct Level {
     Grid g{};
     int span = 1;
@@ -60,9 +74,9 @@ struct sgld_ctx {
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
-    sgld::TileMap tiles{1, 1, 1, 16};
-    int tiledGrid = 1;    // persistent grid of the tiled K.u = min(work items, tiledCap)
-    int tiledCap = 1;     // resident blocks of the tiled K.u on the device
+    sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
+    int kOcc[2] = {1, 1};           // resident tiled K.u blocks per compute unit
+    int numCU = 0;
     double stats[8] = {0};
     sgld::BatchState *batch = nullptr;  // multi-RHS buffers, allocated on first use
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }

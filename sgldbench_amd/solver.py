@@ -309,18 +309,19 @@ class FEASolver:
         out = np.zeros(8)
         check(lib().sgld_ctx_autotune(self._h, reps, out))
         return dict(kbyu_variant="tiled" if out[0] == 1 else "dense", z_chunk=int(out[1]),
-                    kbyu_ms_fp64=out[2], kbyu_ms_fp32=out[3],
+                    z_chunk_fp32=int(out[7]), kbyu_ms_fp64=out[2], kbyu_ms_fp32=out[3],
                     restriction="tiled" if out[4] == 1 else "node-centric", vcycle_ms=(out[5], out[6]))
 
-    def set_schedule(self, kbyu_variant="tiled", z_chunk=16, restriction=None):
+    def set_schedule(self, kbyu_variant="tiled", z_chunk=16, restriction=None, blocks_per_cu=0):
         """Pins a K.u / restriction schedule (what autotune picks)."""
         rt = {None: -1, "node-centric": 0, "tiled": 1}[restriction]
-        check(lib().sgld_ctx_set_schedule(self._h, 1 if kbyu_variant == "tiled" else 0, z_chunk, rt))
+        check(lib().sgld_ctx_set_schedule(self._h, 1 if kbyu_variant == "tiled" else 0, z_chunk, rt,
+                                          blocks_per_cu))
 
     def stats(self):
         out = np.zeros(8)
         check(lib().sgld_ctx_stats(self._h, out))
-        return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "kbyu_launches",
+        return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "reserved",
                          "kbyu_ms_fp64", "kbyu_ms_fp32"], out.tolist()))
 
 
