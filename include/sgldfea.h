@@ -218,9 +218,28 @@ int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF,
                           double tol, int maxIT, int precision, int batch, int *its,
                           double *relres, double *latency);
 
+/* Hs = TopOpti_SetupDensityFilter_mex(rMin, numElements, eleMapForward, resX,
+ * resY, resZ) (src/MEXfuncs/TopOpti_SetupDensityFilter_mex.cpp:10-51):
+ * Hs[e1] = sum over solid e2 within the box of half-width ceil(rMin)-1 of
+ * max(0, rMin - |c_e1 - c_e2|).  eleMapForward: resX resY resZ voxels (row
+ * fastest), 1-based element id or 0; Hs: numElements.  Host pointers. */
+int sgld_density_filter_setup(double rMin, int64_t numElements, const int32_t *eleMapForward,
+                              int resX, int resY, int resZ, double *Hs);
+/* Y = TopOpti_PerformDensityFiltering_mex(x, rMin, numElements, eleMapForward,
+ * resX, resY, resZ) (src/MEXfuncs/TopOpti_PerformDensityFiltering_mex.cpp:
+ * 10-52): Y[e1] = sum_e2 max(0, rMin - |c_e1 - c_e2|) x[e2] (the caller
+ * divides by Hs).  Bit-identical to the reference loop order in FP64. */
+int sgld_density_filter(const double *x, double rMin, int64_t numElements,
+                        const int32_t *eleMapForward, int resX, int resY, int resZ, double *Y);
+/* The same filter on a context's finest mesh with device vectors
+ * (numElements, compressed order); dx == NULL computes Hs.  The filter
+ * kernel's time goes to sgld_ctx_stats out[5]. */
+int sgld_ctx_density_filter_device(sgld_ctx *ctx, const double *dx, double rMin, double *dY);
+
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
- * last assemble, [5] reserved (0), [6] average K.u kernel milliseconds
+ * last assemble, [5] milliseconds of the last density-filter kernel
+ * (sgld_ctx_density_filter_device), [6] average K.u kernel milliseconds
  * (hipEvent, FP64), [7] average K.u kernel milliseconds (FP32).  For
  * SGLD_PREC_MIXED [3] counts reliable updates. */
 int sgld_ctx_stats(const sgld_ctx *ctx, double *out);

@@ -61,6 +61,8 @@ def lib():
         L.or_mesh_pcg.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_double, C.c_int,
                                   _dp, C.POINTER(C.c_int)]
         L.or_mesh_unit_compliance.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_density_filter_setup.argtypes = [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
+        L.or_density_filter.argtypes = [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
         L.or_mesh_time_pcg_iterations.restype = C.c_double
         L.or_mesh_time_pcg_iterations.argtypes = [C.c_void_p, C.c_double, _dp, C.c_int,
                                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -160,6 +162,24 @@ def unit_compliance_mex(U, eNodMat, Ke):
     lib().or_unit_compliance_mex(f64(U), i32(eNodMat.ravel(order="F")), ne,
                                  f64(np.asarray(Ke).ravel(order="F")), ce)
     return ce
+
+
+def density_filter_setup(rMin, eleMapForward, resX, resY, resZ):
+    """Hs = TopOpti_SetupDensityFilter_mex(rMin, numElements, eleMapForward, resX, resY, resZ)."""
+    efwd = i32(np.asarray(eleMapForward).ravel(order="F"))
+    ne = int(efwd.max()) if efwd.size else 0
+    Hs = np.zeros(ne)
+    lib().or_density_filter_setup(rMin, ne, efwd, resX, resY, resZ, Hs)
+    return Hs
+
+
+def density_filter(x, rMin, eleMapForward, resX, resY, resZ):
+    """Y = TopOpti_PerformDensityFiltering_mex(x, rMin, numElements, eleMapForward, resX, resY, resZ)."""
+    efwd = i32(np.asarray(eleMapForward).ravel(order="F"))
+    x = f64(x)
+    Y = np.zeros(x.size)
+    lib().or_density_filter(x, rMin, x.size, efwd, resX, resY, resZ, Y)
+    return Y
 
 
 class Mesh:

@@ -890,3 +890,47 @@ double or_mesh_time_pcg_iterations(const or_mesh *M, double omega, const double 
     free(y); free(r); free(z); free(p);
     return t;
 }
+
+
+/* ------------------------------ density filter ------------------------------ */
+/* Both MEX functions visit the neighbours of e1 with layer outermost, then
+ * column, then row, and add max(0, rMin - sqrt(d2)) (times x) for solid ones;
+ * the sums keep that order so results match to the last bit. */
+static void density_filter_impl(const double *x, double rMin, int64_t numElements,
+                                const int32_t *efwd, int resX, int resY, int resZ, double *Y) {
+    const int R = (int)ceil(rMin) - 1;
+    const int64_t resXY = (int64_t)resX * resY;
+    (void)numElements;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int kk = 0; kk < resZ; ++kk)
+        for (int ii = 0; ii < resX; ++ii)
+            for (int jj = 0; jj < resY; ++jj) {
+                const int32_t e1 = efwd[kk * resXY + (int64_t)ii * resY + jj];
+                if (!e1) continue;
+                double acc = 0.0;
+                const int k0 = kk - R > 0 ? kk - R : 0, k1 = kk + R < resZ - 1 ? kk + R : resZ - 1;
+                const int i0 = ii - R > 0 ? ii - R : 0, i1 = ii + R < resX - 1 ? ii + R : resX - 1;
+                const int j0 = jj - R > 0 ? jj - R : 0, j1 = jj + R < resY - 1 ? jj + R : resY - 1;
+                for (int k2 = k0; k2 <= k1; ++k2)
+                    for (int i2 = i0; i2 <= i1; ++i2)
+                        for (int j2 = j0; j2 <= j1; ++j2) {
+                            const int32_t e2 = efwd[k2 * resXY + (int64_t)i2 * resY + j2];
+                            if (!e2) continue;
+                            const int d2 = (ii - i2) * (ii - i2) + (jj - j2) * (jj - j2) + (kk - k2) * (kk - k2);
+                            const double w = rMin - sqrt((double)d2);
+                            const double wp = w > 0.0 ? w : 0.0;
+                            acc += x ? wp * x[e2 - 1] : wp;
+                        }
+                Y[e1 - 1] = acc;
+            }
+}
+
+void or_density_filter_setup(double rMin, int64_t numElements, const int32_t *eleMapForward,
+                             int resX, int resY, int resZ, double *Hs) {
+    density_filter_impl(NULL, rMin, numElements, eleMapForward, resX, resY, resZ, Hs);
+}
+
+void or_density_filter(const double *x, double rMin, int64_t numElements,
+                       const int32_t *eleMapForward, int resX, int resY, int resZ, double *Y) {
+    density_filter_impl(x, rMin, numElements, eleMapForward, resX, resY, resZ, Y);
+}

@@ -102,6 +102,17 @@ int or_mesh_pcg(const or_mesh *m, double omega, const double *b, double *x,
 /* TopOpti_ComputeUnitCompliance.m:1-21. */
 void or_mesh_unit_compliance(const or_mesh *m, const double *U, double *ce);
 
+/* Density filter of the topology optimisation (cone weights
+ * w = max(0, rMin - |c_e1 - c_e2|) over the solid voxels within the
+ * ceil(rMin)-1 box):
+ *   or_density_filter_setup: Hs[e1] = sum_e2 w (TopOpti_SetupDensityFilter_mex.cpp:10-51)
+ *   or_density_filter:       Y[e1]  = sum_e2 w x[e2] (TopOpti_PerformDensityFiltering_mex.cpp:10-52)
+ * eleMapForward: resX resY resZ voxels (row fastest), 1-based element id or 0. */
+void or_density_filter_setup(double rMin, int64_t numElements, const int32_t *eleMapForward,
+                             int resX, int resY, int resZ, double *Hs);
+void or_density_filter(const double *x, double rMin, int64_t numElements,
+                       const int32_t *eleMapForward, int resX, int resY, int resZ, double *Y);
+
 /* Timing helper for the CPU baseline: runs `its` PCG iterations exactly
  * (no early exit) and returns seconds spent; setup excluded. */
 double or_mesh_time_pcg_iterations(const or_mesh *m, double omega, const double *b,

@@ -58,6 +58,9 @@ SIGNATURES = {
     "sgld_ctx_set_schedule": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "sgld_multi_pcg_device": [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.c_int,
                               C.c_int, C.c_int, _ip, _dp, _dp],
+    "sgld_density_filter_setup": [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
+    "sgld_density_filter": [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
+    "sgld_ctx_density_filter_device": [_vp, _vp, C.c_double, _vp],
     "sgld_ctx_stats": [_vp, _dp],
 }
 _RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}
@@ -96,7 +99,7 @@ class SGLDError(RuntimeError):
         self.identifier = msg.split(":")[0] + ":" + msg.split(":")[1] if msg.count(":") >= 2 else msg
 
 
-ERR_MAXIT = 4
+OK, ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_MAXIT, ERR_STATE, ERR_NOMEM = range(7)  # include/sgldfea.h
 
 
 def check(rc, allow=()):

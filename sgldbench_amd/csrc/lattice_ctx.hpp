@@ -60,6 +60,7 @@ struct sgld_ctx {
     bool full = true;
     std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
     sgld::DevBuf<int32_t> nodMap, eleMap;
+    sgld::DevBuf<int32_t> efwd;  // voxel -> 1-based element id (0 void), built on first filter use
     sgld::DevBuf<double> E;
     sgld::DevBuf<float> E32;
     sgld::DevBuf<double> Minv;

@@ -145,6 +145,24 @@ def FEA_VoxelBasedElementStiffnessMatrix(poissonRatio=0.3, cellSize=1.0):
     return Ke.reshape(24, 24, order="F")
 
 
+def TopOpti_SetupDensityFilter_mex(rMin, numElements, eleMapForward, resX, resY, resZ):
+    """Hs (numElements) of the cone density filter; eleMapForward: voxels in
+    (resY, resX, resZ) column-major order, 1-based element id or 0."""
+    m = np.ascontiguousarray(np.asarray(eleMapForward).ravel(order="F"), dtype=np.int32)
+    Hs = np.zeros(int(numElements))
+    check(lib().sgld_density_filter_setup(float(rMin), int(numElements), m, int(resX), int(resY), int(resZ), Hs))
+    return Hs
+
+
+def TopOpti_PerformDensityFiltering_mex(x, rMin, numElements, eleMapForward, resX, resY, resZ):
+    """Y = sum_e2 max(0, rMin - dist) x[e2] per solid element."""
+    m = np.ascontiguousarray(np.asarray(eleMapForward).ravel(order="F"), dtype=np.int32)
+    x = _f64(x).ravel()
+    Y = np.zeros(int(numElements))
+    check(lib().sgld_density_filter(x, float(rMin), int(numElements), m, int(resX), int(resY), int(resZ), Y))
+    return Y
+
+
 class FEASolver:
     """Device-resident meshHierarchy_ plus the solve functions that read it:
     FEA_SetupVoxelBased, Solving_AssembleFEAstencil, Solving_KbyU_MatrixFree,
@@ -303,6 +321,11 @@ class FEASolver:
                                                 Emin, scale, C.c_void_p(ddc_ptr), C.byref(c)))
         return c.value
 
+    def density_filter_device(self, dx_ptr, rMin, dy_ptr):
+        """Density filter on this mesh with device vectors; dx_ptr 0 gives Hs."""
+        check(lib().sgld_ctx_density_filter_device(self._h, C.c_void_p(dx_ptr) if dx_ptr else None, float(rMin),
+                                                   C.c_void_p(dy_ptr)))
+
     def autotune(self, reps=10):
         """Picks the fastest K.u variant / z-chunk and span-2 restriction for
         this mesh (kept in the context)."""
@@ -321,7 +344,7 @@ class FEASolver:
     def stats(self):
         out = np.zeros(8)
         check(lib().sgld_ctx_stats(self._h, out))
-        return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "reserved",
+        return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "filter_ms",
                          "kbyu_ms_fp64", "kbyu_ms_fp32"], out.tolist()))
 
 
