@@ -236,6 +236,14 @@ int sgld_density_filter(const double *x, double rMin, int64_t numElements,
  * kernel's time goes to sgld_ctx_stats out[5]. */
 int sgld_ctx_density_filter_device(sgld_ctx *ctx, const double *dx, double rMin, double *dY);
 
+/* [cartesianStressField, vonMisesStressField] = FEA_StressAnalysis()
+ * (src/FEA_StressAnalysis.m:1-66): nodal stresses extrapolated from the 8
+ * Gauss points of each element (E_e D B u_e), averaged over the solid
+ * elements at each node; stress: numNodes x 6 column-major (xx yy zz yz zx
+ * xy), vonMises: numNodes.  Host pointers; _device: device pointers. */
+int sgld_ctx_stress(sgld_ctx *ctx, const double *U, double *stress, double *vonMises);
+int sgld_ctx_stress_device(sgld_ctx *ctx, const double *dU, double *dstress, double *dvonMises);
+
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
  * last assemble, [5] milliseconds of the last density-filter kernel

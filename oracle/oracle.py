@@ -63,6 +63,7 @@ def lib():
         L.or_mesh_unit_compliance.argtypes = [C.c_void_p, _dp, _dp]
         L.or_density_filter_setup.argtypes = [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
         L.or_density_filter.argtypes = [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
+        L.or_mesh_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.or_mesh_time_pcg_iterations.restype = C.c_double
         L.or_mesh_time_pcg_iterations.argtypes = [C.c_void_p, C.c_double, _dp, C.c_int,
                                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -281,6 +282,14 @@ class Mesh:
         ce = np.zeros(self.info[0]["numElements"])
         lib().or_mesh_unit_compliance(self._h, f64(U), ce)
         return ce
+
+    def stress(self, U):
+        """[cartesianStressField, vonMisesStressField] = FEA_StressAnalysis()"""
+        nn = self.info[0]["numNodes"]
+        st = np.zeros(6 * nn)
+        vm = np.zeros(nn)
+        lib().or_mesh_stress(self._h, f64(U), st, vm)
+        return st.reshape(nn, 6, order="F"), vm
 
     def time_pcg_iterations(self, b, omega, its):
         tk = C.c_double(0)

@@ -367,6 +367,7 @@ typedef struct {
 
 struct or_mesh {
     int numLevels, nonDyadic;
+    double nu, cellSize;
     double Ke[576];
     double *eleModulus;
     or_level L[OR_MAX_LEVELS];
@@ -511,6 +512,8 @@ or_mesh *or_mesh_create(int resX, int resY, int resZ, const uint8_t *voxels,
     M->numLevels = numLevels;
     M->nonDyadic = nonDyadic;
     or_element_stiffness(nu, cellSize, M->Ke);
+    M->nu = nu;
+    M->cellSize = cellSize;
     or_level *L1 = &M->L[0];
     L1->resX = resX; L1->resY = resY; L1->resZ = resZ; L1->span = 1;
     const int64_t nLat = (int64_t)resX * resY * resZ;
@@ -933,4 +936,112 @@ void or_density_filter_setup(double rMin, int64_t numElements, const int32_t *el
 void or_density_filter(const double *x, double rMin, int64_t numElements,
                        const int32_t *eleMapForward, int resX, int resY, int resZ, double *Y) {
     density_filter_impl(x, rMin, numElements, eleMapForward, resX, resY, resZ, Y);
+}
+
+
+/* ---------------------------- stress recovery ------------------------------ */
+/* FEA_StressAnalysis.m:1-66: per element the stresses at the 8 Gauss points
+ * E_e D B_g u_e (FEA_HookeLaw.m, FEA_ElementStrainMatrix.m; Gauss points in
+ * node order, FEA_GaussianIntegral.m), extrapolated to the element's nodes
+ * with the inverse of the shape-function matrix at the Gauss points
+ * (OuterInterpolationMat, :68-93), summed at the nodes and divided by the
+ * number of solid elements at each node; von Mises per node (:55-59).
+ * stress: numNodes x 6 column-major (xx yy zz yz zx xy). */
+static void or_invert8(const double A[8][8], double Ainv[8][8]) {
+    double M[8][16];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 16; ++j) M[i][j] = j < 8 ? A[i][j] : (j - 8 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 8; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 8; ++r)
+            if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < 16; ++j) { double tmp = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = tmp; }
+        const double d = M[c][c];
+        for (int j = 0; j < 16; ++j) M[c][j] /= d;
+        for (int r = 0; r < 8; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            for (int j = 0; j < 16; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) Ainv[i][j] = M[i][j + 8];
+}
+
+void or_mesh_stress(const or_mesh *M, const double *U, double *stress, double *vonMises) {
+    const or_level *L = &M->L[0];
+    const double g = 0.577350269189626, nu = M->nu, h = M->cellSize;
+    double S[6][6];
+    memset(S, 0, sizeof(S));
+    const double c1 = (1 + nu) * (1 - 2 * nu), c2 = 2 * (1 + nu);
+    S[0][0] = S[1][1] = S[2][2] = (1 - nu) / c1;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) S[i][j] = nu / c1;
+    S[3][3] = S[4][4] = S[5][5] = 1.0 / c2;
+    const double invJ = 2.0 / h;
+    double DB[8][6][24], N[8][8], Ninv[8][8];
+    for (int q = 0; q < 8; ++q) {
+        const double s = OR_CS[q] * g, t = OR_CT[q] * g, p = OR_CP[q] * g;
+        double B[6][24];
+        memset(B, 0, sizeof(B));
+        for (int j = 0; j < 8; ++j) {
+            N[q][j] = 0.125 * (1 + OR_CS[j] * s) * (1 + OR_CT[j] * t) * (1 + OR_CP[j] * p);
+            const double dNs = 0.125 * OR_CS[j] * (1 + OR_CT[j] * t) * (1 + OR_CP[j] * p);
+            const double dNt = 0.125 * OR_CT[j] * (1 + OR_CS[j] * s) * (1 + OR_CP[j] * p);
+            const double dNp = 0.125 * OR_CP[j] * (1 + OR_CS[j] * s) * (1 + OR_CT[j] * t);
+            const double dx = invJ * dNs, dy = invJ * dNt, dz = invJ * dNp;
+            B[0][3 * j] = dx;
+            B[1][3 * j + 1] = dy;
+            B[2][3 * j + 2] = dz;
+            B[3][3 * j + 1] = dz; B[3][3 * j + 2] = dy;
+            B[4][3 * j] = dz;     B[4][3 * j + 2] = dx;
+            B[5][3 * j] = dy;     B[5][3 * j + 1] = dx;
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int k = 0; k < 24; ++k) {
+                double acc = 0;
+                for (int l = 0; l < 6; ++l) acc += S[i][l] * B[l][k];
+                DB[q][i][k] = acc;
+            }
+    }
+    or_invert8(N, Ninv);
+    const int64_t nn = L->numNodes, ne = L->numElements;
+    double *acc = (double *)calloc((size_t)nn * 6, sizeof(double));
+    int *cnt = (int *)calloc((size_t)nn, sizeof(int));
+    for (int64_t e = 0; e < ne; ++e) {
+        double ue[24], sg[8][6];
+        for (int k = 0; k < 8; ++k) {
+            const int64_t node = L->eNodMat[e + ne * k] - 1;
+            for (int a = 0; a < 3; ++a) ue[3 * k + a] = U[3 * node + a];
+        }
+        for (int q = 0; q < 8; ++q)
+            for (int i = 0; i < 6; ++i) {
+                double v = 0;
+                for (int k = 0; k < 24; ++k) v += DB[q][i][k] * ue[k];
+                sg[q][i] = v * M->eleModulus[e];
+            }
+        for (int n = 0; n < 8; ++n) {
+            const int64_t node = L->eNodMat[e + ne * n] - 1;
+            for (int i = 0; i < 6; ++i) {
+                double v = 0;
+                for (int q = 0; q < 8; ++q) v += Ninv[n][q] * sg[q][i];
+                acc[node + nn * i] += v;
+            }
+            cnt[node] += 1;
+        }
+    }
+    for (int64_t n = 0; n < nn; ++n) {
+        double sv[6];
+        for (int i = 0; i < 6; ++i) {
+            sv[i] = cnt[n] ? acc[n + nn * i] / cnt[n] : 0.0;
+            stress[n + nn * i] = sv[i];
+        }
+        vonMises[n] = sqrt(0.5 * ((sv[0] - sv[1]) * (sv[0] - sv[1]) + (sv[1] - sv[2]) * (sv[1] - sv[2]) +
+                                  (sv[2] - sv[0]) * (sv[2] - sv[0])) +
+                           3 * (sv[5] * sv[5] + sv[3] * sv[3] + sv[4] * sv[4]));
+    }
+    free(acc);
+    free(cnt);
 }

@@ -113,6 +113,10 @@ void or_density_filter_setup(double rMin, int64_t numElements, const int32_t *el
 void or_density_filter(const double *x, double rMin, int64_t numElements,
                        const int32_t *eleMapForward, int resX, int resY, int resZ, double *Y);
 
+/* FEA_StressAnalysis.m:1-93: nodal Cartesian stresses (numNodes x 6,
+ * column-major, xx yy zz yz zx xy) and von Mises stress (numNodes). */
+void or_mesh_stress(const or_mesh *m, const double *U, double *stress, double *vonMises);
+
 /* Timing helper for the CPU baseline: runs `its` PCG iterations exactly
  * (no early exit) and returns seconds spent; setup excluded. */
 double or_mesh_time_pcg_iterations(const or_mesh *m, double omega, const double *b,

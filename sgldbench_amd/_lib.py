@@ -61,6 +61,8 @@ SIGNATURES = {
     "sgld_density_filter_setup": [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
     "sgld_density_filter": [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
     "sgld_ctx_density_filter_device": [_vp, _vp, C.c_double, _vp],
+    "sgld_ctx_stress": [_vp, _dp, _dp, _dp],
+    "sgld_ctx_stress_device": [_vp, _vp, _vp, _vp],
     "sgld_ctx_stats": [_vp, _dp],
 }
 _RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}

@@ -89,6 +89,7 @@ constexpr int kCornerZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 // Host helpers shared by the stateless and stateful paths.
 void element_stiffness(double nu, double h, double *Ke);
+void stress_operator(double nu, double h, double *T);  // 48 x 24, row-major
 void transfer_table(int span, double *RI);          // (span+1)^3 x 8 col-major
 void child_prolongation(int span, int q, double p[8][8]);
 void galerkin_child_add(const double p[8][8], const double *K, double *out);

@@ -150,13 +150,6 @@ extern "C" int sgld_density_filter(const double *x, double rMin, int64_t numElem
 extern "C" int sgld_ctx_density_filter_device(sgld_ctx *C, const double *dx, double rMin, double *dY) {
     SGLD_ARG(C && dY, "sgld:arg: null argument");
     const Grid &g = C->lv[0].g;
-    if (!C->efwd.p) {  // voxel -> 1-based element id map of the finest level, built once
-        std::vector<int32_t> m(g.elems(), 0);
-        if (C->full)
-            for (int64_t v = 0; v < g.elems(); ++v) m[v] = (int32_t)(v + 1);
-        else
-            for (size_t i = 0; i < C->hEleMap.size(); ++i) m[C->hEleMap[i]] = (int32_t)(i + 1);
-        SGLD_TRY(C->efwd.upload(m.data(), m.size()));
-    }
+    SGLD_TRY(ensure_efwd(C));
     return run_filter(C->s, dx, rMin, C->efwd.p, g.nx, g.ny, g.nz, dY, C);
 }

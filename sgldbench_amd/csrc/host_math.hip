@@ -71,6 +71,65 @@ void element_stiffness(double nu, double h, double *Ke) {
     }
 }
 
+// Nodal-stress operator of one element for E = 1 (FEA_StressAnalysis.m:17-46):
+// T (48 x 24, row 6 n + k: stress component k at element-local node n) =
+// (N^-1 x I6) blockdiag(D) [B_1; ...; B_8], N the shape functions at the 8
+// Gauss points (OuterInterpolationMat, :68-93; Gauss points in node order).
+void stress_operator(double nu, double h, double *T) {
+    const double g = 0.577350269189626;
+    const int sgn[2] = {-1, 1};
+    double D[6][6] = {};
+    const double c1 = (1 + nu) * (1 - 2 * nu), c2 = 2 * (1 + nu);
+    D[0][0] = D[1][1] = D[2][2] = (1 - nu) / c1;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) D[i][j] = nu / c1;
+    D[3][3] = D[4][4] = D[5][5] = 1.0 / c2;
+    const double invJ = 2.0 / h;
+    double DB[8][6][24] = {};
+    double M[8][16];  // [N | I] for Gauss-Jordan inversion
+    for (int q = 0; q < 8; ++q) {
+        const double s = sgn[kCornerX[q]] * g, t = sgn[kCornerY[q]] * g, p = sgn[kCornerZ[q]] * g;
+        double B[6][24] = {};
+        for (int j = 0; j < 8; ++j) {
+            const int cs = sgn[kCornerX[j]], ct = sgn[kCornerY[j]], cp = sgn[kCornerZ[j]];
+            M[q][j] = 0.125 * (1 + cs * s) * (1 + ct * t) * (1 + cp * p);
+            M[q][8 + j] = q == j ? 1.0 : 0.0;
+            const double dx = invJ * 0.125 * cs * (1 + ct * t) * (1 + cp * p);
+            const double dy = invJ * 0.125 * ct * (1 + cs * s) * (1 + cp * p);
+            const double dz = invJ * 0.125 * cp * (1 + cs * s) * (1 + ct * t);
+            B[0][3 * j] = dx;
+            B[1][3 * j + 1] = dy;
+            B[2][3 * j + 2] = dz;
+            B[3][3 * j + 1] = dz; B[3][3 * j + 2] = dy;
+            B[4][3 * j] = dz;     B[4][3 * j + 2] = dx;
+            B[5][3 * j] = dy;     B[5][3 * j + 1] = dx;
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int k = 0; k < 24; ++k) {
+                double acc = 0;
+                for (int l = 0; l < 6; ++l) acc += D[i][l] * B[l][k];
+                DB[q][i][k] = acc;
+            }
+    }
+    for (int c = 0; c < 8; ++c) {  // N is well conditioned (entries (1 +- g)^3 / 8): no pivoting needed
+        const double d = M[c][c];
+        for (int j = 0; j < 16; ++j) M[c][j] /= d;
+        for (int r = 0; r < 8; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            for (int j = 0; j < 16; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int n = 0; n < 8; ++n)
+        for (int k = 0; k < 6; ++k)
+            for (int m = 0; m < 24; ++m) {
+                double acc = 0;
+                for (int q = 0; q < 8; ++q) acc += M[n][8 + q] * DB[q][k][m];
+                T[(6 * n + k) * 24 + m] = acc;
+            }
+}
+
 // Trilinear weights of block node n = rr + (s+1)cc + (s+1)^2 ll (row rr from
 // the top, so y = (s - rr)/s) on coarse corner J: the 'inNODE' tables of
 // src/Solving_Operator4MultiGridRestrictionAndInterpolation.m:4-31 (span 2)

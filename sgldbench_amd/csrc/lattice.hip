@@ -1025,6 +1025,61 @@ __global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ 
     if (threadIdx.x == 0) partial[blockIdx.x] = s * scale;
 }
 
+// Nodal stresses (FEA_StressAnalysis.m:33-59), node-centric: each solid node
+// gathers, from each solid element around it, E_e T_j u_e (T_j: the 6 rows of
+// the element nodal-stress operator for the node's local index j), averages
+// over those elements, and forms the von Mises stress.  stress: nn x 6
+// column-major in compressed node order.
+__global__ void __launch_bounds__(256) k_stress(const double *__restrict__ u, const double *__restrict__ E,
+                                                const int32_t *__restrict__ efwd, Grid gd,
+                                                const double *__restrict__ T, const int32_t *__restrict__ nodMap,
+                                                int64_t nn, double *__restrict__ stress, double *__restrict__ vm) {
+    const int64_t NY = gd.NY(), NX = gd.NX(), pl = NY * NX;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = nodMap ? nodMap[i] : i;
+        const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / pl);
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        int cnt = 0;
+        for (int dl = 0; dl < 2; ++dl)
+            for (int dc = 0; dc < 2; ++dc)
+                for (int dr = 0; dr < 2; ++dr) {
+                    const int er = r - 1 + dr, ec = c - 1 + dc, el = l - 1 + dl;
+                    if (er < 0 || er >= gd.ny || ec < 0 || ec >= gd.nx || el < 0 || el >= gd.nz) continue;
+                    const int64_t v = er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el;
+                    if (!efwd[v]) continue;
+                    const int j = corner_index(1 - dc, dr, 1 - dl);
+                    const double *Tj = T + 6 * j * 24;
+                    double s6[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+                        const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+                        const int zk = (k >= 4);
+                        const int64_t gk = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const double uv = u[3 * gk + a];
+#pragma unroll
+                            for (int q = 0; q < 6; ++q) s6[q] += Tj[q * 24 + 3 * k + a] * uv;
+                        }
+                    }
+                    const double Ee = E[v];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) acc[q] += Ee * s6[q];
+                    ++cnt;
+                }
+        double sv[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            sv[q] = cnt ? acc[q] / cnt : 0.0;
+            stress[i + nn * q] = sv[q];
+        }
+        vm[i] = sqrt(0.5 * ((sv[0] - sv[1]) * (sv[0] - sv[1]) + (sv[1] - sv[2]) * (sv[1] - sv[2]) +
+                            (sv[2] - sv[0]) * (sv[2] - sv[0])) +
+                     3.0 * (sv[5] * sv[5] + sv[3] * sv[3] + sv[4] * sv[4]));
+    }
+}
+
 }  // namespace
 }  // namespace sgld
 
@@ -2209,5 +2264,51 @@ extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, i
     }
     C->kbyuVariant = kbyuVariant;
     C->restrictTiled = restrictTiled;
+    return SGLD_OK;
+}
+
+
+// ------------------------------ stress recovery ------------------------------
+namespace sgld {
+int ensure_efwd(sgld_ctx *C) {
+    if (C->efwd.p) return SGLD_OK;
+    const Grid &g = C->lv[0].g;
+    std::vector<int32_t> m(g.elems(), 0);
+    if (C->full)
+        for (int64_t v = 0; v < g.elems(); ++v) m[v] = (int32_t)(v + 1);
+    else
+        for (size_t i = 0; i < C->hEleMap.size(); ++i) m[C->hEleMap[i]] = (int32_t)(i + 1);
+    return C->efwd.upload(m.data(), m.size());
+}
+}  // namespace sgld
+
+extern "C" int sgld_ctx_stress_device(sgld_ctx *C, const double *dU, double *dstress, double *dvm) {
+    SGLD_ARG(C && dU && dstress && dvm, "sgld:arg: null argument");
+    SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
+    SGLD_TRY(ensure_efwd(C));
+    if (!C->stressT.p) {
+        std::vector<double> T(48 * 24);
+        stress_operator(C->nu, C->h, T.data());
+        SGLD_TRY(C->stressT.upload(T.data(), T.size()));
+    }
+    const int64_t nn = C->lv[0].numNodes;
+    SGLD_TRY(to_lattice(C, dU, C->z.p));
+    hipLaunchKernelGGL(k_stress, dim3(bounded_blocks(nn, 256)), dim3(256), 0, C->s, C->z.p, C->E.p, C->efwd.p,
+                       C->lv[0].g, C->stressT.p, C->full ? nullptr : C->nodMap.p, nn, dstress, dvm);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_stress(sgld_ctx *C, const double *U, double *stress, double *vonMises) {
+    SGLD_ARG(C && U && stress && vonMises, "sgld:arg: null argument");
+    const int64_t nc = 3 * C->lv[0].numNodes, nn = C->lv[0].numNodes;
+    DevBuf<double> dU, ds, dv;
+    SGLD_TRY(dU.upload(U, nc));
+    SGLD_TRY(ds.alloc(6 * nn));
+    SGLD_TRY(dv.alloc(nn));
+    SGLD_TRY(sgld_ctx_stress_device(C, dU.p, ds.p, dv.p));
+    SGLD_HIP(hipMemcpy(stress, ds.p, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost));
+    SGLD_HIP(hipMemcpy(vonMises, dv.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
     return SGLD_OK;
 }

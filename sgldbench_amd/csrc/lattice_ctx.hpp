@@ -60,7 +60,8 @@ struct sgld_ctx {
     bool full = true;
     std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
     sgld::DevBuf<int32_t> nodMap, eleMap;
-    sgld::DevBuf<int32_t> efwd;  // voxel -> 1-based element id (0 void), built on first filter use
+    sgld::DevBuf<int32_t> efwd;  // voxel -> 1-based element id (0 void), built on first use
+    sgld::DevBuf<double> stressT;  // element nodal-stress operator (48 x 24), built on first use
     sgld::DevBuf<double> E;
     sgld::DevBuf<float> E32;
     sgld::DevBuf<double> Minv;
@@ -95,5 +96,6 @@ int check_ready(sgld_ctx *C);
 // r = b - K x on lattice FP64 vectors (fixed and void DOFs 0), with
 // rr = r.r and bb = b.b; host-synchronous.
 int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb);
+int ensure_efwd(sgld_ctx *C);  // voxel -> 1-based element id map of the finest level
 
 }  // namespace sgld

@@ -321,6 +321,16 @@ class FEASolver:
                                                 Emin, scale, C.c_void_p(ddc_ptr), C.byref(c)))
         return c.value
 
+    def stress(self, U):
+        """[cartesianStressField, vonMisesStressField] = FEA_StressAnalysis():
+        (numNodes, 6) stresses (xx yy zz yz zx xy) and (numNodes,) von Mises."""
+        U = _f64(U).ravel()
+        nn = self.levels[0]["numNodes"]
+        st = np.zeros(6 * nn)
+        vm = np.zeros(nn)
+        check(lib().sgld_ctx_stress(self._h, U, st, vm))
+        return st.reshape(nn, 6, order="F"), vm
+
     def density_filter_device(self, dx_ptr, rMin, dy_ptr):
         """Density filter on this mesh with device vectors; dx_ptr 0 gives Hs."""
         check(lib().sgld_ctx_density_filter_device(self._h, C.c_void_p(dx_ptr) if dx_ptr else None, float(rMin),
