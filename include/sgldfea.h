@@ -244,6 +244,19 @@ int sgld_ctx_density_filter_device(sgld_ctx *ctx, const double *dx, double rMin,
 int sgld_ctx_stress(sgld_ctx *ctx, const double *U, double *stress, double *vonMises);
 int sgld_ctx_stress_device(sgld_ctx *ctx, const double *dU, double *dstress, double *dvonMises);
 
+/* tar = TopOpti_PDEFiltering_matrixFree(src) after
+ * TopOpti_BuildPDEfilter_matrixFree() (src/TopOpti_PDEFiltering_matrixFree.m:
+ * 1-56, src/TopOpti_BuildPDEfilter_matrixFree.m:7-37): element values to
+ * nodes, Jacobi-preconditioned CG on (iRmin^2 K + M) with the 8 x 8 element
+ * matrix of rHatMin (Solving_KbyU_MatrixFree8x8_mex.cpp product), tolerance
+ * tol and maxIT as tol_ / maxIT_ (defaults 1e-3 / 500), nodes back to
+ * elements.  src, tar: numElements; its: CG iterations.  Host pointers;
+ * _device: device pointers. */
+int sgld_ctx_pde_filter(sgld_ctx *ctx, const double *src, double rHatMin, double tol, int maxIT,
+                        double *tar, int *its);
+int sgld_ctx_pde_filter_device(sgld_ctx *ctx, const double *dsrc, double rHatMin, double tol,
+                               int maxIT, double *dtar, int *its);
+
 /* Statistics of the last solve: out[0] solve seconds, [1] iterations (inner
  * total), [2] true relres, [3] outer refinements, [4] setup seconds of the
  * last assemble, [5] milliseconds of the last density-filter kernel

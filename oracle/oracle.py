@@ -64,6 +64,8 @@ def lib():
         L.or_density_filter_setup.argtypes = [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
         L.or_density_filter.argtypes = [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp]
         L.or_mesh_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_pde_element.argtypes = [C.c_double, _dp]
+        L.or_mesh_pde_filter.argtypes = [C.c_void_p, C.c_double, _dp, C.c_double, C.c_int, _dp]
         L.or_mesh_time_pcg_iterations.restype = C.c_double
         L.or_mesh_time_pcg_iterations.argtypes = [C.c_void_p, C.c_double, _dp, C.c_int,
                                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -163,6 +165,12 @@ def unit_compliance_mex(U, eNodMat, Ke):
     lib().or_unit_compliance_mex(f64(U), i32(eNodMat.ravel(order="F")), ne,
                                  f64(np.asarray(Ke).ravel(order="F")), ce)
     return ce
+
+
+def pde_element(rHatMin):
+    Ke = np.zeros(64)
+    lib().or_pde_element(rHatMin, Ke)
+    return Ke.reshape(8, 8, order="F")
 
 
 def density_filter_setup(rMin, eleMapForward, resX, resY, resZ):
@@ -290,6 +298,12 @@ class Mesh:
         vm = np.zeros(nn)
         lib().or_mesh_stress(self._h, f64(U), st, vm)
         return st.reshape(nn, 6, order="F"), vm
+
+    def pde_filter(self, src, rHatMin=8.0, tol=1e-3, max_it=500):
+        """tar = TopOpti_PDEFiltering_matrixFree(src); returns (tar, its)."""
+        tar = np.zeros(self.info[0]["numElements"])
+        its = lib().or_mesh_pde_filter(self._h, rHatMin, f64(src), tol, max_it, tar)
+        return tar, its
 
     def time_pcg_iterations(self, b, omega, its):
         tk = C.c_double(0)

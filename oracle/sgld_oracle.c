@@ -1045,3 +1045,100 @@ void or_mesh_stress(const or_mesh *M, const double *U, double *stress, double *v
     free(acc);
     free(cnt);
 }
+
+
+/* ------------------------------- PDE filter -------------------------------- */
+/* TopOpti_BuildPDEfilter_matrixFree.m:7-37: KePDE = iRmin^2 dShape' dShape +
+ * N' diag(w detJ) N with iRmin = rHatMin / 2 / sqrt(3) (unit cell, detJ 1/8),
+ * dShape the parametric shape derivatives at the 8 Gauss points
+ * (FEA_DeShapeFunction.m); Jacobi preconditioner 1 / sum of diag(KePDE). */
+void or_pde_element(double rHatMin, double *Ke8) {
+    const double g = 0.577350269189626;
+    double KEF0[8][8], KEF1[8][8];
+    memset(KEF0, 0, sizeof(KEF0));
+    memset(KEF1, 0, sizeof(KEF1));
+    for (int q = 0; q < 8; ++q) {
+        const double s = OR_CS[q] * g, t = OR_CT[q] * g, p = OR_CP[q] * g;
+        double N[8], dN[3][8];
+        for (int j = 0; j < 8; ++j) {
+            N[j] = 0.125 * (1 + OR_CS[j] * s) * (1 + OR_CT[j] * t) * (1 + OR_CP[j] * p);
+            dN[0][j] = 0.125 * OR_CS[j] * (1 + OR_CT[j] * t) * (1 + OR_CP[j] * p);
+            dN[1][j] = 0.125 * OR_CT[j] * (1 + OR_CS[j] * s) * (1 + OR_CP[j] * p);
+            dN[2][j] = 0.125 * OR_CP[j] * (1 + OR_CS[j] * s) * (1 + OR_CT[j] * t);
+        }
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) {
+                for (int d = 0; d < 3; ++d) KEF0[a][b] += dN[d][a] * dN[d][b];
+                KEF1[a][b] += N[a] * (1.0 / 8.0) * N[b];
+            }
+    }
+    const double iRmin = rHatMin / 2.0 / sqrt(3.0);
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) Ke8[a + 8 * b] = iRmin * iRmin * KEF0[a][b] + KEF1[a][b];
+}
+
+/* TopOpti_PDEFiltering_matrixFree.m:1-33 with MatTimesVec_matrixFree (:35-55):
+ * element values to nodes (sum of src/8 over the elements at a node), PCG
+ * (Solving_PreconditionedConjugateGradientSolver.m, x0 = 0) with the Jacobi
+ * preconditioner, nodes back to elements (mean of the 8 nodes).  Returns the
+ * iteration count. */
+int or_mesh_pde_filter(const or_mesh *M, double rHatMin, const double *src, double tol, int maxIT,
+                       double *tar) {
+    const or_level *L = &M->L[0];
+    const int64_t nn = L->numNodes, ne = L->numElements;
+    double Ke8[64];
+    or_pde_element(rHatMin, Ke8);
+    double *b = (double *)calloc(nn, sizeof(double)), *x = (double *)calloc(nn, sizeof(double));
+    double *r = (double *)calloc(nn, sizeof(double)), *z = (double *)calloc(nn, sizeof(double));
+    double *p = (double *)calloc(nn, sizeof(double)), *q = (double *)calloc(nn, sizeof(double));
+    double *dinv = (double *)calloc(nn, sizeof(double));
+    for (int jj = 0; jj < 8; ++jj)
+        for (int64_t e = 0; e < ne; ++e) {
+            const int64_t n = L->eNodMat[e + ne * jj] - 1;
+            b[n] += src[e] * (1.0 / 8.0);
+            dinv[n] += Ke8[jj + 8 * jj];
+        }
+    for (int64_t n = 0; n < nn; ++n) dinv[n] = 1.0 / dinv[n];
+    double bb = 0;
+    for (int64_t n = 0; n < nn; ++n) bb += b[n] * b[n];
+    const double normB = sqrt(bb);
+    for (int64_t n = 0; n < nn; ++n) { r[n] = b[n]; z[n] = dinv[n] * r[n]; p[n] = z[n]; }
+    double rho = 0;
+    for (int64_t n = 0; n < nn; ++n) rho += z[n] * r[n];
+    int its = 0;
+    for (its = 1; its <= maxIT; ++its) {
+        memset(q, 0, sizeof(double) * nn);
+        for (int64_t e = 0; e < ne; ++e) {
+            double ue[8];
+            int64_t nodes[8];
+            for (int k = 0; k < 8; ++k) { nodes[k] = L->eNodMat[e + ne * k] - 1; ue[k] = p[nodes[k]]; }
+            for (int j = 0; j < 8; ++j) {
+                double y = 0;
+                for (int k = 0; k < 8; ++k) y += Ke8[j + 8 * k] * ue[k];
+                q[nodes[j]] += y;
+            }
+        }
+        double pq = 0;
+        for (int64_t n = 0; n < nn; ++n) pq += p[n] * q[n];
+        const double lambda = rho / pq;
+        double rr = 0;
+        for (int64_t n = 0; n < nn; ++n) {
+            x[n] += lambda * p[n];
+            r[n] -= lambda * q[n];
+            rr += r[n] * r[n];
+        }
+        if (normB == 0.0 || sqrt(rr) / normB < tol) break;
+        double rho2 = 0;
+        for (int64_t n = 0; n < nn; ++n) { z[n] = dinv[n] * r[n]; rho2 += z[n] * r[n]; }
+        for (int64_t n = 0; n < nn; ++n) p[n] = z[n] + (rho2 / rho) * p[n];
+        rho = rho2;
+    }
+    if (its > maxIT) its = maxIT;
+    for (int64_t e = 0; e < ne; ++e) {
+        double sum = 0;
+        for (int k = 0; k < 8; ++k) sum += x[L->eNodMat[e + ne * k] - 1];
+        tar[e] = sum * (1.0 / 8.0);
+    }
+    free(b); free(x); free(r); free(z); free(p); free(q); free(dinv);
+    return its;
+}

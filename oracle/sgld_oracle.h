@@ -117,6 +117,13 @@ void or_density_filter(const double *x, double rMin, int64_t numElements,
  * column-major, xx yy zz yz zx xy) and von Mises stress (numNodes). */
 void or_mesh_stress(const or_mesh *m, const double *U, double *stress, double *vonMises);
 
+/* PDE (Helmholtz) density filter, TopOpti_BuildPDEfilter_matrixFree.m and
+ * TopOpti_PDEFiltering_matrixFree.m: element matrix (8 x 8, column-major) and
+ * the filter itself (element values in and out); returns the PCG iterations. */
+void or_pde_element(double rHatMin, double *Ke8);
+int or_mesh_pde_filter(const or_mesh *m, double rHatMin, const double *src, double tol, int maxIT,
+                       double *tar);
+
 /* Timing helper for the CPU baseline: runs `its` PCG iterations exactly
  * (no early exit) and returns seconds spent; setup excluded. */
 double or_mesh_time_pcg_iterations(const or_mesh *m, double omega, const double *b,

@@ -63,6 +63,8 @@ SIGNATURES = {
     "sgld_ctx_density_filter_device": [_vp, _vp, C.c_double, _vp],
     "sgld_ctx_stress": [_vp, _dp, _dp, _dp],
     "sgld_ctx_stress_device": [_vp, _vp, _vp, _vp],
+    "sgld_ctx_pde_filter": [_vp, _dp, C.c_double, C.c_double, C.c_int, _dp, C.POINTER(C.c_int)],
+    "sgld_ctx_pde_filter_device": [_vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],
     "sgld_ctx_stats": [_vp, _dp],
 }
 _RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}

@@ -5,7 +5,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgldfea.so")
-SOURCES = ["host_math.hip", "mex_ops.hip", "lattice.hip", "batched.hip", "filter.hip"]
+SOURCES = ["host_math.hip", "mex_ops.hip", "lattice.hip", "batched.hip", "filter.hip", "pde_filter.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

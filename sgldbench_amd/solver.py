@@ -331,6 +331,14 @@ class FEASolver:
         check(lib().sgld_ctx_stress(self._h, U, st, vm))
         return st.reshape(nn, 6, order="F"), vm
 
+    def pde_filter(self, src, rHatMin=8.0, tol=1e-3, maxIT=500):
+        """tar = TopOpti_PDEFiltering_matrixFree(src); returns (tar, its)."""
+        src = _f64(src).ravel()
+        tar = np.zeros(self.numElements)
+        its = C.c_int(0)
+        check(lib().sgld_ctx_pde_filter(self._h, src, float(rHatMin), float(tol), int(maxIT), tar, C.byref(its)))
+        return tar, its.value
+
     def density_filter_device(self, dx_ptr, rMin, dy_ptr):
         """Density filter on this mesh with device vectors; dx_ptr 0 gives Hs."""
         check(lib().sgld_ctx_density_filter_device(self._h, C.c_void_p(dx_ptr) if dx_ptr else None, float(rMin),
