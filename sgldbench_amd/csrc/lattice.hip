@@ -48,6 +48,13 @@ __device__ __forceinline__ double block_sum(double v) {
 
 __device__ __forceinline__ bool stopped(const Ctl *ctl) { return ctl && ctl->done; }
 
+// Grid-stride passes over the entries of a Span (all entries, or the listed
+// 64-entry chunks of an irregular domain).
+#define FOR_SPAN(i, S)                                                                           \
+    for (int64_t t_ = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t_ < (S).count;            \
+         t_ += (int64_t)gridDim.x * blockDim.x)                                                    \
+        if (const int64_t i = (S).at(t_); i < (S).n)
+
 // ----------------------------------------------------------------------------
 // K.u: y = sum_e E_e Ke u_e at every lattice node, gathered node-centrically
 // over the node's 8 incident voxels (no atomics, no colours).  Fixed and void
@@ -57,14 +64,13 @@ __device__ __forceinline__ bool stopped(const Ctl *ctl) { return ctl && ctl->don
 template <class T>
 __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__restrict__ y,
                                               const T *__restrict__ E,
-                                              const uint8_t *__restrict__ nflag, Grid gd,
+                                              const uint8_t *__restrict__ nflag, Grid gd, Span S,
                                               double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t NY = gd.NY(), NX = gd.NX(), nN = gd.nodes();
+    const int64_t NY = gd.NY(), NX = gd.NX();
     const int64_t pl = NY * NX;
     double dotacc = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nN;
-         g += (int64_t)gridDim.x * blockDim.x) {
+    FOR_SPAN(g, S) {
         const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / pl);
         const uint8_t f = nflag[g];
         T acc[3] = {0, 0, 0};
@@ -203,7 +209,10 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
 // Persistent, XCD-aware grid: block b runs on XCD b % 8 and walks the work
 // items (patch x z-chunk) of that XCD's contiguous share, so patches sharing
 // halo nodes run behind the same L2.  Node layers are prefetched two
-// iterations ahead into registers and staged in a 3-slot LDS ring.
+// iterations ahead into registers and staged in a 3-slot LDS ring.  On an
+// irregular domain `wlist` holds the (ascending) work items whose output
+// nodes touch a solid element; the others would only write zeros over void
+// nodes that are zero already.
 // ----------------------------------------------------------------------------
 
 template <class T>
@@ -229,7 +238,8 @@ template <class T>
 __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
-                                                       TileMap tm, double *__restrict__ partial,
+                                                       TileMap tm, const int32_t *__restrict__ wlist,
+                                                       int64_t wn, double *__restrict__ partial,
                                                        const Ctl *ctl) {
     if (stopped(ctl)) return;
     __shared__ T us[3][kTN * kTN * 3];
@@ -239,7 +249,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
     const int64_t pl = (int64_t)NY * NX;       // nodes per layer
     const int64_t ple = (int64_t)gd.ny * gd.nx;  // elements per layer
     const int ei = t % kTE, ej = t / kTE;
-    const int64_t W = tm.work();
+    const int64_t W = wlist ? wn : tm.work();  // work items: all, or the listed ones
     const int G = gridDim.x;
     const int nxcd = G < kXCD ? G : kXCD;  // XCDs that hold at least one block
     const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
@@ -255,7 +265,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
     const int fo = 3 * ((oi + 1) + kTN * (oj + 1));
     double dotacc = 0.0;
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
-        int64_t rem = w;
+        int64_t rem = wlist ? wlist[w] : w;
         const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
         rem -= (int64_t)tz * tm.tilesR * tm.tilesC;
         const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
@@ -583,14 +593,13 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
                                                  T *__restrict__ xf, Grid gf,
                                                  const T *__restrict__ rf,
                                                  const T *__restrict__ dinv,
-                                                 const uint8_t *__restrict__ fflag, int finest,
+                                                 const uint8_t *__restrict__ fflag, int finest, Span nodes,
                                                  double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t NYf = gf.NY(), NXf = gf.NX(), nN = gf.nodes();
+    const int64_t NYf = gf.NY(), NXf = gf.NX();
     const int64_t NYc = gc.NY(), NXc = gc.NX();
     double dotacc = 0.0;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nN;
-         n += (int64_t)gridDim.x * blockDim.x) {
+    FOR_SPAN(n, nodes) {
         const int r = (int)(n % NYf), c = (int)((n / NYf) % NXf), l = (int)(n / (NYf * NXf));
         const uint8_t f = fflag[n];
         T v0 = 0, v1 = 0, v2 = 0;
@@ -649,25 +658,19 @@ __global__ void __launch_bounds__(256) k_coarse_gemv(const double *__restrict__ 
 
 // ----------------------------- PCG vector kernels ---------------------------
 template <class T>
-__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b,
-                                             int64_t n, double *__restrict__ partial,
-                                             const Ctl *ctl) {
+__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b, Span S,
+                                             double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        acc += (double)a[i] * (double)b[i];
+    FOR_SPAN(i, S) acc += (double)a[i] * (double)b[i];
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // r = b - q (q may be null: r = b).
 template <class T>
-__global__ void k_residual(const T *__restrict__ b, const T *__restrict__ q, T *__restrict__ r,
-                           int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        r[i] = q ? b[i] - q[i] : b[i];
+__global__ void k_residual(const T *__restrict__ b, const T *__restrict__ q, T *__restrict__ r, Span S) {
+    FOR_SPAN(i, S) r[i] = q ? b[i] - q[i] : b[i];
 }
 
 // Solving_PreconditionedConjugateGradientSolver.m:15-18:
@@ -675,14 +678,13 @@ __global__ void k_residual(const T *__restrict__ b, const T *__restrict__ q, T *
 template <class T>
 __global__ void __launch_bounds__(256) k_update_xr(T *__restrict__ x, T *__restrict__ r,
                                                    const T *__restrict__ p,
-                                                   const T *__restrict__ q, int64_t n,
+                                                   const T *__restrict__ q, Span S,
                                                    double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
     const double lam = ctl->lambda;
     const T lt = (T)lam;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    FOR_SPAN(i, S) {
         x[i] = x[i] + lt * p[i];
         const T rn = r[i] - lt * q[i];
         r[i] = rn;
@@ -695,51 +697,37 @@ __global__ void __launch_bounds__(256) k_update_xr(T *__restrict__ x, T *__restr
 // :28 p = z + (x2/x1) p, into the other p buffer
 template <class T>
 __global__ void k_update_p_into(T *__restrict__ pnew, const T *__restrict__ z,
-                                const T *__restrict__ pold, int64_t n, const Ctl *ctl) {
+                                const T *__restrict__ pold, Span S, const Ctl *ctl) {
     if (stopped(ctl)) return;
     const T beta = (T)ctl->beta;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        pnew[i] = z[i] + beta * pold[i];
+    FOR_SPAN(i, S) pnew[i] = z[i] + beta * pold[i];
 }
 
 template <class T>
-__global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+__global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, Span S) {
+    FOR_SPAN(i, S) dst[i] = src[i];
 }
 
 // dst = (To)(src * scale)
 template <class To, class Ti>
-__global__ void k_convert(To *__restrict__ dst, const Ti *__restrict__ src, double scale,
-                          int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = (To)((double)src[i] * scale);
+__global__ void k_convert(To *__restrict__ dst, const Ti *__restrict__ src, double scale, Span S) {
+    FOR_SPAN(i, S) dst[i] = (To)((double)src[i] * scale);
+}
+
+template <class T>
+__global__ void k_fill(T *__restrict__ v, T val, Span S) {
+    FOR_SPAN(i, S) v[i] = val;
 }
 
 // v *= f (in place)
 template <class T>
-__global__ void k_fill(T *__restrict__ v, T val, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        v[i] = val;
-}
-
-template <class T>
-__global__ void k_scale(T *v, T f, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        v[i] *= f;
+__global__ void k_scale(T *v, T f, Span S) {
+    FOR_SPAN(i, S) v[i] *= f;
 }
 
 // x += scale * d  (FP64 x, FP32 correction d)
-__global__ void k_axpy_mixed(double *__restrict__ x, const float *__restrict__ d, double scale,
-                             int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        x[i] += scale * (double)d[i];
+__global__ void k_axpy_mixed(double *__restrict__ x, const float *__restrict__ d, double scale, Span S) {
+    FOR_SPAN(i, S) x[i] += scale * (double)d[i];
 }
 
 enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_PQ, FIN_RR, FIN_RHO };
@@ -1111,21 +1099,24 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     const Level &l0 = C->lv[0];
     int nb;
     if (zin) {  // the direction update runs as its own bandwidth-bound pass
-        const int64_t n = 3 * l0.g.nodes();
-        hipLaunchKernelGGL(k_update_p_into<T>, dim3(grid_blocks(n)), dim3(256), 0, C->s, pout, zin, u, n,
+        const Span S = dof_span(C);
+        hipLaunchKernelGGL(k_update_p_into<T>, dim3(grid_blocks(S.count)), dim3(256), 0, C->s, pout, zin, u, S,
                            ctl);
         SGLD_HIP(hipGetLastError());
         u = pout;
     }
     if (C->kbyuVariant == 1) {
-        const TiledSchedule &ts = C->ksched[sizeof(T) == 8 ? 0 : 1];
+        const int q = sizeof(T) == 8 ? 0 : 1;
+        const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
         hipLaunchKernelGGL(k_kbyu_tiled<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
-                           l0.nflag.p, l0.g, ts.tm, partial, ctl);
+                           l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
+                           C->nKwork[q], partial, ctl);
     } else {
-        nb = grid_blocks(l0.g.nodes());
+        const Span S = node_span(C);
+        nb = grid_blocks(S.count);
         hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
-                           l0.g, partial, ctl);
+                           l0.g, S, partial, ctl);
     }
     SGLD_HIP(hipGetLastError());
     if (nbOut) *nbOut = nb;
@@ -1218,14 +1209,15 @@ template <class T>
 int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double *partial,
                     const Ctl *ctl, int *nbOut) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
-    const int nb = grid_blocks(F.g.nodes());
     const int finest = (k == 1);
+    const Span S = finest ? node_span(C) : Span::all(F.g.nodes());
+    const int nb = grid_blocks(S.count);
     if (Cc.span == 2)
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
-                           lvl_dinv<T>(F), F.nflag.p, finest, partial, ctl);
+                           lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
     else
         hipLaunchKernelGGL((k_prolong<T, 4>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
-                           lvl_dinv<T>(F), F.nflag.p, finest, partial, ctl);
+                           lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
     SGLD_HIP(hipGetLastError());
     if (nbOut) *nbOut = nb;
     return SGLD_OK;
@@ -1264,13 +1256,13 @@ int fin(sgld_ctx *C, const double *partial, int nb, int op) {
 // (pold), writes the new p (pnew) and q = K pnew.
 template <class T>
 int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *z, const T *pold, T *pnew, T *q) {
-    const int64_t n = C->nDof();
+    const Span S = dof_span(C);
     Ctl *ctl = C->ctl.p;
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, pold, q, slot(C, 0), ctl, &nb, z, pnew));
     SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
-    const int nbv = grid_blocks(n);
-    hipLaunchKernelGGL(k_update_xr<T>, dim3(nbv), dim3(256), 0, C->s, x, r, (const T *)pnew, (const T *)q, n,
+    const int nbv = grid_blocks(S.count);
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(nbv), dim3(256), 0, C->s, x, r, (const T *)pnew, (const T *)q, S,
                        slot(C, 1), ctl);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
@@ -1302,22 +1294,22 @@ struct SolveState {
 
 int fp64_begin(sgld_ctx *C, SolveState &S) {
     // Solving_PreconditionedConjugateGradientSolver.m:7-12 on lattice vectors (x holds x0)
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     *C->hctl = Ctl{};
     C->hctl->tol = S.tol;
     C->hctl->maxit = S.maxit;
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->b.p, n, slot(C, 3), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->b.p, V, slot(C, 3), nullptr);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 3), nbv, FIN_NORMB));
     SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, n);
+    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, V);
     SGLD_HIP(hipGetLastError());
     int nb = 0;
     SGLD_TRY(enqueue_vcycle<double>(C, C->r.p, C->z.p, slot(C, 2), nullptr, &nb));
     SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));  // rho = z.r, beta = 0: first p = z
-    hipLaunchKernelGGL(k_fill<double>, dim3(nbv), dim3(256), 0, C->s, C->p.p, 0.0, n);  // beta 0 x p
+    hipLaunchKernelGGL(k_fill<double>, dim3(nbv), dim3(256), 0, C->s, C->p.p, 0.0, V);  // beta 0 x p
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
     SGLD_HIP(hipStreamSynchronize(C->s));
     if (!(C->hctl->normb > 0.0)) {  // F = 0: solution 0, nothing to iterate
@@ -1330,8 +1322,8 @@ int fp64_begin(sgld_ctx *C, SolveState &S) {
 }
 
 int mixed_begin(sgld_ctx *C, SolveState &S) {
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     double res = 0, normr = 0;
     SGLD_TRY(true_residual(C, &res, &normr));
     S.its = 0;
@@ -1350,8 +1342,8 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     S.scale = normr;
     S.itsAtUpdate = 0;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
-                       1.0 / S.scale, n);
-    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, n);
+                       1.0 / S.scale, V);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
     SGLD_HIP(hipGetLastError());
     *C->hctl = Ctl{};
     C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
@@ -1361,7 +1353,7 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     int nb = 0;
     SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
     SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));
-    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, n);  // beta 0 x p
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, V);  // beta 0 x p
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
 }
@@ -1403,17 +1395,17 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     // when its recursive residual has dropped by delta (or after kMaxStretch
     // iterations), fold the FP32 correction into the FP64 solution and
     // replace the residual by the FP64 true residual (reliable update).
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     // the batch alternates p32 -> p32b -> p32 ...: after an odd number of its
     // iterations the latest direction is in p32b; the next batch starts from p32
     if ((st.its - S.its) % 2)
-        hipLaunchKernelGGL(k_copy<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (const float *)C->p32b.p, n);
+        hipLaunchKernelGGL(k_copy<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (const float *)C->p32b.p, V);
     S.its = st.its;
     if (!st.done && S.its - S.itsAtUpdate < SolveState::kMaxStretch) return SGLD_OK;
     S.itsAtUpdate = S.its;
-    hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, S.scale, n);
-    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, n);
+    hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, S.scale, V);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
     SGLD_HIP(hipGetLastError());
     double res = 0, normr = 0;
     SGLD_TRY(true_residual(C, &res, &normr));
@@ -1427,8 +1419,8 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     const double f = S.scale / normr;
     S.scale = normr;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
-                       1.0 / S.scale, n);
-    hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (float)f, n);
+                       1.0 / S.scale, V);
+    hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (float)f, V);
     SGLD_HIP(hipGetLastError());
     *C->hctl = st;
     C->hctl->done = 0;
@@ -1501,14 +1493,14 @@ int true_residual(sgld_ctx *C, double *res, double *normr) {
 namespace sgld {
 
 int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb) {
-    const int64_t n = C->nDof();
-    const int nbv = grid_blocks(n);
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     SGLD_TRY(enqueue_kbyu<double>(C, x, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, n);
+    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, V);
     SGLD_HIP(hipGetLastError());
     std::vector<double> part(2 * kRedBlocks);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, r, r, n, slot(C, 0), nullptr);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, b, b, n, slot(C, 1), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, r, r, V, slot(C, 0), nullptr);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, b, b, V, slot(C, 1), nullptr);
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
                             hipMemcpyDeviceToHost, C->s));
@@ -1517,6 +1509,29 @@ int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double 
     for (int i = 0; i < nbv; ++i) { a += part[i]; c += part[kRedBlocks + i]; }
     *rr = a;
     *bb = c;
+    return SGLD_OK;
+}
+
+int set_kchunk(sgld_ctx *C, int q, int cz) {
+    TiledSchedule &ts = C->ksched[q];
+    const Grid &g = C->lv[0].g;
+    ts.set_chunk((int)g.NZ(), cz);
+    if (C->full) return SGLD_OK;
+    // Work item w = tz P + (tr + tilesR tc) writes node layers [tz cz, tz cz + cz)
+    // of its patch; the elements at those nodes are in layers [tz cz - 1, tz cz + cz).
+    const int64_t P = (int64_t)ts.tm.tilesR * ts.tm.tilesC;
+    std::vector<int32_t> items;
+    for (int tz = 0; tz < ts.tm.tilesZ; ++tz) {
+        const int l0 = std::max(tz * cz - 1, 0), l1 = std::min(tz * cz + cz, g.nz);
+        for (int64_t p = 0; p < P; ++p) {
+            bool any = false;
+            for (int l = l0; l < l1 && !any; ++l) any = C->hPatchLayer[l * P + p] != 0;
+            if (any) items.push_back((int32_t)(tz * P + p));
+        }
+    }
+    C->nKwork[q] = (int64_t)items.size();
+    SGLD_TRY(C->kwork[q].upload(items.data(), items.size()));
+    ts.grid = (int)std::max<int64_t>(1, std::min<int64_t>(C->nKwork[q], ts.cap));
     return SGLD_OK;
 }
 
@@ -1634,6 +1649,39 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             if (f0.esolid[v]) C->hEleMap.push_back((int32_t)v);
         SGLD_TRY(C->nodMap.upload(C->hNodMap.data(), C->hNodMap.size()));
         SGLD_TRY(C->eleMap.upload(C->hEleMap.data(), C->hEleMap.size()));
+        // 64-entry chunks of the finest node and DOF vectors that hold a solid node
+        const Grid &g0 = f0.g;
+        const int64_t nNodes = g0.nodes();
+        std::vector<int32_t> nodeCh, dofCh;
+        for (int64_t c = 0; 64 * c < nNodes; ++c) {
+            bool any = false;
+            for (int64_t g = 64 * c; g < std::min(64 * c + 64, nNodes) && !any; ++g) any = f0.hflag[g] & kSolid;
+            if (any) nodeCh.push_back((int32_t)c);
+        }
+        for (int64_t c = 0; 64 * c < 3 * nNodes; ++c) {
+            bool any = false;
+            for (int64_t g = 64 * c / 3; g <= std::min(64 * c + 63, 3 * nNodes - 1) / 3 && !any; ++g)
+                any = f0.hflag[g] & kSolid;
+            if (any) dofCh.push_back((int32_t)c);
+        }
+        C->nNodeChunks = (int64_t)nodeCh.size();
+        C->nDofChunks = (int64_t)dofCh.size();
+        SGLD_TRY(C->nodeChunks.upload(nodeCh.data(), nodeCh.size()));
+        SGLD_TRY(C->dofChunks.upload(dofCh.data(), dofCh.size()));
+        // element layers of each tiled K.u patch window (element rows R0-1 .. R0+kTO-1
+        // and columns C0-1 .. C0+kTO-1 of patch (tr, tc), R0 = kTO tr, C0 = kTO tc)
+        // that hold a solid element
+        const int tilesR = (int)((g0.NY() + kTO - 1) / kTO), tilesC = (int)((g0.NX() + kTO - 1) / kTO);
+        const int64_t P = (int64_t)tilesR * tilesC;
+        C->hPatchLayer.assign(P * g0.nz, 0);
+        for (int64_t v = 0; v < g0.elems(); ++v) {
+            if (!f0.esolid[v]) continue;
+            const int r = (int)(v % g0.ny), c = (int)((v / g0.ny) % g0.nx);
+            const int64_t l = v / ((int64_t)g0.ny * g0.nx);
+            for (int tc = c / kTO; tc <= std::min(tilesC - 1, (c + 1) / kTO); ++tc)
+                for (int tr = r / kTO; tr <= std::min(tilesR - 1, (r + 1) / kTO); ++tr)
+                    C->hPatchLayer[l * P + tr + (int64_t)tilesR * tc] = 1;
+        }
     }
     SGLD_HIP(hipStreamCreateWithFlags(&C->s, hipStreamNonBlocking));
     SGLD_HIP(hipEventCreate(&C->e0));
@@ -1692,7 +1740,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             ts.tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
             ts.cap = (int)std::min<int64_t>((int64_t)ncu * std::max(1, per[q]), kRedBlocks);
             // z-chunks of 8 node layers: measured best for both precisions on 256x128x128
-            ts.set_chunk((int)g0.NZ(), 8);
+            SGLD_TRY(set_kchunk(C.get(), q, 8));
         }
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
@@ -2003,14 +2051,14 @@ extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, i
         SGLD_HIP(hipEventRecord(C->e1, C->s));
         SGLD_TRY(from_lattice(C, C->q.p, dy));
     } else {
-        const int64_t n = C->nDof();
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, C->s, C->z32.p,
-                           C->z.p, 1.0, n);
+        const Span V = dof_span(C);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(V.count)), dim3(256), 0, C->s, C->z32.p,
+                           C->z.p, 1.0, V);
         SGLD_HIP(hipEventRecord(C->e0, C->s));
         SGLD_TRY(enqueue_kbyu<float>(C, C->z32.p, C->q32.p, nullptr, nullptr, nullptr));
         SGLD_HIP(hipEventRecord(C->e1, C->s));
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_blocks(n)), dim3(256), 0, C->s, C->q.p,
-                           C->q32.p, 1.0, n);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_blocks(V.count)), dim3(256), 0, C->s, C->q.p,
+                           C->q32.p, 1.0, V);
         SGLD_TRY(from_lattice(C, C->q.p, dy));
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
@@ -2085,7 +2133,6 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
     SGLD_ARG(C && report && reps > 0, "sgld:arg: bad arguments");
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
     SGLD_TRY(use_constants(C));
-    const int NZ = (int)C->lv[0].g.NZ();
     const int variant0 = C->kbyuVariant;
     const int czs[3] = {8, 16, 32};
     double tiled[2] = {1e300, 1e300};
@@ -2094,7 +2141,7 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
         // each precision's z-chunk on its own (persistent grid = resident blocks)
         for (int q = 0; q < 2; ++q)
             for (int cz : czs) {
-                C->ksched[q].set_chunk(NZ, cz);
+                SGLD_TRY(set_kchunk(C, q, cz));
                 double t = 0;
                 SGLD_TRY(sgld_ctx_bench_kbyu(C, q == 0 ? SGLD_PREC_FP64 : SGLD_PREC_MIXED, reps, &t));
                 if (t < tiled[q]) {
@@ -2102,7 +2149,7 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
                     bestCz[q] = cz;
                 }
             }
-        for (int q = 0; q < 2; ++q) C->ksched[q].set_chunk(NZ, bestCz[q]);
+        for (int q = 0; q < 2; ++q) SGLD_TRY(set_kchunk(C, q, bestCz[q]));
     }
     // against the node-centric dense K.u; the mixed solver spends its
     // iterations in FP32 K.u and one FP64 K.u per reliable update
@@ -2259,7 +2306,7 @@ extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, i
         for (int q = 0; q < 2; ++q) {
             const int per = blocksPerCU > 0 ? blocksPerCU : std::max(1, C->kOcc[q]);
             C->ksched[q].cap = (int)std::min<int64_t>((int64_t)C->numCU * per, kRedBlocks);
-            C->ksched[q].set_chunk((int)C->lv[0].g.NZ(), zChunk);
+            SGLD_TRY(set_kchunk(C, q, zChunk));
         }
     }
     C->kbyuVariant = kbyuVariant;

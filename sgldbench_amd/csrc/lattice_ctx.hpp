@@ -77,6 +77,15 @@ struct sgld_ctx {
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
+    // Irregular domains (full == false): the finest-level passes visit only
+    // what holds solid material.  dofChunks / nodeChunks list the 64-entry
+    // chunks of the DOF and node vectors with a solid node; kwork[q] lists the
+    // tiled K.u work items of schedule q whose output nodes touch a solid
+    // element; hPatchLayer[l * patches + patch] says whether element layer l of
+    // a patch's 16 x 16 element window holds a solid element.
+    sgld::DevBuf<int32_t> dofChunks, nodeChunks, kwork[2];
+    int64_t nDofChunks = 0, nNodeChunks = 0, nKwork[2] = {0, 0};
+    std::vector<uint8_t> hPatchLayer;
     int kOcc[2] = {1, 1};           // resident tiled K.u blocks per compute unit
     int numCU = 0;
     double stats[8] = {0};
@@ -89,6 +98,18 @@ namespace sgld {
 inline int grid_blocks(int64_t n) {
     return (int)std::min<int64_t>(kVecBlocks, std::max<int64_t>(1, (n + 255) / 256));
 }
+// Finest-level DOF / node passes: every entry of a full cuboid, the listed
+// chunks of an irregular domain.
+inline Span dof_span(const sgld_ctx *C) {
+    return C->full ? Span::all(C->nDof()) : Span{C->dofChunks.p, C->nDof(), 64 * C->nDofChunks};
+}
+inline Span node_span(const sgld_ctx *C) {
+    const int64_t n = C->lv[0].g.nodes();
+    return C->full ? Span::all(n) : Span{C->nodeChunks.p, n, 64 * C->nNodeChunks};
+}
+// Sets the z-chunk of tiled K.u schedule q (0 FP64, 1 FP32) and, on an
+// irregular domain, rebuilds its list of work items.
+int set_kchunk(sgld_ctx *C, int q, int cz);
 double now_s();
 int to_lattice(sgld_ctx *C, const double *dsrc, double *ddst);    // compressed -> lattice DOFs
 int from_lattice(sgld_ctx *C, const double *dsrc, double *ddst);  // lattice -> compressed DOFs

@@ -37,6 +37,24 @@ struct Ctl {
     int its, maxit, done, pad;
 };
 
+// Entries visited by a grid-stride pass over a vector of length n: all of
+// them when `chunks` is null, else the listed 64-entry chunks (chunk c is
+// [64c, 64c + 64) clipped to n).  Irregular domains list the chunks that hold
+// a solid node; entries outside them are void, stay zero and are skipped.
+// Position t of the pass maps to entry at(t); positions t of one wavefront
+// share t >> 6 (grid strides are multiples of 64), so the chunk id is a
+// scalar load.
+struct Span {
+    const int32_t *chunks;
+    int64_t n, count;  // count = n, or 64 x listed chunks
+    __host__ __device__ static Span all(int64_t len) { return Span{nullptr, len, len}; }
+    __device__ __forceinline__ int64_t at(int64_t t) const {
+        if (!chunks) return t;
+        const int ci = __builtin_amdgcn_readfirstlane((int)(t >> 6));
+        return ((int64_t)chunks[ci] << 6) + (t & 63);
+    }
+};
+
 constexpr int kRedBlocks = 8192;  // partial-sum slots of one reduction (max grid of a reducing kernel)
 constexpr int kVecBlocks = 2048;  // grid of the grid-stride vector kernels
 constexpr int kRedSlots = 4;      // independent reductions per solve
