@@ -256,13 +256,17 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
     const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
     const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
     for (int k = t; k < 2 * kTN * kTN * 3; k += 256) (&acc[0][0])[k] = 0.0;
-    const int k1 = t + 256;
-    const bool has1 = k1 < kTN * kTN;
-    const int oi = t % kTO, oj = t / kTO;
-    // LDS offsets (in T) of this face's corners and of this thread's output node
+    // LDS offsets (in T) of this face's corners
     const int f00 = 3 * ((ei + 1) + kTN * ej), f10 = 3 * ((ei + 1) + kTN * (ej + 1));
     const int f01 = 3 * (ei + kTN * ej), f11 = 3 * (ei + kTN * (ej + 1));
-    const int fo = 3 * ((oi + 1) + kTN * (oj + 1));
+    // A staged node layer is kTN column runs of 3 kTN contiguous values (the
+    // LDS layout 3 (row + kTN col) + a); thread t copies values j = t + 256 q,
+    // so every global access of a wavefront is contiguous.  The outputs are
+    // the interior of the staged layer, written from the same slots: slot q's
+    // lattice DOF offset sOff (from the patch's staged corner) also gives its
+    // node (sOff / 3) and component (sOff mod 3), and its LDS index is j.
+    const FastDiv d3 = FastDiv::make3();
+    int sOff[kStageSlots];
     double dotacc = 0.0;
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
         int64_t rem = wlist ? wlist[w] : w;
@@ -273,47 +277,49 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
         const int Zend = min(Z0 + tm.cz, NZ);
         const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
         const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
-        // staged-node sources (k = t and k1), -1 when outside the lattice
-        int64_t g0 = -1, g1 = -1;
-        {
-            const int r0 = R0 - 1 + t % kTN, c0 = C0 - 1 + t / kTN;
-            if (r0 >= 0 && r0 < NY && c0 >= 0 && c0 < NX) g0 = r0 + (int64_t)NY * c0;
-            if (has1) {
-                const int r1 = R0 - 1 + k1 % kTN, c1 = C0 - 1 + k1 / kTN;
-                if (r1 >= 0 && r1 < NY && c1 >= 0 && c1 < NX) g1 = r1 + (int64_t)NY * c1;
-            }
+        unsigned sMask = 0, oMask = 0;  // slots inside the lattice; slots that are outputs
+#pragma unroll
+        for (int q = 0; q < kStageSlots; ++q) {
+            const int j = t + 256 * q, col = j / (3 * kTN), off = j - 3 * kTN * col, row = off / 3;
+            const int gr = R0 - 1 + row, gcl = C0 - 1 + col;
+            const bool ok = j < 3 * kTN * kTN && gr >= 0 && gr < NY && gcl >= 0 && gcl < NX;
+            sOff[q] = ok ? 3 * NY * col + off : 0;
+            sMask |= (unsigned)ok << q;
+            oMask |= (unsigned)(ok && row >= 1 && row <= kTO && col >= 1 && col <= kTO) << q;
         }
-        const bool outNode = t < kTO * kTO && R0 + oi < NY && C0 + oj < NX;
         // Pointers advanced one layer per iteration (no index arithmetic in the loop).
-        const T *pu0 = u + 3 * (g0 < 0 ? 0 : g0 + pl * (Z0 + 2));   // layer fetched next: el+3
-        const T *pu1 = u + 3 * (g1 < 0 ? 0 : g1 + pl * (Z0 + 2));
+        const int64_t sBase = 3 * ((int64_t)(R0 - 1) + (int64_t)NY * (C0 - 1));
+        const T *pu = u + sBase + 3 * pl * (Z0 + 2);   // layer fetched next: el+3
         const T *pE = E + (einside ? er + (int64_t)gd.ny * ec + ple * Z0 : 0);  // layer el+1
-        const int64_t gOut = (R0 + oi) + (int64_t)NY * (C0 + oj);
-        const uint8_t *pF = nflag + (outNode ? gOut + pl * (Z0 - 1) : 0);      // layer el
-        T *py = y + (outNode ? 3 * (gOut + pl * (Z0 - 1)) : 0);
-        T pf[2][3];
-        auto load3 = [](const T *p, bool ok, T (&q)[3]) {
-            if (ok) { q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; }
-            else { q[0] = T(0); q[1] = T(0); q[2] = T(0); }
+        const uint8_t *pF = nflag + sBase / 3 + pl * (Z0 - 1);  // layer el
+        T *py = y + sBase + 3 * pl * (Z0 - 1);
+        T pf[kStageSlots];
+        // Unconditional loads from a valid address, the value selected after:
+        // conditional loads become branches, and at each branch join the
+        // compiler waits for every load in flight (the prefetch included).
+        // The raw values stay in flight; stage() zeroes the invalid ones when
+        // it writes the layer to LDS (a select at load time makes the compiler
+        // wait for the load at the end of the step).
+        auto loadL = [&](const T *p, bool lok, T (&q)[kStageSlots]) {
+#pragma unroll
+            for (int k = 0; k < kStageSlots; ++k) q[k] = *((lok && ((sMask >> k) & 1)) ? p + sOff[k] : u);
+        };
+        auto stage = [&](T *dst, const T (&q)[kStageSlots], bool lok) {
+#pragma unroll
+            for (int k = 0; k < kStageSlots; ++k)
+                if (k < kStageSlots - 1 || t + 256 * k < 3 * kTN * kTN)
+                    dst[t + 256 * k] = (lok && ((sMask >> k) & 1)) ? q[k] : T(0);
         };
         T *sPrev = us[0], *sCur = us[1], *sNext = us[2];  // layers el, el+1, el+2
         double *aCur = acc[0], *aNext = acc[1];
         {   // stage node layers Z0-1 and Z0; layer Z0+1 stays in flight in pf
-            T p0[2][3], p1[2][3];
-            const bool l0ok = Z0 - 1 >= 0, l2ok = Z0 + 1 < NZ;
-            load3(u + 3 * (g0 + pl * (Z0 - 1)), g0 >= 0 && l0ok, p0[0]);
-            load3(u + 3 * (g1 + pl * (Z0 - 1)), g1 >= 0 && l0ok, p0[1]);
-            load3(u + 3 * (g0 + pl * Z0), g0 >= 0, p1[0]);
-            load3(u + 3 * (g1 + pl * Z0), g1 >= 0, p1[1]);
-            load3(u + 3 * (g0 + pl * (Z0 + 1)), g0 >= 0 && l2ok, pf[0]);
-            load3(u + 3 * (g1 + pl * (Z0 + 1)), g1 >= 0 && l2ok, pf[1]);
+            T p0[kStageSlots], p1[kStageSlots];
+            loadL(u + sBase + 3 * pl * (Z0 - 1), Z0 - 1 >= 0, p0);
+            loadL(u + sBase + 3 * pl * Z0, true, p1);
+            loadL(u + sBase + 3 * pl * (Z0 + 1), Z0 + 1 < NZ, pf);
             __syncthreads();  // previous work item done with LDS
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                sPrev[3 * t + a] = p0[0][a];
-                sCur[3 * t + a] = p1[0][a];
-                if (has1) { sPrev[3 * k1 + a] = p0[1][a]; sCur[3 * k1 + a] = p1[1][a]; }
-            }
+            stage(sPrev, p0, Z0 - 1 >= 0);
+            stage(sCur, p1, true);
         }
         __syncthreads();
         T Flo[3][4], Alo[3][4];
@@ -323,6 +329,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
 #pragma unroll
             for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
         T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
+        bool eokCur = true;  // Ecur valid (else the element is outside the lattice)
         int lfetch = Z0 + 2;  // node layer of the next fetch
         // One element layer.  On entry sPrev/sCur hold node layers el and el+1,
         // `ready` holds layer el+2 (loaded during the previous step), Flo are
@@ -330,12 +337,19 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
         // forces from element layer el-1.  `spare` receives layer el+3 and
         // stays in flight through the next step; the caller alternates the
         // two buffers so no register copy waits on a load.
-        auto step = [&](int el, T (&ready)[2][3], T (&spare)[2][3]) {
-            const T Enext = (einside && el + 1 < gd.nz) ? *pE : T(0);
-            const uint8_t fcur = (outNode && el >= Z0) ? *pF : (uint8_t)0;
-            const bool lok = lfetch < NZ;
-            load3(pu0, g0 >= 0 && lok, spare[0]);
-            load3(pu1, g1 >= 0 && lok, spare[1]);
+        auto step = [&](int el, T (&ready)[kStageSlots], T (&spare)[kStageSlots]) {
+            // loads in the order they are consumed: output flags of node layer
+            // el (after the barrier), the modulus of element layer el+1 (end of
+            // the step), node layer el+3 (two steps later)
+            uint8_t flv[kStageSlots];
+#pragma unroll
+            for (int q = 0; q < kStageSlots; ++q) {
+                const bool ok = el >= Z0 && ((oMask >> q) & 1);
+                flv[q] = *(ok ? pF + d3.div((uint32_t)sOff[q]) : nflag);
+            }
+            const bool eok = einside && el + 1 < gd.nz;
+            const T Enext = *(eok ? pE : E);  // raw; zeroed where it is used
+            loadL(pu, lfetch < NZ, spare);
             T m[3][8];
             {
                 T Fhi[3][4];
@@ -349,7 +363,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                         Flo[a][q] = Fhi[a][q];
                     }
             }
-            mode_forces<T>(m, Ecur);
+            mode_forces<T>(m, eokCur ? Ecur : T(0));
             if (el >= Z0) {
                 // lower face complete: inverse (x, y) transform, scatter 4 nodes
 #pragma unroll
@@ -368,27 +382,25 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
 #pragma unroll
                 for (int q = 0; q < 4; ++q) Alo[a][q] = m[a][q] + m[a][q + 4];
             Ecur = Enext;
+            eokCur = eok;
             __syncthreads();  // node layer el accumulated
-            if (el >= Z0 && outNode) {
-                const double *a3 = aCur + fo;
-                const T *un = sPrev + fo;
+            if (el >= Z0) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const T v = ((fcur & kSolid) && !(fcur & (1 << a))) ? (T)a3[a] : T(0);
-                    py[a] = v;
-                    dotacc += (double)un[a] * (double)v;
+                for (int q = 0; q < kStageSlots; ++q) {
+                    if (!((oMask >> q) & 1)) continue;
+                    const int a = sOff[q] - 3 * (int)d3.div((uint32_t)sOff[q]);
+                    const uint32_t fl = flv[q];
+                    const int j = t + 256 * q;
+                    const T v = ((fl & kSolid) && !(fl & (1 << a))) ? (T)aCur[j] : T(0);
+                    py[sOff[q]] = v;
+                    dotacc += (double)sPrev[j] * (double)v;
                 }
-
             }
             // clear the other accumulator (read during the previous step,
             // scattered into during the next); halo nodes get discarded sums
             for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
             // layer el+2 into the slot of layer el-1 (free since the barrier above)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                sNext[3 * t + a] = ready[0][a];
-                if (has1) sNext[3 * k1 + a] = ready[1][a];
-            }
+            stage(sNext, ready, el + 2 < NZ);
             {
                 T *tmp = sPrev;
                 sPrev = sCur;
@@ -398,15 +410,14 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                 aCur = aNext;
                 aNext = ta;
             }
-            pu0 += 3 * pl;
-            pu1 += 3 * pl;
+            pu += 3 * pl;
             pE += ple;
             pF += pl;
             py += 3 * pl;
             ++lfetch;
             __syncthreads();  // layer el+2 staged, accumulator clear
         };
-        T pn[2][3];
+        T pn[kStageSlots];
         for (int el = Z0 - 1; el < Zend; el += 2) {
             step(el, pf, pn);
             if (el + 1 < Zend) step(el + 1, pn, pf);
@@ -424,43 +435,44 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
 // ----------------------------------------------------------------------------
 template <class T, int S>
 __global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid gf,
-                                                  T *__restrict__ rc, Grid gc,
+                                                  T *__restrict__ rc, Grid gc, NodeDiv dc,
                                                   const uint8_t *__restrict__ cflag,
                                                   const Ctl *ctl) {
     if (stopped(ctl)) return;
     const int64_t G = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (G >= gc.nodes()) return;
-    const int64_t NYc = gc.NY(), NXc = gc.NX();
-    const int R = (int)(G % NYc), Cc = (int)((G / NYc) % NXc), Lc = (int)(G / (NYc * NXc));
+    int R, Cc, Lc;
+    dc.rcl((uint32_t)G, R, Cc, Lc);
     const int64_t NYf = gf.NY(), NXf = gf.NX(), NZf = gf.NZ();
+    // Branch-free stencil: a fine index outside the lattice is clamped and
+    // gets weight 0, so the loads issue together instead of one by one.
     T a0 = 0, a1 = 0, a2 = 0;
-    if (cflag[G] & kSolid) {
 #pragma unroll
-        for (int dl = -(S - 1); dl <= S - 1; ++dl) {
-            const int64_t lf = (int64_t)S * Lc + dl;
-            if (lf < 0 || lf >= NZf) continue;
-            const T wz = T(S - (dl < 0 ? -dl : dl)) / T(S);
+    for (int dl = -(S - 1); dl <= S - 1; ++dl) {
+        const int64_t lu = (int64_t)S * Lc + dl;
+        const int64_t lf = lu < 0 ? 0 : (lu >= NZf ? NZf - 1 : lu);
+        const T wz = (lu == lf) ? T(S - (dl < 0 ? -dl : dl)) / T(S) : T(0);
 #pragma unroll
-            for (int dc = -(S - 1); dc <= S - 1; ++dc) {
-                const int64_t cf = (int64_t)S * Cc + dc;
-                if (cf < 0 || cf >= NXf) continue;
-                const T wx = T(S - (dc < 0 ? -dc : dc)) / T(S);
+        for (int dc = -(S - 1); dc <= S - 1; ++dc) {
+            const int64_t cu = (int64_t)S * Cc + dc;
+            const int64_t cf = cu < 0 ? 0 : (cu >= NXf ? NXf - 1 : cu);
+            const T wx = (cu == cf) ? wz * T(S - (dc < 0 ? -dc : dc)) / T(S) : T(0);
 #pragma unroll
-                for (int dr = -(S - 1); dr <= S - 1; ++dr) {
-                    const int64_t rr = (int64_t)S * R + dr;
-                    if (rr < 0 || rr >= NYf) continue;
-                    const T w = wz * wx * (T(S - (dr < 0 ? -dr : dr)) / T(S));
-                    const int64_t n = rr + NYf * cf + NYf * NXf * lf;
-                    a0 += w * rf[3 * n];
-                    a1 += w * rf[3 * n + 1];
-                    a2 += w * rf[3 * n + 2];
-                }
+            for (int dr = -(S - 1); dr <= S - 1; ++dr) {
+                const int64_t ru = (int64_t)S * R + dr;
+                const int64_t rr = ru < 0 ? 0 : (ru >= NYf ? NYf - 1 : ru);
+                const T w = (ru == rr) ? wx * T(S - (dr < 0 ? -dr : dr)) / T(S) : T(0);
+                const T *src = rf + 3 * (rr + NYf * cf + NYf * NXf * lf);
+                a0 += w * src[0];
+                a1 += w * src[1];
+                a2 += w * src[2];
             }
         }
     }
-    rc[3 * G] = a0;
-    rc[3 * G + 1] = a1;
-    rc[3 * G + 2] = a2;
+    const bool solid = cflag[G] & kSolid;
+    rc[3 * G] = solid ? a0 : T(0);
+    rc[3 * G + 1] = solid ? a1 : T(0);
+    rc[3 * G + 2] = solid ? a2 : T(0);
 }
 
 // ----------------------------------------------------------------------------
@@ -510,15 +522,19 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
             if (r >= 0 && r < NYf && c >= 0 && c < NXf) goff[i] = r + (int64_t)NYf * c;
         }
     }
+    // Loads are unconditional (a valid address for staged nodes outside the
+    // lattice) and stay in flight; stash() zeroes the invalid ones.  With
+    // conditional loads the compiler waits for the prefetch at each branch.
     T pf[kRPer][3];
+    bool pok[kRPer];
     auto fetch = [&](int lf) {
 #pragma unroll
         for (int i = 0; i < kRPer; ++i) {
-            const bool ok = goff[i] >= 0 && lf >= 0 && lf < NZf;
-            const T *src = rf + 3 * (goff[i] + plf * lf);
-            pf[i][0] = ok ? src[0] : T(0);
-            pf[i][1] = ok ? src[1] : T(0);
-            pf[i][2] = ok ? src[2] : T(0);
+            pok[i] = goff[i] >= 0 && lf >= 0 && lf < NZf;
+            const T *src = pok[i] ? rf + 3 * (goff[i] + plf * lf) : rf;
+            pf[i][0] = src[0];
+            pf[i][1] = src[1];
+            pf[i][2] = src[2];
         }
     };
     auto stash = [&](T *dst) {
@@ -526,9 +542,9 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
         for (int i = 0; i < kRPer; ++i) {
             const int k = t + 256 * i;
             if (k < kRFN) {
-                dst[3 * k] = pf[i][0];
-                dst[3 * k + 1] = pf[i][1];
-                dst[3 * k + 2] = pf[i][2];
+                dst[3 * k] = pok[i] ? pf[i][0] : T(0);
+                dst[3 * k + 1] = pok[i] ? pf[i][1] : T(0);
+                dst[3 * k + 2] = pok[i] ? pf[i][2] : T(0);
             }
         }
     };
@@ -582,6 +598,28 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
     }
 }
 
+// Component a of (P x_c)(n), span S, at any fine node n (the caller zeroes
+// void nodes).  Branch-free: per axis the two coarse neighbours are q and
+// q + (m > 0) with weights (S - m)/S and m/S, so all eight loads are in the
+// coarse lattice and issue together (with data-dependent skips the compiler
+// waits for each gather in turn).
+template <class T, int S>
+__device__ __forceinline__ T interp_dof(const T *__restrict__ xc, Grid gc, const NodeDiv &df, uint32_t n,
+                                        int a) {
+    int r, c, l;
+    df.rcl(n, r, c, l);
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+    const T *base = xc + 3 * (qr + NYc * qc + NYc * NXc * ql) + a;
+    const int64_t dr = 3 * (mr > 0), dc = 3 * NYc * (mc > 0), dl = 3 * NYc * NXc * (ml > 0);
+    const T wr1 = T(mr) / T(S), wc1 = T(mc) / T(S), wl1 = T(ml) / T(S);
+    const T wr0 = T(1) - wr1, wc0 = T(1) - wc1, wl0 = T(1) - wl1;
+    const T v000 = base[0], v100 = base[dr], v010 = base[dc], v110 = base[dc + dr];
+    const T v001 = base[dl], v101 = base[dl + dr], v011 = base[dl + dc], v111 = base[dl + dc + dr];
+    return wl0 * (wc0 * (wr0 * v000 + wr1 * v100) + wc1 * (wr0 * v010 + wr1 * v110)) +
+           wl1 * (wc0 * (wr0 * v001 + wr1 * v101) + wc1 * (wr0 * v011 + wr1 * v111));
+}
+
 // ----------------------------------------------------------------------------
 // Prolongation + both Jacobi terms of one V-cycle level (Solving_Vcycle.m:35-49):
 // x_f = P x_c + dinv_f r_f with dinv = 2 omega / diag; on the finest level the
@@ -590,54 +628,139 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
 // ----------------------------------------------------------------------------
 template <class T, int S>
 __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid gc,
-                                                 T *__restrict__ xf, Grid gf,
+                                                 T *__restrict__ xf, NodeDiv df,
                                                  const T *__restrict__ rf,
                                                  const T *__restrict__ dinv,
-                                                 const uint8_t *__restrict__ fflag, int finest, Span nodes,
+                                                 const uint8_t *__restrict__ fflag, int finest, Span dofs,
                                                  double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t NYf = gf.NY(), NXf = gf.NX();
-    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    const FastDiv d3 = FastDiv::make3();
     double dotacc = 0.0;
-    FOR_SPAN(n, nodes) {
-        const int r = (int)(n % NYf), c = (int)((n / NYf) % NXf), l = (int)(n / (NYf * NXf));
+    FOR_SPAN(i, dofs) {  // one DOF per lane: every vector access is contiguous
+        const uint32_t n = d3.div((uint32_t)i);
+        const int a = (int)((uint32_t)i - 3 * n);
+        const T pv = interp_dof<T, S>(xc, gc, df, n, a);
+        const T sv = dinv[i] * rf[i];
         const uint8_t f = fflag[n];
-        T v0 = 0, v1 = 0, v2 = 0;
-        if (f & kSolid) {
-            const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
-#pragma unroll
-            for (int bl = 0; bl < 2; ++bl) {
-                if (bl && !ml) continue;
-                const T wl = bl ? T(ml) / T(S) : T(S - ml) / T(S);
-#pragma unroll
-                for (int bc = 0; bc < 2; ++bc) {
-                    if (bc && !mc) continue;
-                    const T wc = bc ? T(mc) / T(S) : T(S - mc) / T(S);
-#pragma unroll
-                    for (int br = 0; br < 2; ++br) {
-                        if (br && !mr) continue;
-                        const T w = wl * wc * (br ? T(mr) / T(S) : T(S - mr) / T(S));
-                        const int64_t G = (qr + br) + NYc * (qc + bc) + NYc * NXc * (ql + bl);
-                        v0 += w * xc[3 * G];
-                        v1 += w * xc[3 * G + 1];
-                        v2 += w * xc[3 * G + 2];
-                    }
-                }
-            }
-        }
-        T out[3] = {v0, v1, v2};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            T val = out[a] + dinv[3 * n + a] * rf[3 * n + a];
-            if (finest && (f & (1 << a))) val = T(0);
-            xf[3 * n + a] = val;
-            if (partial) dotacc += (double)val * (double)rf[3 * n + a];
-        }
+        const T val = ((f & kSolid) && !(finest && (f & (1 << a)))) ? pv + sv : T(0);
+        xf[i] = val;
+        if (partial) dotacc += (double)val * (double)rf[i];
     }
     if (partial) {
         const double s = block_sum(dotacc);
         if (threadIdx.x == 0) partial[blockIdx.x] = s;
     }
+}
+
+// ----------------------------------------------------------------------------
+// Finest-level V-cycle output folded into the PCG direction update
+// (Solving_Vcycle.m:35-50 followed by
+// Solving_PreconditionedConjugateGradientSolver.m:28 and :15): per node
+//   z = mask(P x_2 + dinv r),  x += lambda p,  p = z + beta p   (in place)
+// with lambda of the previous iteration (its y += lambda p, deferred to here
+// so that p is read once) and beta = rho'/rho of this one.  z is never
+// stored: rho' = z.r is formed without it as x_2.r_2 + sum dinv r^2, because
+// P^T = R, r_2 = R r and r = 0 on fixed DOFs.
+// ----------------------------------------------------------------------------
+template <class T, int S>
+__global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, Grid gc, NodeDiv df,
+                                                     const T *__restrict__ rf, const T *__restrict__ dinv,
+                                                     const uint8_t *__restrict__ fflag, T *__restrict__ x,
+                                                     T *__restrict__ p, Span dofs, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const T lam = (T)ctl->lambda, beta = (T)ctl->beta;
+    const FastDiv d3 = FastDiv::make3();
+    FOR_SPAN(i, dofs) {  // one DOF per lane: every vector access is contiguous
+        const uint32_t n = d3.div((uint32_t)i);
+        const int a = (int)((uint32_t)i - 3 * n);
+        const T pv = interp_dof<T, S>(xc, gc, df, n, a);
+        const T sv = dinv[i] * rf[i];
+        const uint8_t f = fflag[n];
+        const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
+        const T po = p[i];
+        x[i] = x[i] + lam * po;
+        p[i] = z + beta * po;
+    }
+}
+
+__device__ __forceinline__ void block_sum2(double &a, double &b) {
+    __shared__ double sh[2][16];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+    }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    a = (threadIdx.x < nw) ? sh[0][threadIdx.x] : 0.0;
+    b = (threadIdx.x < nw) ? sh[1][threadIdx.x] : 0.0;
+    if (w == 0)
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 64);
+            b += __shfl_down(b, o, 64);
+        }
+}
+
+// Solving_PreconditionedConjugateGradientSolver.m:16-17: r -= lambda q with
+// partials of r.r (stopping test) and of sum dinv r^2 (the fine smoother's
+// share of the next rho).
+template <class T>
+__global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
+                                                  const T *__restrict__ dinv, Span S,
+                                                  double *__restrict__ partRR, double *__restrict__ partDR,
+                                                  const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const T lt = (T)ctl->lambda;
+    double rr = 0.0, dr = 0.0;
+    FOR_SPAN(i, S) {
+        const T rn = r[i] - lt * q[i];
+        r[i] = rn;
+        const double rd = (double)rn;
+        rr += rd * rd;
+        dr += (double)dinv[i] * rd * rd;
+    }
+    block_sum2(rr, dr);
+    if (threadIdx.x == 0) {
+        partRR[blockIdx.x] = rr;
+        partDR[blockIdx.x] = dr;
+    }
+}
+
+// partial of sum dinv r^2 (rho of a residual that k_update_r did not produce)
+template <class T>
+__global__ void __launch_bounds__(256) k_dinv_rr(const T *__restrict__ r, const T *__restrict__ dinv, Span S,
+                                                 double *__restrict__ partial) {
+    double acc = 0.0;
+    FOR_SPAN(i, S) {
+        const double rd = (double)r[i];
+        acc += (double)dinv[i] * rd * rd;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// r = b - q on free DOFs, 0 on fixed DOFs (DOF i = 3 node + a).  The
+// reference's loads never sit on fixed DOFs (FEA_ApplyBoundaryCondition.m:9-10
+// releases a loaded node's fixation), and K.u is zero there, so r = 0 on
+// fixed DOFs from then on.
+__global__ void k_residual_masked(const double *__restrict__ b, const double *__restrict__ q,
+                                  double *__restrict__ r, const uint8_t *__restrict__ nflag, Span S) {
+    FOR_SPAN(i, S) {
+        const uint32_t node = (uint32_t)(i / 3), a = (uint32_t)(i - 3 * (int64_t)node);
+        r[i] = (nflag[node] & (1u << a)) ? 0.0 : b[i] - q[i];
+    }
+}
+
+// x += lambda p with lambda of the last executed iteration (the deferred
+// update of a solve that stopped, or before a reliable update).
+template <class T>
+__global__ void k_axpy_lambda(T *__restrict__ x, const T *__restrict__ p, Span S, const Ctl *ctl) {
+    const T lam = (T)ctl->lambda;
+    FOR_SPAN(i, S) x[i] = x[i] + lam * p[i];
 }
 
 // Coarsest level: x = Minv r with the dense inverse of K_L on free DOFs
@@ -734,11 +857,13 @@ enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_PQ, FIN_RR, FIN_RHO };
 
 // Second reduction stage (fixed order, deterministic) and the scalar updates
 // of the PCG recurrence.
-__global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial, int nb, int op,
+__global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial, int nb,
+                                              const double *__restrict__ partial2, int nb2, int op,
                                               Ctl *ctl) {
     if (op != FIN_NORMB && op != FIN_RHO0 && ctl->done) return;
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
+    for (int i = threadIdx.x; i < nb2; i += blockDim.x) acc += partial2[i];
     const double s = block_sum(acc);
     if (threadIdx.x != 0) return;
     switch (op) {
@@ -1189,17 +1314,20 @@ template <class T>
 int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int nb = blocks_for(Cc.g.nodes(), 256);
-    const bool tiled = C->restrictTiled < 0 ? F.g.nodes() >= (int64_t)1 << 16 : C->restrictTiled != 0;
+    // by size: the tiled kernel marches 2 czc + 1 fine layers per block, which
+    // only pays off when there are enough coarse nodes to fill the device
+    const bool tiled = C->restrictTiled < 0 ? Cc.g.nodes() >= C->restrictTiledMin : C->restrictTiled != 0;
+    const NodeDiv dc = NodeDiv::make(Cc.g);
     if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
         const int czc = 4, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
         hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(tilesR * tilesC * tilesZ), dim3(256), 0, C->s, rf,
                            F.g, rc, Cc.g, Cc.nflag.p, czc, tilesR, tilesC, ctl);
     } else if (Cc.span == 2)
-        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g,
+        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
                            Cc.nflag.p, ctl);
     else
-        hipLaunchKernelGGL((k_restrict<T, 4>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g,
+        hipLaunchKernelGGL((k_restrict<T, 4>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
                            Cc.nflag.p, ctl);
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
@@ -1210,13 +1338,14 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
                     const Ctl *ctl, int *nbOut) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int finest = (k == 1);
-    const Span S = finest ? node_span(C) : Span::all(F.g.nodes());
+    const Span S = finest ? dof_span(C) : Span::all(3 * F.g.nodes());
     const int nb = grid_blocks(S.count);
+    const NodeDiv df = NodeDiv::make(F.g);
     if (Cc.span == 2)
-        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, df, rf,
                            lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
     else
-        hipLaunchKernelGGL((k_prolong<T, 4>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, F.g, rf,
+        hipLaunchKernelGGL((k_prolong<T, 4>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, df, rf,
                            lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
     SGLD_HIP(hipGetLastError());
     if (nbOut) *nbOut = nb;
@@ -1244,30 +1373,88 @@ int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ct
     return SGLD_OK;
 }
 
-int fin(sgld_ctx *C, const double *partial, int nb, int op) {
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, C->s, partial, nb, op, C->ctl.p);
+int fin(sgld_ctx *C, const double *partial, int nb, int op, const double *partial2 = nullptr, int nb2 = 0) {
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, C->s, partial, nb, partial2, nb2, op, C->ctl.p);
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
 }
 
-// One PCG iteration (Solving_PreconditionedConjugateGradientSolver.m:14-29),
-// rotated so that the direction update p = z + beta p of the previous
-// iteration is fused into this iteration's K.u: reads z and the old p
-// (pold), writes the new p (pnew) and q = K pnew.
+// The V-cycle below the finest level for residual r (Solving_Vcycle.m:12-49
+// for levels >= 2): restrictions down to the coarsest, its dense solve, and
+// prolongation + smoothing back up to level 2, whose x_2 stays in lv[1].
+// partialX2 receives the partials of x_2 . r_2 (count in *nbX2).
 template <class T>
-int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *z, const T *pold, T *pnew, T *q) {
+int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl) {
+    const int L = C->L;
+    const T *rf = r;
+    for (int k = 1; k < L; ++k) {
+        SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
+        rf = lvl_r<T>(C->lv[k]);
+    }
+    Level &Lc = C->lv[L - 1];
+    const int64_t n = C->nCoarseDof;
+    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->Minv.p, n,
+                       lvl_r<T>(Lc), lvl_x<T>(Lc), ctl);
+    SGLD_HIP(hipGetLastError());
+    for (int k = L - 1; k >= 2; --k)
+        SGLD_TRY(enqueue_prolong<T>(C, k, lvl_x<T>(C->lv[k]), lvl_x<T>(C->lv[k - 1]), lvl_r<T>(C->lv[k - 1]),
+                                    k == 2 ? partialX2 : nullptr, ctl, k == 2 ? nbX2 : nullptr));
+    if (L == 2) {  // level 2 is the coarsest: x_2 . r_2 by its own pass
+        const Span S2 = Span::all(3 * C->lv[1].g.nodes());
+        *nbX2 = grid_blocks(S2.count);
+        hipLaunchKernelGGL(k_dot<T>, dim3(*nbX2), dim3(256), 0, C->s, (const T *)lvl_x<T>(Lc),
+                           (const T *)lvl_r<T>(Lc), S2, partialX2, ctl);
+        SGLD_HIP(hipGetLastError());
+    }
+    return SGLD_OK;
+}
+
+// rho = z.r for a residual r set outside the iteration (start of a solve,
+// reliable update), z = M r: sum dinv r^2 on the finest level plus x_2 . r_2
+// from the coarse V-cycle; op FIN_RHO0 (restart, beta = 0) or FIN_RHO.
+template <class T>
+int enqueue_rho(sgld_ctx *C, const T *r, int op) {
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
+    hipLaunchKernelGGL(k_dinv_rr<T>, dim3(nbv), dim3(256), 0, C->s, r, lvl_dinv<T>(C->lv[0]), V, slot(C, 3));
+    SGLD_HIP(hipGetLastError());
+    int nb2 = 0;
+    SGLD_TRY(enqueue_vcycle_coarse<T>(C, r, slot(C, 2), &nb2, C->ctl.p));
+    return fin(C, slot(C, 2), nb2, op, slot(C, 3), nbv);
+}
+
+// One PCG iteration (Solving_PreconditionedConjugateGradientSolver.m:14-29,
+// with z = Solving_Vcycle(r)), rotated: it starts with the finest level of the
+// previous iteration's V-cycle, fused with that iteration's deferred
+// y += lambda p and this one's p = z + beta p (k_prolong_dir), then
+// q = K p, lambda = rho / p.q, r -= lambda q with the stopping test on the
+// recursive residual, and the coarse V-cycle of the new r with rho' and beta.
+// Finest-level traffic: 6 vector passes + 1/8 for the direction update, K.u,
+// 4 for the residual update, 1 + 1/8 for the first restriction.
+template <class T>
+int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     const Span S = dof_span(C);
     Ctl *ctl = C->ctl.p;
-    int nb = 0;
-    SGLD_TRY(enqueue_kbyu<T>(C, pold, q, slot(C, 0), ctl, &nb, z, pnew));
-    SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
+    Level &l0 = C->lv[0], &l1 = C->lv[1];
+    const NodeDiv d0 = NodeDiv::make(l0.g);
     const int nbv = grid_blocks(S.count);
-    hipLaunchKernelGGL(k_update_xr<T>, dim3(nbv), dim3(256), 0, C->s, x, r, (const T *)pnew, (const T *)q, S,
-                       slot(C, 1), ctl);
+    if (l1.span == 2)
+        hipLaunchKernelGGL((k_prolong_dir<T, 2>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
+                           (const T *)r, lvl_dinv<T>(l0), l0.nflag.p, x, p, S, (const Ctl *)ctl);
+    else
+        hipLaunchKernelGGL((k_prolong_dir<T, 4>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
+                           (const T *)r, lvl_dinv<T>(l0), l0.nflag.p, x, p, S, (const Ctl *)ctl);
+    SGLD_HIP(hipGetLastError());
+    int nb = 0;
+    SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
+    SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
+    hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, lvl_dinv<T>(l0), S,
+                       slot(C, 1), slot(C, 3), (const Ctl *)ctl);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
-    SGLD_TRY(enqueue_vcycle<T>(C, r, z, slot(C, 2), ctl, &nb));
-    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO));
+    int nb2 = 0;
+    SGLD_TRY(enqueue_vcycle_coarse<T>(C, r, slot(C, 2), &nb2, ctl));
+    SGLD_TRY(fin(C, slot(C, 2), nb2, FIN_RHO, slot(C, 3), nbv));
     return SGLD_OK;
 }
 
@@ -1304,11 +1491,10 @@ int fp64_begin(sgld_ctx *C, SolveState &S) {
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 3), nbv, FIN_NORMB));
     SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p, V);
+    hipLaunchKernelGGL(k_residual_masked, dim3(nbv), dim3(256), 0, C->s, C->b.p, C->q.p, C->r.p,
+                       C->lv[0].nflag.p, V);
     SGLD_HIP(hipGetLastError());
-    int nb = 0;
-    SGLD_TRY(enqueue_vcycle<double>(C, C->r.p, C->z.p, slot(C, 2), nullptr, &nb));
-    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));  // rho = z.r, beta = 0: first p = z
+    SGLD_TRY(enqueue_rho<double>(C, C->r.p, FIN_RHO0));  // rho = z.r, beta = 0: first p = z
     hipLaunchKernelGGL(k_fill<double>, dim3(nbv), dim3(256), 0, C->s, C->p.p, 0.0, V);  // beta 0 x p
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
     SGLD_HIP(hipStreamSynchronize(C->s));
@@ -1350,9 +1536,7 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     C->hctl->maxit = S.maxit;
     C->hctl->normb = 1.0;
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    int nb = 0;
-    SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), nullptr, &nb));
-    SGLD_TRY(fin(C, slot(C, 2), nb, FIN_RHO0));
+    SGLD_TRY(enqueue_rho<float>(C, C->r32.p, FIN_RHO0));
     hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, V);  // beta 0 x p
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
@@ -1363,17 +1547,15 @@ int pcg_begin(sgld_ctx *C, SolveState &S) {
     return S.precision == SGLD_PREC_FP64 ? fp64_begin(C, S) : mixed_begin(C, S);
 }
 
-// `batch` (even) iterations alternating the two direction buffers, starting
-// from the first (the latest direction is kept there between batches).
+// `batch` iterations.  The last iteration's y += lambda p stays pending (it is
+// folded into the next iteration's first pass); pcg_after_batch applies it
+// when the solve stops or before a reliable update.
 int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
-    for (int i = 0; i < S.batch; i += 2) {
-        if (S.precision == SGLD_PREC_FP64) {
-            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->z.p, C->p.p, C->p2.p, C->q.p));
-            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->z.p, C->p2.p, C->p.p, C->q.p));
-        } else {
-            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->z32.p, C->p32.p, C->p32b.p, C->q32.p));
-            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->z32.p, C->p32b.p, C->p32.p, C->q32.p));
-        }
+    for (int i = 0; i < S.batch; ++i) {
+        if (S.precision == SGLD_PREC_FP64)
+            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->p.p, C->q.p));
+        else
+            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->p32.p, C->q32.p));
     }
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
     return SGLD_OK;
@@ -1382,8 +1564,13 @@ int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
 // Host step after a completed batch (the control block is in C->hctl).
 int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     const Ctl st = *C->hctl;
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     if (S.precision == SGLD_PREC_FP64) {
-        if (st.done) {
+        if (st.done) {  // apply the stopped iteration's y += lambda p
+            hipLaunchKernelGGL(k_axpy_lambda<double>, dim3(nbv), dim3(256), 0, C->s, C->x.p, (const double *)C->p.p,
+                               V, (const Ctl *)C->ctl.p);
+            SGLD_HIP(hipGetLastError());
             S.finished = true;
             S.its = st.its;
             S.res = st.res;
@@ -1395,15 +1582,11 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     // when its recursive residual has dropped by delta (or after kMaxStretch
     // iterations), fold the FP32 correction into the FP64 solution and
     // replace the residual by the FP64 true residual (reliable update).
-    const Span V = dof_span(C);
-    const int nbv = grid_blocks(V.count);
-    // the batch alternates p32 -> p32b -> p32 ...: after an odd number of its
-    // iterations the latest direction is in p32b; the next batch starts from p32
-    if ((st.its - S.its) % 2)
-        hipLaunchKernelGGL(k_copy<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (const float *)C->p32b.p, V);
     S.its = st.its;
     if (!st.done && S.its - S.itsAtUpdate < SolveState::kMaxStretch) return SGLD_OK;
     S.itsAtUpdate = S.its;
+    hipLaunchKernelGGL(k_axpy_lambda<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, (const float *)C->p32.p, V,
+                       (const Ctl *)C->ctl.p);  // the pending x += lambda p
     hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, S.scale, V);
     hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
     SGLD_HIP(hipGetLastError());
@@ -1425,14 +1608,13 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     *C->hctl = st;
     C->hctl->done = 0;
     C->hctl->rho = st.rho * f * f;
+    C->hctl->lambda = 0.0;  // applied above
     C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    // z = M r and beta for the replaced residual, then continue from p.  After a
-    // forced update (FP32 stalled mid-stretch) rho already belongs to the
+    // rho = z.r and beta for the replaced residual, then continue from p.  After
+    // a forced update (FP32 stalled mid-stretch) rho already belongs to the
     // recursive residual, so the direction restarts (beta = 0).
-    int nb = 0;
-    SGLD_TRY(enqueue_vcycle<float>(C, C->r32.p, C->z32.p, slot(C, 2), C->ctl.p, &nb));
-    SGLD_TRY(fin(C, slot(C, 2), nb, st.done ? FIN_RHO : FIN_RHO0));
+    SGLD_TRY(enqueue_rho<float>(C, C->r32.p, st.done ? FIN_RHO : FIN_RHO0));
     return SGLD_OK;
 }
 
@@ -1496,7 +1678,7 @@ int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double 
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
     SGLD_TRY(enqueue_kbyu<double>(C, x, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual<double>, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, V);
+    hipLaunchKernelGGL(k_residual_masked, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, C->lv[0].nflag.p, V);
     SGLD_HIP(hipGetLastError());
     std::vector<double> part(2 * kRedBlocks);
     hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, r, r, V, slot(C, 0), nullptr);
@@ -1592,6 +1774,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     // Solving_BuildingMeshHierarchy.m:12-16
     if (nonDyadic == 1 && numLevels >= 4) numLevels -= 1; else nonDyadic = 0;
     C->L = numLevels;
+    if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     C->nonDyadic = nonDyadic;
     C->nu = nu;
     C->h = h;

@@ -14,6 +14,7 @@ constexpr int kTE = 16;        // element columns per K.u patch edge
 constexpr int kTN = kTE + 1;   // staged nodes per patch edge
 constexpr int kTO = kTE - 1;   // output nodes per patch edge
 constexpr int kXCD = 8;        // accelerator complex dies (one L2 each)
+constexpr int kStageSlots = (3 * kTN * kTN + 255) / 256;  // staged values per thread and node layer
 
 // Work decomposition of the patch-tiled K.u: tilesR x tilesC patches of
 // kTO x kTO output node columns, each cut into z-chunks of cz node layers.
@@ -76,6 +77,7 @@ struct sgld_ctx {
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
+    int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
     // what holds solid material.  dofChunks / nodeChunks list the 64-entry

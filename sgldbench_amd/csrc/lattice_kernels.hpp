@@ -30,6 +30,40 @@ struct Grid {
     __host__ __device__ int64_t elems() const { return (int64_t)nx * ny * nz; }
 };
 
+// Division by a fixed d for 0 <= n < 2^31 without a divide instruction:
+// q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1 (Granlund & Montgomery 1994, unsigned
+// round-up variant; the sum cannot wrap because umulhi(n, m) < n < 2^31).
+struct FastDiv {
+    uint32_t d = 1, m = 1, s = 0;
+    static FastDiv make(uint32_t dv) {
+        FastDiv f;
+        f.d = dv;
+        f.s = 0;
+        while ((1ull << f.s) < dv) ++f.s;
+        f.m = (uint32_t)((((1ull << 32) * ((1ull << f.s) - dv)) / dv) + 1);
+        return f;
+    }
+    __host__ __device__ static constexpr FastDiv make3() { return FastDiv{3u, 1431655766u, 2u}; }
+    // (64-bit product >> 32 of zero-extended operands compiles to v_mul_hi_u32)
+    __host__ __device__ uint32_t div(uint32_t n) const {
+        return ((uint32_t)(((uint64_t)n * m) >> 32) + n) >> s;
+    }
+};
+
+// (row, col, layer) of lattice node n < 2^31 of a grid with NY x NX nodes per layer.
+struct NodeDiv {
+    FastDiv NY, NX;
+    static NodeDiv make(const Grid &g) { return NodeDiv{FastDiv::make(g.ny + 1), FastDiv::make(g.nx + 1)}; }
+    __host__ __device__ __forceinline__ void rcl(uint32_t n, int &r, int &c, int &l) const {
+        const uint32_t q = NY.div(n);
+        r = (int)(n - q * NY.d);
+        const uint32_t lq = NX.div(q);
+        c = (int)(q - lq * NX.d);
+        l = (int)lq;
+    }
+};
+
 // Device-side PCG state (one per solve); all solver kernels read it so that a
 // batch of iterations can be queued (or graph-captured) without host syncs.
 struct Ctl {
