@@ -135,8 +135,10 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
     // For the isotropic cube, axis permutations leave K' invariant, so nine
     // numbers define it (km9: own-axis linear, cross-axis linear, bilinear
     // with own axis, bilinear without, trilinear, then the four couplings).
-    const T dLo = km9<T>(0), dLx = km9<T>(1), dBo = km9<T>(2), dBx = km9<T>(3), dT = km9<T>(4);
-    const T c0 = km9<T>(5), c1 = km9<T>(6), c2 = km9<T>(7), c3 = km9<T>(8);
+    // E scales the nine coefficients (9 multiplies instead of 21 on the modes)
+    const T dLo = Ee * km9<T>(0), dLx = Ee * km9<T>(1), dBo = Ee * km9<T>(2), dBx = Ee * km9<T>(3),
+            dT = Ee * km9<T>(4);
+    const T c0 = Ee * km9<T>(5), c1 = Ee * km9<T>(6), c2 = Ee * km9<T>(7), c3 = Ee * km9<T>(8);
     {
         const T x = uh[0][1], yv = uh[1][2], z = uh[2][4];
         uh[0][1] = dLo * x + c0 * (yv + z);
@@ -182,12 +184,8 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         uh[a][7] *= dT;
-        uh[a][0] = T(0);
+        uh[a][0] = T(0);  // translation carries no force (callers skip it)
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int m = 1; m < 8; ++m) uh[a][m] *= Ee;
 }
 
 // ----------------------------------------------------------------------------
@@ -214,6 +212,28 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
 // nodes touch a solid element; the others would only write zeros over void
 // nodes that are zero already.
 // ----------------------------------------------------------------------------
+
+// Raw buffer access (CDNA buffer instructions): a load whose byte offset is
+// at or past the descriptor's range returns 0 and a store there is dropped,
+// so slots outside the lattice need neither a select nor a branch.
+constexpr uint32_t kOob = 0x7FFFFFF0u;  // offset past every range used here
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <class T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                          r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+}
 
 template <class T>
 __device__ __forceinline__ void face_modes(const T *__restrict__ lay, int ei, int ej, T (&F)[3][4]) {
@@ -262,11 +282,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
     // A staged node layer is kTN column runs of 3 kTN contiguous values (the
     // LDS layout 3 (row + kTN col) + a); thread t copies values j = t + 256 q,
     // so every global access of a wavefront is contiguous.  The outputs are
-    // the interior of the staged layer, written from the same slots: slot q's
-    // lattice DOF offset sOff (from the patch's staged corner) also gives its
-    // node (sOff / 3) and component (sOff mod 3), and its LDS index is j.
+    // the interior of the staged layer, written from the same slots.  Per
+    // slot: lOff, the byte offset of its value from the staged corner of a
+    // node layer (kOob outside the lattice), and oOff, the same for outputs
+    // (kOob unless an output node); node flags sit at oOff / (3 sizeof(T)).
     const FastDiv d3 = FastDiv::make3();
-    int sOff[kStageSlots];
+    uint32_t lOff[kStageSlots], oOff[kStageSlots];
     double dotacc = 0.0;
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
         int64_t rem = wlist ? wlist[w] : w;
@@ -277,49 +298,42 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
         const int Zend = min(Z0 + tm.cz, NZ);
         const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
         const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
-        unsigned sMask = 0, oMask = 0;  // slots inside the lattice; slots that are outputs
 #pragma unroll
         for (int q = 0; q < kStageSlots; ++q) {
             const int j = t + 256 * q, col = j / (3 * kTN), off = j - 3 * kTN * col, row = off / 3;
             const int gr = R0 - 1 + row, gcl = C0 - 1 + col;
             const bool ok = j < 3 * kTN * kTN && gr >= 0 && gr < NY && gcl >= 0 && gcl < NX;
-            sOff[q] = ok ? 3 * NY * col + off : 0;
-            sMask |= (unsigned)ok << q;
-            oMask |= (unsigned)(ok && row >= 1 && row <= kTO && col >= 1 && col <= kTO) << q;
+            const uint32_t byteOff = (uint32_t)sizeof(T) * (uint32_t)(3 * NY * col + off);
+            lOff[q] = ok ? byteOff : kOob;
+            oOff[q] = (ok && row >= 1 && row <= kTO && col >= 1 && col <= kTO) ? byteOff : kOob;
         }
-        // Pointers advanced one layer per iteration (no index arithmetic in the loop).
+        // per node layer: the staged corner (R0 - 1, C0 - 1) and the bytes
+        // spanned by the staged columns
         const int64_t sBase = 3 * ((int64_t)(R0 - 1) + (int64_t)NY * (C0 - 1));
-        const T *pu = u + sBase + 3 * pl * (Z0 + 2);   // layer fetched next: el+3
+        const uint32_t vBytes = (uint32_t)sizeof(T) * (uint32_t)(3 * NY * (kTN - 1) + 3 * kTN);
+        const uint32_t fBytes = (uint32_t)(NY * (kTN - 1) + kTN);
+        auto uRsrc = [&](int L) { return buf_rsrc(u + sBase + 3 * pl * L, (L >= 0 && L < NZ) ? vBytes : 0u); };
         const T *pE = E + (einside ? er + (int64_t)gd.ny * ec + ple * Z0 : 0);  // layer el+1
-        const uint8_t *pF = nflag + sBase / 3 + pl * (Z0 - 1);  // layer el
-        T *py = y + sBase + 3 * pl * (Z0 - 1);
         T pf[kStageSlots];
-        // Unconditional loads from a valid address, the value selected after:
-        // conditional loads become branches, and at each branch join the
-        // compiler waits for every load in flight (the prefetch included).
-        // The raw values stay in flight; stage() zeroes the invalid ones when
-        // it writes the layer to LDS (a select at load time makes the compiler
-        // wait for the load at the end of the step).
-        auto loadL = [&](const T *p, bool lok, T (&q)[kStageSlots]) {
+        auto loadL = [&](__amdgpu_buffer_rsrc_t r, T (&q)[kStageSlots]) {
 #pragma unroll
-            for (int k = 0; k < kStageSlots; ++k) q[k] = *((lok && ((sMask >> k) & 1)) ? p + sOff[k] : u);
+            for (int k = 0; k < kStageSlots; ++k) q[k] = buf_load<T>(r, lOff[k]);
         };
-        auto stage = [&](T *dst, const T (&q)[kStageSlots], bool lok) {
+        auto stage = [&](T *dst, const T (&q)[kStageSlots]) {
 #pragma unroll
             for (int k = 0; k < kStageSlots; ++k)
-                if (k < kStageSlots - 1 || t + 256 * k < 3 * kTN * kTN)
-                    dst[t + 256 * k] = (lok && ((sMask >> k) & 1)) ? q[k] : T(0);
+                if (k < kStageSlots - 1 || t + 256 * k < 3 * kTN * kTN) dst[t + 256 * k] = q[k];
         };
         T *sPrev = us[0], *sCur = us[1], *sNext = us[2];  // layers el, el+1, el+2
         double *aCur = acc[0], *aNext = acc[1];
         {   // stage node layers Z0-1 and Z0; layer Z0+1 stays in flight in pf
             T p0[kStageSlots], p1[kStageSlots];
-            loadL(u + sBase + 3 * pl * (Z0 - 1), Z0 - 1 >= 0, p0);
-            loadL(u + sBase + 3 * pl * Z0, true, p1);
-            loadL(u + sBase + 3 * pl * (Z0 + 1), Z0 + 1 < NZ, pf);
+            loadL(uRsrc(Z0 - 1), p0);
+            loadL(uRsrc(Z0), p1);
+            loadL(uRsrc(Z0 + 1), pf);
             __syncthreads();  // previous work item done with LDS
-            stage(sPrev, p0, Z0 - 1 >= 0);
-            stage(sCur, p1, true);
+            stage(sPrev, p0);
+            stage(sCur, p1);
         }
         __syncthreads();
         T Flo[3][4], Alo[3][4];
@@ -330,7 +344,6 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
         T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
         bool eokCur = true;  // Ecur valid (else the element is outside the lattice)
-        int lfetch = Z0 + 2;  // node layer of the next fetch
         // One element layer.  On entry sPrev/sCur hold node layers el and el+1,
         // `ready` holds layer el+2 (loaded during the previous step), Flo are
         // the face modes of layer el and Alo the part of layer el's face
@@ -341,15 +354,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             // loads in the order they are consumed: output flags of node layer
             // el (after the barrier), the modulus of element layer el+1 (end of
             // the step), node layer el+3 (two steps later)
-            uint8_t flv[kStageSlots];
+            const bool out = el >= Z0;
+            const __amdgpu_buffer_rsrc_t fr = buf_rsrc(nflag + sBase / 3 + pl * el, out ? fBytes : 0u);
+            uint32_t flv[kStageSlots];
 #pragma unroll
-            for (int q = 0; q < kStageSlots; ++q) {
-                const bool ok = el >= Z0 && ((oMask >> q) & 1);
-                flv[q] = *(ok ? pF + d3.div((uint32_t)sOff[q]) : nflag);
-            }
+            for (int q = 0; q < kStageSlots; ++q)
+                flv[q] = __builtin_amdgcn_raw_buffer_load_b8(fr, (int)d3.div(oOff[q] / (uint32_t)sizeof(T)), 0, 0);
             const bool eok = einside && el + 1 < gd.nz;
             const T Enext = *(eok ? pE : E);  // raw; zeroed where it is used
-            loadL(pu, lfetch < NZ, spare);
+            loadL(uRsrc(el + 3), spare);
             T m[3][8];
             {
                 T Fhi[3][4];
@@ -366,9 +379,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             mode_forces<T>(m, eokCur ? Ecur : T(0));
             if (el >= Z0) {
                 // lower face complete: inverse (x, y) transform, scatter 4 nodes
+                // (force mode 0 is zero: written out, since x + 0 is not folded)
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const T A0 = Alo[a][0] + (m[a][0] - m[a][4]), A1 = Alo[a][1] + (m[a][1] - m[a][5]);
+                    const T A0 = Alo[a][0] - m[a][4], A1 = Alo[a][1] + (m[a][1] - m[a][5]);
                     const T A2 = Alo[a][2] + (m[a][2] - m[a][6]), A3 = Alo[a][3] + (m[a][3] - m[a][7]);
                     const T t0 = A0 - A2, t1 = A1 - A3, t2 = A0 + A2, t3 = A1 + A3;
                     atomicAdd(aCur + f00 + a, (double)(t0 - t1));
@@ -378,21 +392,24 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                 }
             }
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a) {
+                Alo[a][0] = m[a][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) Alo[a][q] = m[a][q] + m[a][q + 4];
+                for (int q = 1; q < 4; ++q) Alo[a][q] = m[a][q] + m[a][q + 4];
+            }
             Ecur = Enext;
             eokCur = eok;
             __syncthreads();  // node layer el accumulated
-            if (el >= Z0) {
+            if (out) {
+                const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * el, vBytes);
 #pragma unroll
                 for (int q = 0; q < kStageSlots; ++q) {
-                    if (!((oMask >> q) & 1)) continue;
-                    const int a = sOff[q] - 3 * (int)d3.div((uint32_t)sOff[q]);
-                    const uint32_t fl = flv[q];
+                    if (q == kStageSlots - 1 && t + 256 * q >= 3 * kTN * kTN) continue;
+                    const uint32_t vi = oOff[q] / (uint32_t)sizeof(T);
+                    const uint32_t a = vi - 3 * d3.div(vi);  // component (any value for kOob: flags are 0 there)
                     const int j = t + 256 * q;
-                    const T v = ((fl & kSolid) && !(fl & (1 << a))) ? (T)aCur[j] : T(0);
-                    py[sOff[q]] = v;
+                    const T v = ((flv[q] & kSolid) && !(flv[q] & (1u << a))) ? (T)aCur[j] : T(0);
+                    buf_store(v, yr, oOff[q]);
                     dotacc += (double)sPrev[j] * (double)v;
                 }
             }
@@ -400,7 +417,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             // scattered into during the next); halo nodes get discarded sums
             for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
             // layer el+2 into the slot of layer el-1 (free since the barrier above)
-            stage(sNext, ready, el + 2 < NZ);
+            stage(sNext, ready);
             {
                 T *tmp = sPrev;
                 sPrev = sCur;
@@ -410,11 +427,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                 aCur = aNext;
                 aNext = ta;
             }
-            pu += 3 * pl;
             pE += ple;
-            pF += pl;
-            py += 3 * pl;
-            ++lfetch;
             __syncthreads();  // layer el+2 staged, accumulator clear
         };
         T pn[kStageSlots];
