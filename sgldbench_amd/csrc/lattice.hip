@@ -776,6 +776,156 @@ __global__ void k_axpy_lambda(T *__restrict__ x, const T *__restrict__ p, Span S
     FOR_SPAN(i, S) x[i] = x[i] + lam * p[i];
 }
 
+// ----------------------------------------------------------------------------
+// Small-level tail of the V-cycle in two launches.  Below level t0 the
+// restrictions and prolongations are latency-bound (a few microseconds per
+// launch whatever the size), so they are collapsed: linear interpolation on
+// nested grids composes exactly, P(t0 <- t0+1) ... P(t0+m-1 <- t0+m) is the
+// trilinear interpolation of span 2^m, and R = P^T.  Hence
+//   r_{t0+m} = R_{2^m} r_{t0}                                    (m = 1..M)
+//   x_{t0}   = D_{t0} r_{t0} + sum_{m<M} P_{2^m} D_{t0+m} r_{t0+m} + P_{2^M} x_{t0+M}
+// which equals Solving_Vcycle.m:12-26 and :35-49 on these levels (D = 2 omega
+// / diag; void coarse nodes: a solid node's stencil touches only corners of
+// non-empty coarse elements, and void entries are zero).
+// ----------------------------------------------------------------------------
+constexpr int kTailMax = 3;  // fused levels below t0: spans 2, 4, 8
+
+template <class T>
+struct TailArgs {
+    int M;                          // fused levels t0+1 .. t0+M (t0+M is the coarsest)
+    Grid g0;                        // level t0
+    NodeDiv d0;
+    const T *r0, *dinv0;
+    const uint8_t *flag0;
+    T *x0;
+    Grid g[kTailMax];
+    NodeDiv dv[kTailMax];
+    T *r[kTailMax];
+    const T *dinv[kTailMax];
+    const uint8_t *flag[kTailMax];
+    const T *xc;                    // x of the coarsest level
+    int blk[kTailMax + 1];          // restriction blocks: level t0+1+m uses [blk[m], blk[m+1])
+};
+
+// lanes per target node of level t0+1+m: one thread (27 taps), one wavefront
+// (343 taps), one block (3375 taps)
+__host__ __device__ constexpr int tail_lanes(int m) { return m == 0 ? 1 : (m == 1 ? 64 : 256); }
+
+// r of one node of level t0+1+m (span S = 2^(m+1) from level t0), gathered by
+// P lanes; spans and lane counts are compile-time so the tap decode is cheap.
+template <class T, int S, int P>
+__device__ __forceinline__ void tail_restrict_node(const TailArgs<T> &A, int m, double (*sh)[4]) {
+    constexpr int Wd = 2 * S - 1, taps = Wd * Wd * Wd;
+    const int lane = threadIdx.x % P;
+    const int64_t G = (int64_t)(blockIdx.x - A.blk[m]) * (256 / P) + threadIdx.x / P;
+    const Grid gc = A.g[m];
+    const bool live = G < gc.nodes();
+    int R = 0, Cc = 0, Lc = 0;
+    if (live) A.dv[m].rcl((uint32_t)G, R, Cc, Lc);
+    const int64_t NYf = A.g0.NY(), NXf = A.g0.NX(), NZf = A.g0.NZ();
+    T a0 = 0, a1 = 0, a2 = 0;
+    if (live) {
+        // branch-free taps: indices clamped into the lattice, weight 0 outside
+#pragma unroll
+        for (int k0 = 0; k0 < taps; k0 += P) {
+            const int k = k0 + lane;
+            const int kk = k < taps ? k : taps - 1;
+            const int dr = kk % Wd - (S - 1), dc = (kk / Wd) % Wd - (S - 1), dl = kk / (Wd * Wd) - (S - 1);
+            const int64_t ru = (int64_t)S * R + dr, cu = (int64_t)S * Cc + dc, lu = (int64_t)S * Lc + dl;
+            const int64_t rr = ru < 0 ? 0 : (ru >= NYf ? NYf - 1 : ru);
+            const int64_t cf = cu < 0 ? 0 : (cu >= NXf ? NXf - 1 : cu);
+            const int64_t lf = lu < 0 ? 0 : (lu >= NZf ? NZf - 1 : lu);
+            const bool in = k < taps && ru == rr && cu == cf && lu == lf;
+            const T w = in ? T((S - (dr < 0 ? -dr : dr)) * (S - (dc < 0 ? -dc : dc)) * (S - (dl < 0 ? -dl : dl))) /
+                                 T(S * S * S)
+                           : T(0);
+            const T *src = A.r0 + 3 * (rr + NYf * cf + NYf * NXf * lf);
+            a0 += w * src[0];
+            a1 += w * src[1];
+            a2 += w * src[2];
+        }
+    }
+    double s0 = a0, s1 = a1, s2 = a2;
+    if (P >= 64) {
+        for (int o = 32; o > 0; o >>= 1) {
+            s0 += __shfl_down(s0, o, 64);
+            s1 += __shfl_down(s1, o, 64);
+            s2 += __shfl_down(s2, o, 64);
+        }
+    }
+    if (P == 256) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            sh[0][w] = s0;
+            sh[1][w] = s1;
+            sh[2][w] = s2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s0 = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+            s1 = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+            s2 = sh[2][0] + sh[2][1] + sh[2][2] + sh[2][3];
+        }
+    }
+    if (live && lane == 0) {
+        const bool solid = A.flag[m][G] & kSolid;
+        A.r[m][3 * G] = solid ? (T)s0 : T(0);
+        A.r[m][3 * G + 1] = solid ? (T)s1 : T(0);
+        A.r[m][3 * G + 2] = solid ? (T)s2 : T(0);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_tail_restrict(TailArgs<T> A, const Ctl *ctl) {
+    __shared__ double sh[3][4];
+    if (stopped(ctl)) return;
+    int m = 0;
+    while (m + 1 < A.M && (int)blockIdx.x >= A.blk[m + 1]) ++m;  // uniform per block
+    if (m == 0) tail_restrict_node<T, 2, tail_lanes(0)>(A, 0, sh);
+    else if (m == 1) tail_restrict_node<T, 4, tail_lanes(1)>(A, 1, sh);
+    else tail_restrict_node<T, 8, tail_lanes(2)>(A, 2, sh);
+}
+
+// component a of sum_corners w(n) v[G] for the span-S interpolation from
+// level grid gc at level-t0 node (r, c, l); `dv` null: plain v, else dv * v.
+template <class T, int S>
+__device__ __forceinline__ T tail_interp(const T *__restrict__ v, const T *__restrict__ dv, Grid gc, int r,
+                                         int c, int l, int a) {
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+    const int64_t base = 3 * (qr + NYc * qc + NYc * NXc * ql) + a;
+    const int64_t dr = 3 * (mr > 0), dc = 3 * NYc * (mc > 0), dl = 3 * NYc * NXc * (ml > 0);
+    const T wr1 = T(mr) / T(S), wc1 = T(mc) / T(S), wl1 = T(ml) / T(S);
+    const T wr0 = T(1) - wr1, wc0 = T(1) - wc1, wl0 = T(1) - wl1;
+    auto at = [&](int64_t o) { return dv ? dv[base + o] * v[base + o] : v[base + o]; };
+    return wl0 * (wc0 * (wr0 * at(0) + wr1 * at(dr)) + wc1 * (wr0 * at(dc) + wr1 * at(dc + dr))) +
+           wl1 * (wc0 * (wr0 * at(dl) + wr1 * at(dl + dr)) + wc1 * (wr0 * at(dl + dc) + wr1 * at(dl + dc + dr)));
+}
+
+template <class T, int M>
+__global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__restrict__ partial, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const FastDiv d3 = FastDiv::make3();
+    double dotacc = 0.0;
+    FOR_SPAN(i, Span::all(3 * A.g0.nodes())) {
+        const uint32_t n = d3.div((uint32_t)i);
+        const int a = (int)((uint32_t)i - 3 * n);
+        int r, c, l;
+        A.d0.rcl(n, r, c, l);
+        T v = A.dinv0[i] * A.r0[i];
+        if (M > 1) v += tail_interp<T, 2>(A.r[0], A.dinv[0], A.g[0], r, c, l, a);
+        if (M > 2) v += tail_interp<T, 4>(A.r[1], A.dinv[1], A.g[1], r, c, l, a);
+        v += tail_interp<T, (2 << (M - 1))>(A.xc, (const T *)nullptr, A.g[M - 1], r, c, l, a);
+        const T val = (A.flag0[n] & kSolid) ? v : T(0);
+        A.x0[i] = val;
+        if (partial) dotacc += (double)val * (double)A.r0[i];
+    }
+    if (partial) {
+        const double s = block_sum(dotacc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
 // Coarsest level: x = Minv r with the dense inverse of K_L on free DOFs
 // (zero rows/columns elsewhere); one wavefront per row.
 template <class T>
@@ -1367,21 +1517,11 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
 
 // z = M r (Solving_Vcycle.m), partial z.r into `partial` when non-null.
 template <class T>
+int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl);
+
+template <class T>
 int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ctl, int *nbOut) {
-    const int L = C->L;
-    const T *rf = r;
-    for (int k = 1; k < L; ++k) {
-        SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
-        rf = lvl_r<T>(C->lv[k]);
-    }
-    Level &Lc = C->lv[L - 1];
-    const int64_t n = C->nCoarseDof;
-    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->Minv.p, n,
-                       lvl_r<T>(Lc), lvl_x<T>(Lc), ctl);
-    SGLD_HIP(hipGetLastError());
-    for (int k = L - 1; k >= 2; --k)
-        SGLD_TRY(enqueue_prolong<T>(C, k, lvl_x<T>(C->lv[k]), lvl_x<T>(C->lv[k - 1]),
-                                    lvl_r<T>(C->lv[k - 1]), nullptr, ctl, nullptr));
+    SGLD_TRY(enqueue_vcycle_coarse<T>(C, r, nullptr, nullptr, ctl));
     SGLD_TRY(enqueue_prolong<T>(C, 1, lvl_x<T>(C->lv[1]), z, r, partial, ctl, nbOut));
     return SGLD_OK;
 }
@@ -1392,27 +1532,84 @@ int fin(sgld_ctx *C, const double *partial, int nb, int op, const double *partia
     return SGLD_OK;
 }
 
+// Top t0 of the fused small-level tail (k_tail_restrict / k_tail_prolong),
+// or -1: the first level k >= 1 with at most 2^17 nodes, raised so that at
+// most kTailMax levels lie below it; none when no level lies below.
+int tail_top(const sgld_ctx *C) {
+    if (!C->tailOn) return -1;
+    const int L = C->L;
+    int k = 1;
+    while (k < L && C->lv[k].g.nodes() > ((int64_t)1 << 17)) ++k;
+    k = std::max(k, L - 1 - kTailMax);
+    return k < L - 1 ? k : -1;
+}
+
+template <class T>
+TailArgs<T> tail_args(sgld_ctx *C, int t0) {
+    TailArgs<T> A{};
+    Level &l0 = C->lv[t0];
+    A.M = C->L - 1 - t0;
+    A.g0 = l0.g;
+    A.d0 = NodeDiv::make(l0.g);
+    A.r0 = lvl_r<T>(l0);
+    A.dinv0 = lvl_dinv<T>(l0);
+    A.flag0 = l0.nflag.p;
+    A.x0 = lvl_x<T>(l0);
+    int b = 0;
+    for (int m = 0; m < A.M; ++m) {
+        Level &lm = C->lv[t0 + 1 + m];
+        A.g[m] = lm.g;
+        A.dv[m] = NodeDiv::make(lm.g);
+        A.r[m] = lvl_r<T>(lm);
+        A.dinv[m] = lvl_dinv<T>(lm);
+        A.flag[m] = lm.nflag.p;
+        A.blk[m] = b;
+        const int perBlock = 256 / tail_lanes(m);
+        b += (int)((lm.g.nodes() + perBlock - 1) / perBlock);
+    }
+    A.blk[A.M] = b;
+    A.xc = lvl_x<T>(C->lv[C->L - 1]);
+    return A;
+}
+
 // The V-cycle below the finest level for residual r (Solving_Vcycle.m:12-49
 // for levels >= 2): restrictions down to the coarsest, its dense solve, and
-// prolongation + smoothing back up to level 2, whose x_2 stays in lv[1].
-// partialX2 receives the partials of x_2 . r_2 (count in *nbX2).
+// prolongation + smoothing back up to level 2, whose x_2 stays in lv[1]; the
+// small levels run as the fused tail.  partialX2 (if not null) receives the
+// partials of x_2 . r_2 (count in *nbX2).
 template <class T>
 int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl) {
-    const int L = C->L;
+    const int L = C->L, t0 = tail_top(C);
+    const int kLast = t0 > 0 ? t0 : L - 1;  // last level reached by single restrictions
     const T *rf = r;
-    for (int k = 1; k < L; ++k) {
+    for (int k = 1; k <= kLast; ++k) {
         SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
         rf = lvl_r<T>(C->lv[k]);
+    }
+    TailArgs<T> A{};
+    if (t0 > 0) {
+        A = tail_args<T>(C, t0);
+        hipLaunchKernelGGL(k_tail_restrict<T>, dim3(A.blk[A.M]), dim3(256), 0, C->s, A, ctl);
+        SGLD_HIP(hipGetLastError());
     }
     Level &Lc = C->lv[L - 1];
     const int64_t n = C->nCoarseDof;
     hipLaunchKernelGGL(k_coarse_gemv<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->Minv.p, n,
                        lvl_r<T>(Lc), lvl_x<T>(Lc), ctl);
     SGLD_HIP(hipGetLastError());
-    for (int k = L - 1; k >= 2; --k)
+    if (t0 > 0) {
+        const int nb = grid_blocks(3 * A.g0.nodes());
+        double *part = t0 == 1 ? partialX2 : nullptr;
+        if (A.M == 1) hipLaunchKernelGGL((k_tail_prolong<T, 1>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
+        else if (A.M == 2) hipLaunchKernelGGL((k_tail_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
+        else hipLaunchKernelGGL((k_tail_prolong<T, 3>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
+        SGLD_HIP(hipGetLastError());
+        if (part && nbX2) *nbX2 = nb;
+    }
+    for (int k = kLast; k >= 2; --k)
         SGLD_TRY(enqueue_prolong<T>(C, k, lvl_x<T>(C->lv[k]), lvl_x<T>(C->lv[k - 1]), lvl_r<T>(C->lv[k - 1]),
                                     k == 2 ? partialX2 : nullptr, ctl, k == 2 ? nbX2 : nullptr));
-    if (L == 2) {  // level 2 is the coarsest: x_2 . r_2 by its own pass
+    if (L == 2 && partialX2) {  // level 2 is the coarsest: x_2 . r_2 by its own pass
         const Span S2 = Span::all(3 * C->lv[1].g.nodes());
         *nbX2 = grid_blocks(S2.count);
         hipLaunchKernelGGL(k_dot<T>, dim3(*nbX2), dim3(256), 0, C->s, (const T *)lvl_x<T>(Lc),
@@ -1788,6 +1985,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (nonDyadic == 1 && numLevels >= 4) numLevels -= 1; else nonDyadic = 0;
     C->L = numLevels;
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
+    if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
     C->nonDyadic = nonDyadic;
     C->nu = nu;
     C->h = h;
