@@ -263,6 +263,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                                                        const Ctl *ctl) {
     if (stopped(ctl)) return;
     __shared__ T us[3][kTN * kTN * 3];
+    // node forces of two layers, FP64 in both precisions: native ds_add_f32
+    // made the FP32 kernel 6.7x slower than ds_add_f64 (337 vs 50 us, cfg 1)
     __shared__ double acc[2][kTN * kTN * 3];
     const int t = threadIdx.x;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
@@ -1483,7 +1485,7 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     const NodeDiv dc = NodeDiv::make(Cc.g);
     if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
-        const int czc = 4, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
+        const int czc = C->restrictCz, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
         hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(tilesR * tilesC * tilesZ), dim3(256), 0, C->s, rf,
                            F.g, rc, Cc.g, Cc.nflag.p, czc, tilesR, tilesC, ctl);
     } else if (Cc.span == 2)
@@ -1986,6 +1988,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     C->L = numLevels;
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
     C->nonDyadic = nonDyadic;
     C->nu = nu;
     C->h = h;

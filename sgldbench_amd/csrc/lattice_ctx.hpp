@@ -79,6 +79,7 @@ struct sgld_ctx {
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
+    int restrictCz = 4;  // coarse layers per block of the tiled span-2 restriction
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
     // what holds solid material.  dofChunks / nodeChunks list the 64-entry
