@@ -376,88 +376,81 @@ __global__ void __launch_bounds__(256) k_bfin(const double *__restrict__ partial
 // ------------------------------ batched V-cycle -----------------------------
 template <int S>
 __global__ void __launch_bounds__(256) k_brestrict(const float *__restrict__ rf, Grid gf, float *__restrict__ rc,
-                                                   Grid gc, const uint8_t *__restrict__ cflag) {
+                                                   Grid gc, NodeDiv dc, const uint8_t *__restrict__ cflag) {
     const int j = threadIdx.x & 15;
     const int64_t nN = gc.nodes();
-    const int64_t NYc = gc.NY(), NXc = gc.NX();
     const int64_t NYf = gf.NY(), NXf = gf.NX(), NZf = gf.NZ();
     for (int64_t G = gtid() >> 4; G < nN; G += gstride() >> 4) {
-        const int R = (int)(G % NYc), Cc = (int)((G / NYc) % NXc), Lc = (int)(G / (NYc * NXc));
+        int R, Cc, Lc;
+        dc.rcl((uint32_t)G, R, Cc, Lc);
+        // branch-free stencil (clamped index, weight 0 outside): the loads
+        // issue together instead of one after another
         float a0 = 0, a1 = 0, a2 = 0;
-        if (cflag[G] & kSolid) {
 #pragma unroll
-            for (int dl = -(S - 1); dl <= S - 1; ++dl) {
-                const int64_t lf = (int64_t)S * Lc + dl;
-                if (lf < 0 || lf >= NZf) continue;
-                const float wz = float(S - (dl < 0 ? -dl : dl)) / float(S);
+        for (int dl = -(S - 1); dl <= S - 1; ++dl) {
+            const int64_t lu = (int64_t)S * Lc + dl;
+            const int64_t lf = lu < 0 ? 0 : (lu >= NZf ? NZf - 1 : lu);
+            const float wz = lu == lf ? float(S - (dl < 0 ? -dl : dl)) / float(S) : 0.0f;
 #pragma unroll
-                for (int dc = -(S - 1); dc <= S - 1; ++dc) {
-                    const int64_t cf = (int64_t)S * Cc + dc;
-                    if (cf < 0 || cf >= NXf) continue;
-                    const float wx = float(S - (dc < 0 ? -dc : dc)) / float(S);
+            for (int dcc = -(S - 1); dcc <= S - 1; ++dcc) {
+                const int64_t cu = (int64_t)S * Cc + dcc;
+                const int64_t cf = cu < 0 ? 0 : (cu >= NXf ? NXf - 1 : cu);
+                const float wx = cu == cf ? wz * float(S - (dcc < 0 ? -dcc : dcc)) / float(S) : 0.0f;
 #pragma unroll
-                    for (int dr = -(S - 1); dr <= S - 1; ++dr) {
-                        const int64_t rr = (int64_t)S * R + dr;
-                        if (rr < 0 || rr >= NYf) continue;
-                        const float w = wz * wx * (float(S - (dr < 0 ? -dr : dr)) / float(S));
-                        const float *pn = rf + 3 * (rr + NYf * cf + NYf * NXf * lf) * NR + j;
-                        a0 += w * pn[0];
-                        a1 += w * pn[NR];
-                        a2 += w * pn[2 * NR];
-                    }
+                for (int dr = -(S - 1); dr <= S - 1; ++dr) {
+                    const int64_t ru = (int64_t)S * R + dr;
+                    const int64_t rr = ru < 0 ? 0 : (ru >= NYf ? NYf - 1 : ru);
+                    const float w = ru == rr ? wx * float(S - (dr < 0 ? -dr : dr)) / float(S) : 0.0f;
+                    const float *pn = rf + 3 * (rr + NYf * cf + NYf * NXf * lf) * NR + j;
+                    a0 += w * pn[0];
+                    a1 += w * pn[NR];
+                    a2 += w * pn[2 * NR];
                 }
             }
         }
+        const bool solid = cflag[G] & kSolid;
         float *po = rc + 3 * G * NR + j;
-        po[0] = a0;
-        po[NR] = a1;
-        po[2 * NR] = a2;
+        po[0] = solid ? a0 : 0.0f;
+        po[NR] = solid ? a1 : 0.0f;
+        po[2 * NR] = solid ? a2 : 0.0f;
     }
 }
 
-// x_f = P x_c + dinv_f r_f; finest: fixed DOFs 0 and partial x_f.r_f per case
+// x_f = P x_c + dinv_f r_f; finest: fixed DOFs 0 and partial x_f.r_f per case.
+// Branch-free interpolation: per axis the coarse neighbours q and q + (m > 0)
+// with weights (S - m)/S and m/S, all inside the coarse lattice.
 template <int S>
 __global__ void __launch_bounds__(256) k_bprolong(const float *__restrict__ xc, Grid gc, float *__restrict__ xf,
-                                                  Grid gf, const float *__restrict__ rf,
+                                                  Grid gf, NodeDiv df, const float *__restrict__ rf,
                                                   const float *__restrict__ dinv,
                                                   const uint8_t *__restrict__ fflag, int finest,
                                                   double *__restrict__ partial) {
     const int j = threadIdx.x & 15;
-    const int64_t NYf = gf.NY(), NXf = gf.NX(), nN = gf.nodes();
+    const int64_t nN = gf.nodes();
     const int64_t NYc = gc.NY(), NXc = gc.NX();
     double dotacc = 0.0;
     for (int64_t n = gtid() >> 4; n < nN; n += gstride() >> 4) {
-        const int r = (int)(n % NYf), c = (int)((n / NYf) % NXf), l = (int)(n / (NYf * NXf));
-        const uint8_t f = fflag[n];
-        float v[3] = {0, 0, 0};
-        if (f & kSolid) {
-            const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+        int r, c, l;
+        df.rcl((uint32_t)n, r, c, l);
+        const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+        const float *base = xc + 3 * (qr + NYc * qc + NYc * NXc * ql) * NR + j;
+        const int64_t dr = 3 * NR * (mr > 0), dcn = 3 * NR * NYc * (mc > 0), dl = 3 * NR * NYc * NXc * (ml > 0);
+        const float wr1 = float(mr) / float(S), wc1 = float(mc) / float(S), wl1 = float(ml) / float(S);
+        const float wr0 = 1.0f - wr1, wc0 = 1.0f - wc1, wl0 = 1.0f - wl1;
+        float v[3];
 #pragma unroll
-            for (int bl = 0; bl < 2; ++bl) {
-                if (bl && !ml) continue;
-                const float wl = bl ? float(ml) / float(S) : float(S - ml) / float(S);
-#pragma unroll
-                for (int bc = 0; bc < 2; ++bc) {
-                    if (bc && !mc) continue;
-                    const float wc = bc ? float(mc) / float(S) : float(S - mc) / float(S);
-#pragma unroll
-                    for (int br = 0; br < 2; ++br) {
-                        if (br && !mr) continue;
-                        const float w = wl * wc * (br ? float(mr) / float(S) : float(S - mr) / float(S));
-                        const float *pc = xc + 3 * ((qr + br) + NYc * (qc + bc) + NYc * NXc * (ql + bl)) * NR + j;
-                        v[0] += w * pc[0];
-                        v[1] += w * pc[NR];
-                        v[2] += w * pc[2 * NR];
-                    }
-                }
-            }
+        for (int a = 0; a < 3; ++a) {
+            const float *b = base + a * NR;
+            v[a] = wl0 * (wc0 * (wr0 * b[0] + wr1 * b[dr]) + wc1 * (wr0 * b[dcn] + wr1 * b[dcn + dr])) +
+                   wl1 * (wc0 * (wr0 * b[dl] + wr1 * b[dl + dr]) + wc1 * (wr0 * b[dl + dcn] + wr1 * b[dl + dcn + dr]));
         }
+        const uint8_t f = fflag[n];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int64_t idx = (3 * n + a) * NR + j;
             const float rv = rf[idx];
             float val = v[a] + dinv[3 * n + a] * rv;
-            if (finest && (f & (1 << a))) val = 0.0f;
+            if (!(f & kSolid) || (finest && (f & (1 << a)))) val = 0.0f;
             xf[idx] = val;
             dotacc += (double)val * (double)rv;
         }
@@ -627,10 +620,13 @@ int enqueue_bvcycle(sgld_ctx *C, const float *r, float *z, int *nbOut) {
     for (int k = 1; k < L; ++k) {
         const Level &F = C->lv[k - 1], &Cc = C->lv[k];
         const int nb = bgrid(Cc.g.nodes() * NR);
+        const NodeDiv dc = NodeDiv::make(Cc.g);
         if (Cc.span == 2)
-            hipLaunchKernelGGL(k_brestrict<2>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, Cc.nflag.p);
+            hipLaunchKernelGGL(k_brestrict<2>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, dc,
+                               Cc.nflag.p);
         else
-            hipLaunchKernelGGL(k_brestrict<4>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, Cc.nflag.p);
+            hipLaunchKernelGGL(k_brestrict<4>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, dc,
+                               Cc.nflag.p);
         SGLD_HIP(hipGetLastError());
         rf = B->lr[k].p;
     }
@@ -644,11 +640,12 @@ int enqueue_bvcycle(sgld_ctx *C, const float *r, float *z, int *nbOut) {
         const float *rfk = finest ? r : B->lr[k - 1].p;
         const int nb = bgrid(F.g.nodes() * NR);
         double *part = finest ? B->part.p : nullptr;
+        const NodeDiv df = NodeDiv::make(F.g);
         if (Cc.span == 2)
-            hipLaunchKernelGGL(k_bprolong<2>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, rfk,
+            hipLaunchKernelGGL(k_bprolong<2>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, df, rfk,
                                F.dinv32.p, F.nflag.p, (int)finest, part);
         else
-            hipLaunchKernelGGL(k_bprolong<4>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, rfk,
+            hipLaunchKernelGGL(k_bprolong<4>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, df, rfk,
                                F.dinv32.p, F.nflag.p, (int)finest, part);
         SGLD_HIP(hipGetLastError());
         if (finest) *nbOut = nb;
@@ -736,6 +733,13 @@ int solve_batched(sgld_ctx *C, int nrhs, const double *dF, double *dU, double to
             if (conv || stretch || h.its[j] >= maxit || !(h.res[j] == h.res[j])) flag |= 1u << j;
             if (!conv && stretch) forced |= 1u << j;
         }
+        // An update event sweeps the interleaved vectors whatever the number
+        // of cases it serves, so cases already past sqrt(delta) of their
+        // drop join it (an early reliable update only adds accuracy).
+        if (flag)
+            for (int j = 0; j < NR; ++j)
+                if (!done[j] && h.active[j] && !((flag >> j) & 1u) && h.res[j] < std::sqrt(itol[j]))
+                    flag |= 1u << j;
         if (flag) {
             std::vector<double> sc(NR, 0.0);
             for (int j = 0; j < NR; ++j) sc[j] = scale[j];
