@@ -48,6 +48,27 @@ __device__ __forceinline__ double block_sum(double v) {
 
 __device__ __forceinline__ bool stopped(const Ctl *ctl) { return ctl && ctl->done; }
 
+// Finest-level Jacobi weights dinv = 2 omega / diag: FP64 for the FP64 solve;
+// for the FP32 inner solve of the mixed mode rounded to bfloat16 (a diagonal
+// preconditioner perturbed by <= 2^-9 relative is still SPD, it is used
+// consistently in z and rho, and the FP64 reliable updates fix the accuracy),
+// which halves the bytes of the two finest passes that read it.
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite f)
+}
+template <class T> struct FineW;
+template <> struct FineW<double> {
+    using D = double;
+    __device__ __forceinline__ static double get(const double *p, int64_t i) { return p[i]; }
+};
+template <> struct FineW<float> {
+    using D = uint16_t;
+    __device__ __forceinline__ static float get(const uint16_t *p, int64_t i) {
+        return __uint_as_float((uint32_t)p[i] << 16);
+    }
+};
+
 // Grid-stride passes over the entries of a Span (all entries, or the listed
 // 64-entry chunks of an irregular domain).
 #define FOR_SPAN(i, S)                                                                           \
@@ -679,7 +700,8 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
 // ----------------------------------------------------------------------------
 template <class T, int S>
 __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, Grid gc, NodeDiv df,
-                                                     const T *__restrict__ rf, const T *__restrict__ dinv,
+                                                     const T *__restrict__ rf,
+                                                     const typename FineW<T>::D *__restrict__ dinv,
                                                      const uint8_t *__restrict__ fflag, T *__restrict__ x,
                                                      T *__restrict__ p, Span dofs, const Ctl *ctl) {
     if (stopped(ctl)) return;
@@ -689,7 +711,7 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         const T pv = interp_dof<T, S>(xc, gc, df, n, a);
-        const T sv = dinv[i] * rf[i];
+        const T sv = FineW<T>::get(dinv, i) * rf[i];
         const uint8_t f = fflag[n];
         const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
         const T po = p[i];
@@ -725,7 +747,7 @@ __device__ __forceinline__ void block_sum2(double &a, double &b) {
 // share of the next rho).
 template <class T>
 __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
-                                                  const T *__restrict__ dinv, Span S,
+                                                  const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
                                                   const Ctl *ctl) {
     if (stopped(ctl)) return;
@@ -736,7 +758,7 @@ __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__
         r[i] = rn;
         const double rd = (double)rn;
         rr += rd * rd;
-        dr += (double)dinv[i] * rd * rd;
+        dr += (double)FineW<T>::get(dinv, i) * rd * rd;
     }
     block_sum2(rr, dr);
     if (threadIdx.x == 0) {
@@ -747,12 +769,12 @@ __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__
 
 // partial of sum dinv r^2 (rho of a residual that k_update_r did not produce)
 template <class T>
-__global__ void __launch_bounds__(256) k_dinv_rr(const T *__restrict__ r, const T *__restrict__ dinv, Span S,
-                                                 double *__restrict__ partial) {
+__global__ void __launch_bounds__(256) k_dinv_rr(const T *__restrict__ r, const typename FineW<T>::D *__restrict__ dinv,
+                                                 Span S, double *__restrict__ partial) {
     double acc = 0.0;
     FOR_SPAN(i, S) {
         const double rd = (double)r[i];
-        acc += (double)dinv[i] * rd * rd;
+        acc += (double)FineW<T>::get(dinv, i) * rd * rd;
     }
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -767,6 +789,40 @@ __global__ void k_residual_masked(const double *__restrict__ b, const double *__
     FOR_SPAN(i, S) {
         const uint32_t node = (uint32_t)(i / 3), a = (uint32_t)(i - 3 * (int64_t)node);
         r[i] = (nflag[node] & (1u << a)) ? 0.0 : b[i] - q[i];
+    }
+}
+
+// Reliable update of the mixed mode: x += scale (x32 + lambda p32), x32 = 0
+// (the FP32 correction, with the stopped iteration's deferred lambda p32,
+// folded into the FP64 solution).
+__global__ void k_fold_mixed(double *__restrict__ x, float *__restrict__ x32, const float *__restrict__ p32,
+                             double scale, Span S, const Ctl *ctl) {
+    const float lam = (float)ctl->lambda;
+    FOR_SPAN(i, S) {
+        x[i] += scale * (double)(x32[i] + lam * p32[i]);
+        x32[i] = 0.0f;
+    }
+}
+
+// r = mask(b - q) (zero on fixed DOFs, as k_residual_masked) with partials of
+// r.r into partRR and b.b into partBB.
+__global__ void __launch_bounds__(256) k_residual_norms(const double *__restrict__ b, const double *__restrict__ q,
+                                                        double *__restrict__ r, const uint8_t *__restrict__ nflag,
+                                                        Span S, double *__restrict__ partRR,
+                                                        double *__restrict__ partBB) {
+    double rr = 0.0, bb = 0.0;
+    FOR_SPAN(i, S) {
+        const uint32_t node = (uint32_t)(i / 3), a = (uint32_t)(i - 3 * (int64_t)node);
+        const double bi = b[i];
+        const double ri = (nflag[node] & (1u << a)) ? 0.0 : bi - q[i];
+        r[i] = ri;
+        rr += ri * ri;
+        bb += bi * bi;
+    }
+    block_sum2(rr, bb);
+    if (threadIdx.x == 0) {
+        partRR[blockIdx.x] = rr;
+        partBB[blockIdx.x] = bb;
     }
 }
 
@@ -1013,11 +1069,6 @@ __global__ void k_scale(T *v, T f, Span S) {
     FOR_SPAN(i, S) v[i] *= f;
 }
 
-// x += scale * d  (FP64 x, FP32 correction d)
-__global__ void k_axpy_mixed(double *__restrict__ x, const float *__restrict__ d, double scale, Span S) {
-    FOR_SPAN(i, S) x[i] += scale * (double)d[i];
-}
-
 enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_PQ, FIN_RR, FIN_RHO };
 
 // Second reduction stage (fixed order, deterministic) and the scalar updates
@@ -1059,7 +1110,8 @@ __global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial
 // Finest diagonal: diag = sum_e E_e Ke_jj (Solving_AssembleFEAstencil.m:102-121);
 // dinv = 2 omega / diag on solid nodes.
 __global__ void k_diag0(const double *__restrict__ E, Grid gd, const uint8_t *__restrict__ nflag,
-                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32) {
+                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32,
+                        uint16_t *__restrict__ dinv16) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= gd.nodes()) return;
     const int64_t NY = gd.NY(), NX = gd.NX();
@@ -1079,6 +1131,7 @@ __global__ void k_diag0(const double *__restrict__ E, Grid gd, const uint8_t *__
         const double v = solid ? 2.0 * omega / d[a] : 0.0;
         dinv[3 * g + a] = v;
         dinv32[3 * g + a] = (float)v;
+        dinv16[3 * g + a] = to_bf16((float)v);
     }
 }
 
@@ -1375,6 +1428,10 @@ template <> float *lvl_x<float>(Level &l) { return l.x32.p; }
 template <class T> const T *lvl_dinv(Level &l);
 template <> const double *lvl_dinv<double>(Level &l) { return l.dinv.p; }
 template <> const float *lvl_dinv<float>(Level &l) { return l.dinv32.p; }
+// finest-level Jacobi weights of the PCG passes (FP64, or bfloat16 for FP32)
+template <class T> const typename FineW<T>::D *fine_dinv(sgld_ctx *C);
+template <> const double *fine_dinv<double>(sgld_ctx *C) { return C->lv[0].dinv.p; }
+template <> const uint16_t *fine_dinv<float>(sgld_ctx *C) { return C->lv[0].dinv16.p; }
 template <class T> const T *ctx_E(sgld_ctx *C);
 template <> const double *ctx_E<double>(sgld_ctx *C) { return C->E.p; }
 template <> const float *ctx_E<float>(sgld_ctx *C) { return C->E32.p; }
@@ -1628,7 +1685,7 @@ template <class T>
 int enqueue_rho(sgld_ctx *C, const T *r, int op) {
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
-    hipLaunchKernelGGL(k_dinv_rr<T>, dim3(nbv), dim3(256), 0, C->s, r, lvl_dinv<T>(C->lv[0]), V, slot(C, 3));
+    hipLaunchKernelGGL(k_dinv_rr<T>, dim3(nbv), dim3(256), 0, C->s, r, fine_dinv<T>(C), V, slot(C, 3));
     SGLD_HIP(hipGetLastError());
     int nb2 = 0;
     SGLD_TRY(enqueue_vcycle_coarse<T>(C, r, slot(C, 2), &nb2, C->ctl.p));
@@ -1652,15 +1709,15 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     const int nbv = grid_blocks(S.count);
     if (l1.span == 2)
         hipLaunchKernelGGL((k_prolong_dir<T, 2>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
-                           (const T *)r, lvl_dinv<T>(l0), l0.nflag.p, x, p, S, (const Ctl *)ctl);
+                           (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
     else
         hipLaunchKernelGGL((k_prolong_dir<T, 4>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
-                           (const T *)r, lvl_dinv<T>(l0), l0.nflag.p, x, p, S, (const Ctl *)ctl);
+                           (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
     SGLD_HIP(hipGetLastError());
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
     SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
-    hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, lvl_dinv<T>(l0), S,
+    hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, fine_dinv<T>(C), S,
                        slot(C, 1), slot(C, 3), (const Ctl *)ctl);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
@@ -1797,10 +1854,8 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     S.its = st.its;
     if (!st.done && S.its - S.itsAtUpdate < SolveState::kMaxStretch) return SGLD_OK;
     S.itsAtUpdate = S.its;
-    hipLaunchKernelGGL(k_axpy_lambda<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, (const float *)C->p32.p, V,
-                       (const Ctl *)C->ctl.p);  // the pending x += lambda p
-    hipLaunchKernelGGL(k_axpy_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, S.scale, V);
-    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
+    hipLaunchKernelGGL(k_fold_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, (const float *)C->p32.p,
+                       S.scale, V, (const Ctl *)C->ctl.p);  // with the pending x32 += lambda p32
     SGLD_HIP(hipGetLastError());
     double res = 0, normr = 0;
     SGLD_TRY(true_residual(C, &res, &normr));
@@ -1890,11 +1945,9 @@ int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double 
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
     SGLD_TRY(enqueue_kbyu<double>(C, x, C->q.p, nullptr, nullptr, nullptr));
-    hipLaunchKernelGGL(k_residual_masked, dim3(nbv), dim3(256), 0, C->s, b, C->q.p, r, C->lv[0].nflag.p, V);
-    SGLD_HIP(hipGetLastError());
     std::vector<double> part(2 * kRedBlocks);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, r, r, V, slot(C, 0), nullptr);
-    hipLaunchKernelGGL(k_dot<double>, dim3(nbv), dim3(256), 0, C->s, b, b, V, slot(C, 1), nullptr);
+    hipLaunchKernelGGL(k_residual_norms, dim3(nbv), dim3(256), 0, C->s, b, (const double *)C->q.p, r,
+                       C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1));
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
                             hipMemcpyDeviceToHost, C->s));
@@ -2092,6 +2145,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         SGLD_TRY(upload_flags(l));
         SGLD_TRY(l.dinv.alloc(n));
         SGLD_TRY(l.dinv32.alloc(n));
+        if (k == 0) SGLD_TRY(l.dinv16.alloc(n));
         if (k > 0) {
             SGLD_TRY(l.r.alloc(n)); SGLD_TRY(l.x.alloc(n));
             SGLD_TRY(l.r32.alloc(n)); SGLD_TRY(l.x32.alloc(n));
@@ -2234,7 +2288,7 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
     C->omega = omega;
     Level &l0 = C->lv[0];
     hipLaunchKernelGGL(k_diag0, dim3(blocks_for(l0.g.nodes(), 256)), dim3(256), 0, C->s, C->E.p, l0.g,
-                       l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p);
+                       l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p, l0.dinv16.p);
     SGLD_HIP(hipGetLastError());
     DevBuf<double> KsPrev, KsCur;
     for (int k = 1; k < C->L; ++k) {

@@ -46,6 +46,7 @@ struct Level {
     DevBuf<uint8_t> nflag;
     DevBuf<double> dinv, r, x;
     DevBuf<float> dinv32, r32, x32;
+    DevBuf<uint16_t> dinv16;  // finest level: bfloat16 Jacobi weights of the FP32 inner solve
 };
 
 struct BatchState;             // batched.hip
