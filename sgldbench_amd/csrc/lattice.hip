@@ -720,6 +720,61 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
     }
 }
 
+// The same pass with the coarse values staged in LDS (span 2, used when the
+// coarse column fits): one work item is the pair of fine columns (2k, 2k+1)
+// of node layer l, a contiguous DOF range; its interpolation reads coarse
+// columns k, k+1 of coarse layers l/2, l/2 + (l odd), which are copied to LDS
+// once (4 contiguous runs of 3 NYc values) instead of being gathered from the
+// vector cache eight times per DOF.  Clamped runs carry weight 0.
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ xc, Grid gc, Grid gf,
+                                                          const T *__restrict__ rf,
+                                                          const typename FineW<T>::D *__restrict__ dinv,
+                                                          const uint8_t *__restrict__ fflag, T *__restrict__ x,
+                                                          T *__restrict__ p, const int32_t *__restrict__ items,
+                                                          int64_t nItems, const Ctl *ctl) {
+    extern __shared__ unsigned char smem_raw[];
+    T *cs = reinterpret_cast<T *>(smem_raw);  // [layer][col][3 NYc]
+    if (stopped(ctl)) return;
+    const T lam = (T)ctl->lambda, beta = (T)ctl->beta;
+    const int NYf = gf.ny + 1, NXf = gf.nx + 1;
+    const int NYc = gc.ny + 1, NXc = gc.nx + 1, NZc = gc.nz + 1;
+    const int ppl = (NXf + 1) / 2, runLen = 3 * NYc, colLen = 3 * NYf;
+    for (int64_t w = blockIdx.x; w < nItems; w += gridDim.x) {
+        const int item = items ? items[w] : (int)w;
+        const int l = item / ppl, k = item - l * ppl;
+        const int c0 = 2 * k, ncol = (2 * k + 1 < NXf) ? 2 : 1;
+        const int ql = l >> 1, ml = l & 1;
+        __syncthreads();  // the previous item is done with LDS
+        for (int t = threadIdx.x; t < 4 * runLen; t += blockDim.x) {
+            const int run = t / runLen, off = t - run * runLen;
+            const int cc = min(k + (run & 1), NXc - 1), ll = min(ql + (run >> 1), NZc - 1);
+            cs[t] = xc[3 * ((int64_t)NYc * (cc + (int64_t)NXc * ll)) + off];
+        }
+        __syncthreads();
+        const int64_t d0 = 3 * ((int64_t)NYf * (c0 + (int64_t)NXf * l));
+        const T wl1 = T(ml) * T(0.5), wl0 = T(1) - wl1;
+        for (int t = threadIdx.x; t < ncol * colLen; t += blockDim.x) {
+            const int cl = t >= colLen ? 1 : 0;  // fine column c0 + cl; odd columns sit between coarse columns
+            const int rem = t - cl * colLen, r = rem / 3, a = rem - 3 * r;
+            const int qr = r >> 1, mr = r & 1;
+            const T wr1 = T(mr) * T(0.5), wr0 = T(1) - wr1, wc1 = T(cl) * T(0.5), wc0 = T(1) - wc1;
+            const T *b0 = cs + 3 * qr + a;  // run (layer 0, col 0)
+            const int dr = 3 * mr;
+            const T v = wl0 * (wc0 * (wr0 * b0[0] + wr1 * b0[dr]) + wc1 * (wr0 * b0[runLen] + wr1 * b0[runLen + dr])) +
+                        wl1 * (wc0 * (wr0 * b0[2 * runLen] + wr1 * b0[2 * runLen + dr]) +
+                               wc1 * (wr0 * b0[3 * runLen] + wr1 * b0[3 * runLen + dr]));
+            const int64_t i = d0 + t;
+            const uint8_t f = fflag[(d0 / 3) + cl * NYf + r];
+            const T sv = FineW<T>::get(dinv, i) * rf[i];
+            const T z = ((f & kSolid) && !(f & (1 << a))) ? v + sv : T(0);
+            const T po = p[i];
+            x[i] = x[i] + lam * po;
+            p[i] = z + beta * po;
+        }
+    }
+}
+
 __device__ __forceinline__ void block_sum2(double &a, double &b) {
     __shared__ double sh[2][16];
     for (int o = 32; o > 0; o >>= 1) {
@@ -1707,7 +1762,18 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     Level &l0 = C->lv[0], &l1 = C->lv[1];
     const NodeDiv d0 = NodeDiv::make(l0.g);
     const int nbv = grid_blocks(S.count);
-    if (l1.span == 2)
+    // LDS-staged coarse columns where the gathers bound the pass: FP64 on a
+    // full cuboid (measured 146 -> 105 us on 256x128x128).  FP32 is HBM-bound
+    // either way, and on irregular domains whole column pairs cost more than
+    // the 64-entry chunks of the DOF pass (cfg 2: 1.86 -> 2.51 s).
+    const size_t colBytes = sizeof(T) * 4 * 3 * (size_t)l1.g.NY();  // staged coarse runs
+    if (l1.span == 2 && C->dirCols && sizeof(T) == 8 && C->full && colBytes <= 48 * 1024) {
+        const int64_t items = ((l0.g.NX() + 1) / 2) * l0.g.NZ();
+        const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(items, 65536));
+        hipLaunchKernelGGL(k_prolong_dir_cols<T>, dim3(nb), dim3(256), colBytes, C->s, (const T *)lvl_x<T>(l1),
+                           l1.g, l0.g, (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, (const int32_t *)nullptr,
+                           items, (const Ctl *)ctl);
+    } else if (l1.span == 2)
         hipLaunchKernelGGL((k_prolong_dir<T, 2>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
                            (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
     else
@@ -2041,6 +2107,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     C->L = numLevels;
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
     C->nonDyadic = nonDyadic;
     C->nu = nu;

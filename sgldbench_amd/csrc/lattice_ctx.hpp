@@ -81,6 +81,7 @@ struct sgld_ctx {
     int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     int restrictCz = 4;  // coarse layers per block of the tiled span-2 restriction
+    bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
     // what holds solid material.  dofChunks / nodeChunks list the 64-entry
