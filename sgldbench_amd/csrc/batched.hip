@@ -317,41 +317,56 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
 __device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
 
-// xc += alpha p; r -= alpha q; partial r.r per case
-__global__ void __launch_bounds__(256) k_bupdate_xr(float *__restrict__ xc, float *__restrict__ r,
-                                                    const float *__restrict__ p, const float *__restrict__ q,
-                                                    int64_t n16, const BCtl *__restrict__ ctl,
-                                                    double *__restrict__ partial) {
+// r -= alpha q per case; partials r.r and sum dinv r^2 per case (the finest
+// smoother's share of the next rho; dinv is per DOF, shared by the cases).
+__global__ void __launch_bounds__(256) k_bupdate_r(float *__restrict__ r, const float *__restrict__ q,
+                                                   const float *__restrict__ dinv, int64_t n16,
+                                                   const BCtl *__restrict__ ctl, double *__restrict__ partRR,
+                                                   double *__restrict__ partDR) {
     const float al = (float)ctl->alpha[threadIdx.x & 15];
-    double acc = 0.0;
+    double rr = 0.0, dr = 0.0;
     for (int64_t i = gtid(); i < n16; i += gstride()) {
-        xc[i] += al * p[i];
         const float rn = r[i] - al * q[i];
         r[i] = rn;
-        acc += (double)rn * (double)rn;
+        const double rd = (double)rn;
+        rr += rd * rd;
+        dr += (double)dinv[i >> 4] * rd * rd;
     }
-    block_sum16(acc, partial + (int64_t)blockIdx.x * NR);
+    block_sum16(rr, partRR + (int64_t)blockIdx.x * NR);
+    block_sum16(dr, partDR + (int64_t)blockIdx.x * NR);
 }
 
-// p_new = z + beta p_old
-__global__ void __launch_bounds__(256) k_bupdate_p(float *__restrict__ pnew, const float *__restrict__ z,
-                                                   const float *__restrict__ pold, int64_t n16,
-                                                   const BCtl *__restrict__ ctl) {
-    const float be = (float)ctl->beta[threadIdx.x & 15];
-    for (int64_t i = gtid(); i < n16; i += gstride())
-        pnew[i] = z[i] + be * pold[i];
+// partials of sum dinv r^2 per case (a residual replaced by reliable updates)
+__global__ void __launch_bounds__(256) k_bdinv_rr(const float *__restrict__ r, const float *__restrict__ dinv,
+                                                  int64_t n16, double *__restrict__ partDR) {
+    double dr = 0.0;
+    for (int64_t i = gtid(); i < n16; i += gstride()) {
+        const double rd = (double)r[i];
+        dr += (double)dinv[i >> 4] * rd * rd;
+    }
+    block_sum16(dr, partDR + (int64_t)blockIdx.x * NR);
+}
+
+// partials of a.b per case over n16 interleaved entries
+__global__ void __launch_bounds__(256) k_bdot(const float *__restrict__ a, const float *__restrict__ b, int64_t n16,
+                                              double *__restrict__ part) {
+    double acc = 0.0;
+    for (int64_t i = gtid(); i < n16; i += gstride()) acc += (double)a[i] * (double)b[i];
+    block_sum16(acc, part + (int64_t)blockIdx.x * NR);
 }
 
 enum BFin { BFIN_ALPHA = 0, BFIN_RR, BFIN_RHO };
 
 // Per-case reduction of partial[nb][16] and the scalar step of the PCG
 // recurrence (Solving_PreconditionedConjugateGradientSolver.m:15-28).
-__global__ void __launch_bounds__(256) k_bfin(const double *__restrict__ partial, int nb, int op,
+__global__ void __launch_bounds__(256) k_bfin(const double *__restrict__ partial, int nb,
+                                              const double *__restrict__ partial2, int nb2, int op,
                                               BCtl *__restrict__ ctl) {
     __shared__ double sh[16][NR];
     const int j = threadIdx.x & 15, grp = threadIdx.x >> 4;
     double s = 0.0;
     for (int b = grp; b < nb; b += 16) s += partial[(int64_t)b * NR + j];
+    for (int b = grp; b < nb2; b += 16) s += partial2[(int64_t)b * NR + j];
     sh[grp][j] = s;
     __syncthreads();
     if (threadIdx.x >= NR) return;
@@ -458,6 +473,44 @@ __global__ void __launch_bounds__(256) k_bprolong(const float *__restrict__ xc, 
     if (partial) block_sum16(dotacc, partial + (int64_t)blockIdx.x * NR);
 }
 
+// Finest level of the V-cycle folded into the direction update, per case j:
+// z = mask(P x_2 + dinv r), xc += alpha_j p (the previous iteration's
+// deferred update), p = z + beta_j p in place; z is never stored (rho comes
+// from x_2.r_2 + sum dinv r^2 because R = P^T and r = 0 on fixed DOFs).
+template <int S>
+__global__ void __launch_bounds__(256) k_bprolong_dir(const float *__restrict__ xc2, Grid gc, Grid gf, NodeDiv df,
+                                                      const float *__restrict__ rf, const float *__restrict__ dinv,
+                                                      const uint8_t *__restrict__ fflag, float *__restrict__ xc,
+                                                      float *__restrict__ p, const BCtl *__restrict__ ctl) {
+    const int j = threadIdx.x & 15;
+    const float al = (float)ctl->alpha[j], be = (float)ctl->beta[j];
+    const int64_t nN = gf.nodes();
+    const int64_t NYc = gc.NY(), NXc = gc.NX();
+    for (int64_t n = gtid() >> 4; n < nN; n += gstride() >> 4) {
+        int r, c, l;
+        df.rcl((uint32_t)n, r, c, l);
+        const int qr = r / S, mr = r % S, qc = c / S, mc = c % S, ql = l / S, ml = l % S;
+        const float *base = xc2 + 3 * (qr + NYc * qc + NYc * NXc * ql) * NR + j;
+        const int64_t dr = 3 * NR * (mr > 0), dcn = 3 * NR * NYc * (mc > 0), dl = 3 * NR * NYc * NXc * (ml > 0);
+        const float wr1 = float(mr) / float(S), wc1 = float(mc) / float(S), wl1 = float(ml) / float(S);
+        const float wr0 = 1.0f - wr1, wc0 = 1.0f - wc1, wl0 = 1.0f - wl1;
+        const uint8_t f = fflag[n];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float *b = base + a * NR;
+            const float v = wl0 * (wc0 * (wr0 * b[0] + wr1 * b[dr]) + wc1 * (wr0 * b[dcn] + wr1 * b[dcn + dr])) +
+                            wl1 * (wc0 * (wr0 * b[dl] + wr1 * b[dl + dr]) +
+                                   wc1 * (wr0 * b[dl + dcn] + wr1 * b[dl + dcn + dr]));
+            const int64_t idx = (3 * n + a) * NR + j;
+            const float sv = dinv[3 * n + a] * rf[idx];
+            const float z = ((f & kSolid) && !(f & (1 << a))) ? v + sv : 0.0f;
+            const float po = p[idx];
+            xc[idx] += al * po;
+            p[idx] = z + be * po;
+        }
+    }
+}
+
 // Coarsest level: x = Minv r for 16 cases; one block per row, threads split
 // the columns 16 ways.
 __global__ void __launch_bounds__(256) k_bcoarse(const double *__restrict__ Minv, int64_t n,
@@ -477,16 +530,20 @@ __global__ void __launch_bounds__(256) k_bcoarse(const double *__restrict__ Minv
 }
 
 // ---------------------------- reliable updates ------------------------------
-// For the cases in `mask`: x64_j += scale_j xc_j, xc_j = 0.
-__global__ void __launch_bounds__(256) k_bfold(double *__restrict__ x64, float *__restrict__ xc, int64_t nd,
-                                               const double *__restrict__ scale, uint32_t mask) {
+// For the cases in `mask`: x64_j += scale_j (xc_j + alpha_j p_j), xc_j = 0
+// (with the deferred update of the case's last iteration).
+__global__ void __launch_bounds__(256) k_bfold(double *__restrict__ x64, float *__restrict__ xc,
+                                               const float *__restrict__ p, int64_t nd,
+                                               const double *__restrict__ scale, uint32_t mask,
+                                               const BCtl *__restrict__ ctl) {
     const int j = threadIdx.x & 15;
     if (!((mask >> j) & 1u)) return;
     const double sc = scale[j];
+    const float al = (float)ctl->alpha[j];
     double *xj = x64 + (int64_t)j * nd;
     for (int64_t i = gtid() >> 4; i < nd; i += gstride() >> 4) {
         const int64_t idx = i * NR + j;
-        xj[i] += sc * (double)xc[idx];
+        xj[i] += sc * (double)(xc[idx] + al * p[idx]);
         xc[idx] = 0.0f;
     }
 }
@@ -534,10 +591,11 @@ __global__ void __launch_bounds__(256) k_bunpack(const float *__restrict__ src, 
 // ================================ host side =================================
 struct BatchState {
     int64_t nd = 0;                      // lattice DOFs of level 0
-    DevBuf<float> r, z, q, xc, p0, p1;   // level 0, 16 cases
+    DevBuf<float> r, q, xc, p0;          // level 0, 16 cases
     std::vector<DevBuf<float>> lr, lx;   // levels >= 1
     DevBuf<double> b64, x64, r64;        // 16 case-major FP64 lattice vectors each
-    DevBuf<double> part, scal;           // partial sums [blocks][16]; per-case scalars
+    DevBuf<double> part, part2, part3;   // partial sums [blocks][16] (three reductions in flight)
+    DevBuf<double> scal;                 // per-case scalars
     DevBuf<float> Kc;                    // Ke (24 x 24, column-major) in FP32
     DevBuf<BCtl> ctl;
     BCtl *hctl = nullptr;                // pinned mirror
@@ -561,11 +619,9 @@ int batch_init(sgld_ctx *C) {
     B->nd = nd;
     const size_t n16 = (size_t)nd * NR;
     SGLD_TRY(B->r.alloc(n16));
-    SGLD_TRY(B->z.alloc(n16));
     SGLD_TRY(B->q.alloc(n16));
     SGLD_TRY(B->xc.alloc(n16));
     SGLD_TRY(B->p0.alloc(n16));
-    SGLD_TRY(B->p1.alloc(n16));
     B->lr.resize(C->L);
     B->lx.resize(C->L);
     for (int k = 1; k < C->L; ++k) {
@@ -577,6 +633,8 @@ int batch_init(sgld_ctx *C) {
     SGLD_TRY(B->x64.alloc(n16));
     SGLD_TRY(B->r64.alloc(n16));
     SGLD_TRY(B->part.alloc((size_t)kBRedBlocks * NR));
+    SGLD_TRY(B->part2.alloc((size_t)kBRedBlocks * NR));
+    SGLD_TRY(B->part3.alloc((size_t)kBRedBlocks * NR));
     SGLD_TRY(B->scal.alloc(4 * NR));
     SGLD_TRY(B->ctl.alloc(1));
     SGLD_HIP(hipHostMalloc((void **)&B->hctl, sizeof(BCtl), hipHostMallocDefault));
@@ -596,8 +654,9 @@ int batch_init(sgld_ctx *C) {
 
 int bgrid(int64_t n16) { return (int)std::min<int64_t>(kBRedBlocks, std::max<int64_t>(1, (n16 + 255) / 256)); }
 
-int bfin(sgld_ctx *C, int nb, int op) {
-    hipLaunchKernelGGL(k_bfin, dim3(1), dim3(256), 0, C->s, C->batch->part.p, nb, op, C->batch->ctl.p);
+int bfin(sgld_ctx *C, int nb, int op, const double *part = nullptr, const double *part2 = nullptr, int nb2 = 0) {
+    hipLaunchKernelGGL(k_bfin, dim3(1), dim3(256), 0, C->s, part ? part : C->batch->part.p, nb, part2, nb2, op,
+                       C->batch->ctl.p);
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
 }
@@ -612,8 +671,10 @@ int enqueue_bkbyu(sgld_ctx *C, const float *p, float *q, int *nbOut) {
     return SGLD_OK;
 }
 
-// z = M r for all 16 cases; partial z.r per case.
-int enqueue_bvcycle(sgld_ctx *C, const float *r, float *z, int *nbOut) {
+// The batched V-cycle below the finest level (restrictions, coarsest solve,
+// prolongation + smoothing up to level 2); partials of x_2.r_2 per case into
+// partX2 when not null (count in *nbX2).
+int enqueue_bvcycle_coarse(sgld_ctx *C, const float *r, double *partX2, int *nbX2) {
     BatchState *B = C->batch;
     const int L = C->L;
     const float *rf = r;
@@ -633,24 +694,57 @@ int enqueue_bvcycle(sgld_ctx *C, const float *r, float *z, int *nbOut) {
     hipLaunchKernelGGL(k_bcoarse, dim3((int)C->nCoarseDof), dim3(256), 0, C->s, C->Minv.p, C->nCoarseDof,
                        B->lr[L - 1].p, B->lx[L - 1].p);
     SGLD_HIP(hipGetLastError());
-    for (int k = L - 1; k >= 1; --k) {
+    for (int k = L - 1; k >= 2; --k) {
         const Level &F = C->lv[k - 1], &Cc = C->lv[k];
-        const bool finest = k == 1;
-        float *xf = finest ? z : B->lx[k - 1].p;
-        const float *rfk = finest ? r : B->lr[k - 1].p;
         const int nb = bgrid(F.g.nodes() * NR);
-        double *part = finest ? B->part.p : nullptr;
+        double *part = k == 2 ? partX2 : nullptr;
         const NodeDiv df = NodeDiv::make(F.g);
         if (Cc.span == 2)
-            hipLaunchKernelGGL(k_bprolong<2>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, df, rfk,
-                               F.dinv32.p, F.nflag.p, (int)finest, part);
+            hipLaunchKernelGGL(k_bprolong<2>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, B->lx[k - 1].p, F.g, df,
+                               B->lr[k - 1].p, F.dinv32.p, F.nflag.p, 0, part);
         else
-            hipLaunchKernelGGL(k_bprolong<4>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, xf, F.g, df, rfk,
-                               F.dinv32.p, F.nflag.p, (int)finest, part);
+            hipLaunchKernelGGL(k_bprolong<4>, dim3(nb), dim3(256), 0, C->s, B->lx[k].p, Cc.g, B->lx[k - 1].p, F.g, df,
+                               B->lr[k - 1].p, F.dinv32.p, F.nflag.p, 0, part);
         SGLD_HIP(hipGetLastError());
-        if (finest) *nbOut = nb;
+        if (part && nbX2) *nbX2 = nb;
+    }
+    if (L == 2 && partX2) {  // level 2 is the coarsest
+        const int64_t n1 = 3 * C->lv[1].g.nodes() * NR;
+        const int nb = bgrid(n1);
+        hipLaunchKernelGGL(k_bdot, dim3(nb), dim3(256), 0, C->s, B->lx[1].p, B->lr[1].p, n1, partX2);
+        SGLD_HIP(hipGetLastError());
+        if (nbX2) *nbX2 = nb;
     }
     return SGLD_OK;
+}
+
+// The finest level of the V-cycle folded into the direction update (see
+// k_bprolong_dir): p = mask(P x_2 + dinv r) + beta p, xc += alpha p.
+int enqueue_bdir(sgld_ctx *C, const float *r, float *p) {
+    BatchState *B = C->batch;
+    const Level &F = C->lv[0], &Cc = C->lv[1];
+    const int nb = bgrid(F.g.nodes() * NR);
+    const NodeDiv df = NodeDiv::make(F.g);
+    if (Cc.span == 2)
+        hipLaunchKernelGGL(k_bprolong_dir<2>, dim3(nb), dim3(256), 0, C->s, B->lx[1].p, Cc.g, F.g, df, r,
+                           F.dinv32.p, F.nflag.p, B->xc.p, p, B->ctl.p);
+    else
+        hipLaunchKernelGGL(k_bprolong_dir<4>, dim3(nb), dim3(256), 0, C->s, B->lx[1].p, Cc.g, F.g, df, r,
+                           F.dinv32.p, F.nflag.p, B->xc.p, p, B->ctl.p);
+    SGLD_HIP(hipGetLastError());
+    return SGLD_OK;
+}
+
+// rho = z.r per case for a residual set outside the iteration: sum dinv r^2
+// (into part2) plus x_2.r_2 from the coarse V-cycle (into part3).
+int enqueue_brho(sgld_ctx *C, const float *r, int64_t n16) {
+    BatchState *B = C->batch;
+    const int nbv = bgrid(n16);
+    hipLaunchKernelGGL(k_bdinv_rr, dim3(nbv), dim3(256), 0, C->s, r, C->lv[0].dinv32.p, n16, B->part2.p);
+    SGLD_HIP(hipGetLastError());
+    int nbX2 = 0;
+    SGLD_TRY(enqueue_bvcycle_coarse(C, r, B->part3.p, &nbX2));
+    return bfin(C, nbX2, BFIN_RHO, B->part3.p, B->part2.p, nbv);
 }
 
 int upload_ctl(sgld_ctx *C) {
@@ -709,19 +803,18 @@ int solve_batched(sgld_ctx *C, int nrhs, const double *dF, double *dU, double to
                        B->scal.p + NR, mask, 0u);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(upload_ctl(C));
-    float *P[2] = {B->p0.p, B->p1.p};
-    int cur = 0, nb = 0;
-    // z = M r, rho = z.r, p = z
-    SGLD_TRY(enqueue_bvcycle(C, B->r.p, B->z.p, &nb));
-    SGLD_TRY(bfin(C, nb, BFIN_RHO));
-    hipLaunchKernelGGL(k_bupdate_p, dim3(nbv), dim3(256), 0, C->s, P[1], B->z.p, P[0], n16, B->ctl.p);
-    SGLD_HIP(hipGetLastError());
-    cur = 1;
+    // Rotated iteration (as the single-RHS solver): the finest level of the
+    // previous V-cycle, its deferred xc += alpha p and p = z + beta p run as
+    // one pass; p starts at 0 so that the first direction is z.
+    float *p = B->p0.p;
+    int nb = 0;
+    SGLD_TRY(enqueue_brho(C, B->r.p, n16));  // restart: beta = 0
     while (true) {
-        SGLD_TRY(enqueue_bkbyu(C, P[cur], B->q.p, &nb));
+        SGLD_TRY(enqueue_bdir(C, B->r.p, p));
+        SGLD_TRY(enqueue_bkbyu(C, p, B->q.p, &nb));
         SGLD_TRY(bfin(C, nb, BFIN_ALPHA));
-        hipLaunchKernelGGL(k_bupdate_xr, dim3(nbv), dim3(256), 0, C->s, B->xc.p, B->r.p, P[cur], B->q.p, n16,
-                           B->ctl.p, B->part.p);
+        hipLaunchKernelGGL(k_bupdate_r, dim3(nbv), dim3(256), 0, C->s, B->r.p, (const float *)B->q.p,
+                           C->lv[0].dinv32.p, n16, B->ctl.p, B->part.p, B->part2.p);
         SGLD_HIP(hipGetLastError());
         SGLD_TRY(bfin(C, nbv, BFIN_RR));
         SGLD_TRY(download_ctl(C));
@@ -745,11 +838,13 @@ int solve_batched(sgld_ctx *C, int nrhs, const double *dF, double *dU, double to
             for (int j = 0; j < NR; ++j) sc[j] = scale[j];
             SGLD_HIP(hipMemcpyAsync(B->scal.p + 2 * NR, sc.data(), sizeof(double) * NR, hipMemcpyHostToDevice,
                                     C->s));
-            hipLaunchKernelGGL(k_bfold, dim3(nbv), dim3(256), 0, C->s, x64, B->xc.p, nd, B->scal.p + 2 * NR, flag);
+            hipLaunchKernelGGL(k_bfold, dim3(nbv), dim3(256), 0, C->s, x64, B->xc.p, (const float *)p, nd,
+                               B->scal.p + 2 * NR, flag, B->ctl.p);
             SGLD_HIP(hipGetLastError());
             uint32_t keep = 0, zero = 0;
             for (int j = 0; j < NR; ++j) {
                 if (!((flag >> j) & 1u)) continue;
+                h.alpha[j] = 0.0;  // the deferred alpha p is in x64 now
                 double rr = 0, bb = 0;
                 SGLD_TRY(residual64(C, b64 + j * nd, x64 + j * nd, r64 + j * nd, &rr, &bb));
                 ++updates;
@@ -773,19 +868,21 @@ int solve_batched(sgld_ctx *C, int nrhs, const double *dF, double *dU, double to
             SGLD_HIP(hipMemcpyAsync(B->scal.p, inv.data(), sizeof(double) * NR, hipMemcpyHostToDevice, C->s));
             SGLD_HIP(hipMemcpyAsync(B->scal.p + NR, fct.data(), sizeof(double) * NR, hipMemcpyHostToDevice,
                                     C->s));
-            hipLaunchKernelGGL(k_breplace, dim3(nbv), dim3(256), 0, C->s, B->r.p, P[cur], r64, nd, B->scal.p,
+            hipLaunchKernelGGL(k_breplace, dim3(nbv), dim3(256), 0, C->s, B->r.p, p, r64, nd, B->scal.p,
                                B->scal.p + NR, keep, zero);
             SGLD_HIP(hipGetLastError());
             SGLD_TRY(upload_ctl(C));
+            // the replaced residuals' share of rho (all cases: one pass)
+            hipLaunchKernelGGL(k_bdinv_rr, dim3(nbv), dim3(256), 0, C->s, (const float *)B->r.p,
+                               C->lv[0].dinv32.p, n16, B->part2.p);
+            SGLD_HIP(hipGetLastError());
         }
         bool any = false;
         for (int j = 0; j < NR; ++j) any |= !done[j];
         if (!any) break;
-        SGLD_TRY(enqueue_bvcycle(C, B->r.p, B->z.p, &nb));
-        SGLD_TRY(bfin(C, nb, BFIN_RHO));
-        hipLaunchKernelGGL(k_bupdate_p, dim3(nbv), dim3(256), 0, C->s, P[1 - cur], B->z.p, P[cur], n16, B->ctl.p);
-        SGLD_HIP(hipGetLastError());
-        cur = 1 - cur;
+        int nbX2 = 0;
+        SGLD_TRY(enqueue_bvcycle_coarse(C, B->r.p, B->part3.p, &nbX2));
+        SGLD_TRY(bfin(C, nbX2, BFIN_RHO, B->part3.p, B->part2.p, nbv));
     }
     for (int j = 0; j < nrhs; ++j) SGLD_TRY(from_lattice(C, x64 + j * nd, dU + j * nc));
     SGLD_HIP(hipStreamSynchronize(C->s));
