@@ -192,3 +192,46 @@ def test_kbyu_fp32_against_fp64():
     m.set_modulus(E)
     ref = m.kbyu(u)
     assert np.abs(a - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_two_level_pcg_matches_oracle():
+    """Two levels: no small-level tail, and x_2.r_2 of the next rho comes from
+    a separate pass over the coarsest level."""
+    vox = holes(16, 8, 8, 3, nholes=2, r=2)
+    g, m, F, fixed, E = build_pair(vox, 2)
+    U, its, rr = g.pcg(F, tol=1e-8, maxIT=500)
+    Uo, itso, _, rc = m.pcg(F, 0.6, 1e-8, 500)
+    assert rc == 0 and g.converged and rr <= 1e-8
+    assert abs(its - itso) <= 1
+    assert abs(F @ U - F @ Uo) <= 1e-9 * abs(F @ Uo)
+    U32, its32, rr32 = g.pcg(F, tol=1e-8, maxIT=1000, precision=solver.PREC_MIXED)
+    assert g.converged and rr32 <= 1e-8
+    assert abs(F @ U32 - F @ Uo) <= 1e-8 * abs(F @ Uo)
+
+
+@pytest.mark.parametrize("levels", [4, 5])
+def test_vcycle_tail_equals_level_chain(levels, monkeypatch):
+    """The fused small-level tail (composite span-2^m restrictions and one
+    prolongation pass) equals the level-by-level chain and the oracle."""
+    vox = holes(64, 32, 32, 11, nholes=4, r=4)
+    F, fixed = problems.cantilever3(vox)
+    x = problems.gaussian_random_density(vox.shape, 4, sigma=2.0).ravel(order="F")
+    E = problems.simp_modulus(x[vox.ravel(order="F") != 0])
+    zs = []
+    for tail in ("1", "0"):
+        monkeypatch.setenv("SGLD_TAIL", tail)
+        g = solver.FEASolver(vox, levels)
+        g.set_fixed(fixed)
+        g.set_modulus(E)
+        g.assemble(0.6)
+        r = np.random.default_rng(21).standard_normal(g.numDOFs)
+        r[fixed == 1] = 0
+        zs.append(g.vcycle(r))
+    m = orc.Mesh(vox, levels)
+    m.set_fixed(fixed)
+    m.set_modulus(E)
+    m.assemble()
+    ref = m.vcycle(r, 0.6)
+    scale = np.abs(ref).max()
+    assert np.abs(zs[0] - zs[1]).max() <= 1e-12 * scale
+    assert np.abs(zs[0] - ref).max() <= 1e-11 * scale
