@@ -472,11 +472,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
 template <class T, int S>
 __global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid gf,
                                                   T *__restrict__ rc, Grid gc, NodeDiv dc,
-                                                  const uint8_t *__restrict__ cflag,
+                                                  const uint8_t *__restrict__ cflag, int skipVoid,
                                                   const Ctl *ctl) {
     if (stopped(ctl)) return;
     const int64_t G = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (G >= gc.nodes()) return;
+    if (skipVoid && !(cflag[G] & kSolid)) {  // irregular domains: void coarse nodes read nothing
+        rc[3 * G] = rc[3 * G + 1] = rc[3 * G + 2] = T(0);
+        return;
+    }
     int R, Cc, Lc;
     dc.rcl((uint32_t)G, R, Cc, Lc);
     const int64_t NYf = gf.NY(), NXf = gf.NX(), NZf = gf.NZ();
@@ -530,11 +534,12 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
                                                          T *__restrict__ rc, Grid gc,
                                                          const uint8_t *__restrict__ cflag,
                                                          int czc, int tilesR, int tilesC,
+                                                         const int32_t *__restrict__ tiles,
                                                          const Ctl *ctl) {
     if (stopped(ctl)) return;
     __shared__ T sh[2][kRFN * 3];
     const int t = threadIdx.x;
-    int bid = blockIdx.x;
+    int bid = tiles ? tiles[blockIdx.x] : (int)blockIdx.x;  // listed tiles: those with a solid coarse node
     const int tr = bid % tilesR;
     bid /= tilesR;
     const int tc = bid % tilesC;
@@ -1591,21 +1596,29 @@ template <class T>
 int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int nb = blocks_for(Cc.g.nodes(), 256);
-    // by size: the tiled kernel marches 2 czc + 1 fine layers per block, which
-    // only pays off when there are enough coarse nodes to fill the device
-    const bool tiled = C->restrictTiled < 0 ? Cc.g.nodes() >= C->restrictTiledMin : C->restrictTiled != 0;
+    // by size and precision: the tiled kernel marches 2 czc + 1 fine layers per
+    // block, which pays off only in FP64 and with enough coarse nodes to fill
+    // the device (256x128x128 first level, FP64 / FP32: tiled 31 / 17 us,
+    // node-centric 35 / 13 us)
+    const bool tiled = C->restrictTiled < 0 ? (sizeof(T) == 8 && Cc.g.nodes() >= C->restrictTiledMin)
+                                            : C->restrictTiled != 0;
     const NodeDiv dc = NodeDiv::make(Cc.g);
     if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
         const int czc = C->restrictCz, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
-        hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(tilesR * tilesC * tilesZ), dim3(256), 0, C->s, rf,
-                           F.g, rc, Cc.g, Cc.nflag.p, czc, tilesR, tilesC, ctl);
+        // irregular domains: only the tiles holding a solid coarse node (the
+        // others stay zero from set-up)
+        const bool list = !C->full && Cc.rtilesCz == czc;
+        const int nbt = list ? (int)Cc.nRtiles : tilesR * tilesC * tilesZ;
+        if (nbt > 0)
+            hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(nbt), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, Cc.nflag.p,
+                               czc, tilesR, tilesC, list ? (const int32_t *)Cc.rtiles.p : nullptr, ctl);
     } else if (Cc.span == 2)
         hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
-                           Cc.nflag.p, ctl);
+                           Cc.nflag.p, (int)!C->full, ctl);
     else
         hipLaunchKernelGGL((k_restrict<T, 4>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
-                           Cc.nflag.p, ctl);
+                           Cc.nflag.p, (int)!C->full, ctl);
     SGLD_HIP(hipGetLastError());
     return SGLD_OK;
 }
@@ -2216,6 +2229,28 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         if (k > 0) {
             SGLD_TRY(l.r.alloc(n)); SGLD_TRY(l.x.alloc(n));
             SGLD_TRY(l.r32.alloc(n)); SGLD_TRY(l.x32.alloc(n));
+            // void entries are never written on irregular domains: start at 0
+            SGLD_TRY(l.r.zero(C->s)); SGLD_TRY(l.r32.zero(C->s));
+            SGLD_TRY(l.x.zero(C->s)); SGLD_TRY(l.x32.zero(C->s));
+            if (!C->full && l.span == 2) {
+                // tiles of the tiled restriction that hold a solid coarse node
+                const int czc = C->restrictCz;
+                const int tilesR = (int)((l.g.NY() + kRR - 1) / kRR), tilesC = (int)((l.g.NX() + kRC - 1) / kRC);
+                const int tilesZ = (int)((l.g.NZ() + czc - 1) / czc);
+                std::vector<uint8_t> act((size_t)tilesR * tilesC * tilesZ, 0);
+                const int64_t NYn = l.g.NY(), NXn = l.g.NX();
+                for (int64_t g = 0; g < l.g.nodes(); ++g) {
+                    if (!(l.hflag[g] & kSolid)) continue;
+                    const int64_t r = g % NYn, c = (g / NYn) % NXn, z = g / (NYn * NXn);
+                    act[(r / kRR) + (int64_t)tilesR * ((c / kRC) + (int64_t)tilesC * (z / czc))] = 1;
+                }
+                std::vector<int32_t> list;
+                for (size_t i = 0; i < act.size(); ++i)
+                    if (act[i]) list.push_back((int32_t)i);
+                l.nRtiles = (int64_t)list.size();
+                SGLD_TRY(l.rtiles.upload(list.data(), list.size()));
+                l.rtilesCz = czc;
+            }
         }
     }
     SGLD_TRY(C->E.alloc(f0.g.elems()));

@@ -47,6 +47,9 @@ struct Level {
     DevBuf<double> dinv, r, x;
     DevBuf<float> dinv32, r32, x32;
     DevBuf<uint16_t> dinv16;  // finest level: bfloat16 Jacobi weights of the FP32 inner solve
+    DevBuf<int32_t> rtiles;   // irregular domains: tiles of the tiled restriction into this level
+    int64_t nRtiles = 0;
+    int rtilesCz = 0;         // z-chunk the tile list was built for (0: none)
 };
 
 struct BatchState;             // batched.hip
@@ -80,7 +83,7 @@ struct sgld_ctx {
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
-    int restrictCz = 4;  // coarse layers per block of the tiled span-2 restriction
+    int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
     bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
