@@ -1628,7 +1628,11 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
                     const Ctl *ctl, int *nbOut) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int finest = (k == 1);
-    const Span S = finest ? dof_span(C) : Span::all(3 * F.g.nodes());
+    // irregular domains: only the level's 64-entry DOF chunks holding a solid
+    // node (the rest stays zero from set-up)
+    const Span S = finest ? dof_span(C)
+                          : (C->full || !F.dofChunks.p ? Span::all(3 * F.g.nodes())
+                                                       : Span{F.dofChunks.p, 3 * F.g.nodes(), 64 * F.nDofChunks});
     const int nb = grid_blocks(S.count);
     const NodeDiv df = NodeDiv::make(F.g);
     if (Cc.span == 2)
@@ -2250,6 +2254,18 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
                 l.nRtiles = (int64_t)list.size();
                 SGLD_TRY(l.rtiles.upload(list.data(), list.size()));
                 l.rtilesCz = czc;
+            }
+            if (!C->full) {
+                // 64-entry chunks of this level's DOF vectors with a solid node
+                std::vector<int32_t> ch;
+                for (int64_t c = 0; 64 * c < n; ++c) {
+                    bool any = false;
+                    for (int64_t g = 64 * c / 3; g <= std::min(64 * c + 63, n - 1) / 3 && !any; ++g)
+                        any = l.hflag[g] & kSolid;
+                    if (any) ch.push_back((int32_t)c);
+                }
+                l.nDofChunks = (int64_t)ch.size();
+                SGLD_TRY(l.dofChunks.upload(ch.data(), ch.size()));
             }
         }
     }

@@ -50,6 +50,8 @@ struct Level {
     DevBuf<int32_t> rtiles;   // irregular domains: tiles of the tiled restriction into this level
     int64_t nRtiles = 0;
     int rtilesCz = 0;         // z-chunk the tile list was built for (0: none)
+    DevBuf<int32_t> dofChunks;  // irregular domains, levels >= 2: 64-entry DOF chunks with a solid node
+    int64_t nDofChunks = 0;
 };
 
 struct BatchState;             // batched.hip
