@@ -2307,7 +2307,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             TiledSchedule &ts = C->ksched[q];
             ts.tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
             ts.tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
-            ts.cap = (int)std::min<int64_t>((int64_t)ncu * std::max(1, per[q]), kRedBlocks);
+            ts.cap = (int)std::min<int64_t>((int64_t)ncu * C->kOcc[q], kRedBlocks);
             // z-chunks of 8 node layers: measured best for both precisions on 256x128x128
             SGLD_TRY(set_kchunk(C.get(), q, 8));
         }
