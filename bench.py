@@ -38,7 +38,10 @@ def parse():
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kbyu-reps", type=int, default=50)
+    ap.add_argument("--kbyu-reps", type=int, default=100, help="timed back-to-back K.u launches")
+    ap.add_argument("--kbyu-warmup", type=int, default=300,
+                    help="untimed K.u launches first: after idle time the FP64 kernel runs a ~30 ms "
+                         "clock transient (10-launch windows 80 -> 92 -> 72 us)")
     ap.add_argument("--kbyu-only", action="store_true",
                     help="only K.u launches plus a 256 MiB torch copy (for rocprofv3 --pmc runs)")
     return ap.parse_args()
@@ -59,11 +62,19 @@ def measured_traffic():
     return None
 
 
+def time_kbyu(a, s, prec):
+    """ms per launch of `a.kbyu_reps` back-to-back K.u launches (HIP events on
+    the solver stream) after `a.kbyu_warmup` untimed ones."""
+    if a.kbyu_warmup > 0:
+        s.bench_kbyu(prec, a.kbyu_warmup)
+    return s.bench_kbyu(prec, a.kbyu_reps)
+
+
 def kbyu_only(a, s, n, ne):
     import torch
     from sgldbench_amd import solver
-    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
-    ms32 = s.bench_kbyu(solver.PREC_MIXED, a.kbyu_reps)
+    ms64 = time_kbyu(a, s, solver.PREC_FP64)
+    ms32 = time_kbyu(a, s, solver.PREC_MIXED)
     src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")  # 256 MiB
     dst = torch.empty_like(src)
     for _ in range(3):
@@ -266,8 +277,10 @@ def run_batched(a):
     c64 = np.einsum("ij,ij->i", F, U["fp64_sequential"])
     cmx = np.einsum("ij,ij->i", F, U["batched_mixed"])
     urel = np.linalg.norm(U["batched_mixed"] - U["fp64_sequential"], axis=1) / np.linalg.norm(U["fp64_sequential"], axis=1)
+    if a.kbyu_warmup > 0:  # each batched launch is ~6 ms: a tenth of the launches covers the transient
+        s.bench_kbyu_batched(max(1, a.kbyu_warmup // 10))
     ms_b = s.bench_kbyu_batched(a.kbyu_reps)
-    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
+    ms64 = time_kbyu(a, s, solver.PREC_FP64)
     useful = 1152.0 * k * ne                       # flops of 16 dense 24x24 element products
     executed = 6 * 2 * 32 * 32 * 16 * (ne / 2.0) * 1.0  # 6 MFMA 32x32x16 per element pair (hi/lo, padding)
     bytes_b = 2 * n * k * 4 + ne * 4
@@ -381,8 +394,8 @@ def main():
     U64 = dU.cpu().numpy()
     comp_fp64 = float(p["F"] @ U64)
     # K.u roofline (FP64 and FP32), hipEvent-timed on the solver stream
-    ms64 = s.bench_kbyu(solver.PREC_FP64, a.kbyu_reps)
-    ms32 = s.bench_kbyu(solver.PREC_MIXED, a.kbyu_reps)
+    ms64 = time_kbyu(a, s, solver.PREC_FP64)
+    ms32 = time_kbyu(a, s, solver.PREC_MIXED)
     bytes64 = 2 * n * 8 + ne * 8
     bytes32 = 2 * n * 4 + ne * 4
     # density filter of the same mesh (rMin 2.5: 5^3 cone), kernel time

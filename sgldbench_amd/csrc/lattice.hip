@@ -228,7 +228,8 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
 // Persistent, XCD-aware grid: block b runs on XCD b % 8 and walks the work
 // items (patch x z-chunk) of that XCD's contiguous share, so patches sharing
 // halo nodes run behind the same L2.  Node layers are prefetched two
-// iterations ahead into registers and staged in a 3-slot LDS ring.  On an
+// iterations ahead into registers and staged in an LDS ring (4 slots with
+// one block barrier per element layer, 3 slots with two).  On an
 // irregular domain `wlist` holds the (ascending) work items whose output
 // nodes touch a solid element; the others would only write zeros over void
 // nodes that are zero already.
@@ -275,7 +276,12 @@ __device__ __forceinline__ void face_modes(const T *__restrict__ lay, int ei, in
     }
 }
 
-template <class T>
+// OneBar selects one block barrier per element layer instead of two: a
+// 4-slot node-layer ring and 3 accumulators put every write-after-read pair
+// (staging layer el+2 over layer el-2, clearing the accumulator of layer el+1
+// last read for layer el-2) one step apart, so the barrier of the step in
+// between orders them.
+template <class T, bool OneBar>
 __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
@@ -283,10 +289,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                                                        int64_t wn, double *__restrict__ partial,
                                                        const Ctl *ctl) {
     if (stopped(ctl)) return;
-    __shared__ T us[3][kTN * kTN * 3];
-    // node forces of two layers, FP64 in both precisions: native ds_add_f32
-    // made the FP32 kernel 6.7x slower than ds_add_f64 (337 vs 50 us, cfg 1)
-    __shared__ double acc[2][kTN * kTN * 3];
+    constexpr int kSlots = OneBar ? 4 : 3, kAcc = OneBar ? 3 : 2;
+    __shared__ T us[kSlots][kTN * kTN * 3];
+    // node forces of two (three) layers, FP64 in both precisions: native
+    // ds_add_f32 made the FP32 kernel 6.7x slower than ds_add_f64 (337 vs 50 us, cfg 1)
+    __shared__ double acc[kAcc][kTN * kTN * 3];
     const int t = threadIdx.x;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
     const int64_t pl = (int64_t)NY * NX;       // nodes per layer
@@ -298,7 +305,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
     const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
     const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
     const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
-    for (int k = t; k < 2 * kTN * kTN * 3; k += 256) (&acc[0][0])[k] = 0.0;
+    for (int k = t; k < kAcc * kTN * kTN * 3; k += 256) (&acc[0][0])[k] = 0.0;
     // LDS offsets (in T) of this face's corners
     const int f00 = 3 * ((ei + 1) + kTN * ej), f10 = 3 * ((ei + 1) + kTN * (ej + 1));
     const int f01 = 3 * (ei + kTN * ej), f11 = 3 * (ei + kTN * (ej + 1));
@@ -347,8 +354,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             for (int k = 0; k < kStageSlots; ++k)
                 if (k < kStageSlots - 1 || t + 256 * k < 3 * kTN * kTN) dst[t + 256 * k] = q[k];
         };
-        T *sPrev = us[0], *sCur = us[1], *sNext = us[2];  // layers el, el+1, el+2
-        double *aCur = acc[0], *aNext = acc[1];
+        // layers el, el+1, el+2 (and el-1 with OneBar)
+        T *sPrev = us[0], *sCur = us[1], *sNext = us[2], *sFree = us[kSlots - 1];
+        double *aCur = acc[0], *aNext = acc[1], *aFree = acc[kAcc - 1];
         {   // stage node layers Z0-1 and Z0; layer Z0+1 stays in flight in pf
             T p0[kStageSlots], p1[kStageSlots];
             loadL(uRsrc(Z0 - 1), p0);
@@ -422,6 +430,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
             }
             Ecur = Enext;
             eokCur = eok;
+            if constexpr (OneBar) {
+                // layer el+2 into the slot of layer el-2 and a clear accumulator
+                // for layer el+1 (both last read before the previous barrier)
+                stage(sNext, ready);
+                for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
+            }
             __syncthreads();  // node layer el accumulated
             if (out) {
                 const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * el, vBytes);
@@ -436,12 +450,23 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                     dotacc += (double)sPrev[j] * (double)v;
                 }
             }
-            // clear the other accumulator (read during the previous step,
-            // scattered into during the next); halo nodes get discarded sums
-            for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
-            // layer el+2 into the slot of layer el-1 (free since the barrier above)
-            stage(sNext, ready);
-            {
+            if constexpr (OneBar) {
+                T *tmp = sPrev;
+                sPrev = sCur;
+                sCur = sNext;
+                sNext = sFree;
+                sFree = tmp;
+                double *ta = aCur;
+                aCur = aNext;
+                aNext = aFree;
+                aFree = ta;
+                pE += ple;
+            } else {
+                // clear the other accumulator (read during the previous step,
+                // scattered into during the next); halo nodes get discarded sums
+                for (int k = t; k < kTN * kTN * 3; k += 256) aNext[k] = 0.0;
+                // layer el+2 into the slot of layer el-1 (free since the barrier above)
+                stage(sNext, ready);
                 T *tmp = sPrev;
                 sPrev = sCur;
                 sCur = sNext;
@@ -449,9 +474,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                 double *ta = aCur;
                 aCur = aNext;
                 aNext = ta;
+                pE += ple;
+                __syncthreads();  // layer el+2 staged, accumulator clear
             }
-            pE += ple;
-            __syncthreads();  // layer el+2 staged, accumulator clear
         };
         T pn[kStageSlots];
         for (int el = Z0 - 1; el < Zend; el += 2) {
@@ -1516,9 +1541,14 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
         const int q = sizeof(T) == 8 ? 0 : 1;
         const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
-        hipLaunchKernelGGL(k_kbyu_tiled<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
-                           l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
-                           C->nKwork[q], partial, ctl);
+        if (C->kbyuOneBar)
+            hipLaunchKernelGGL((k_kbyu_tiled<T, true>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
+                               l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
+                               C->nKwork[q], partial, ctl);
+        else
+            hipLaunchKernelGGL((k_kbyu_tiled<T, false>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
+                               l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
+                               C->nKwork[q], partial, ctl);
     } else {
         const Span S = node_span(C);
         nb = grid_blocks(S.count);
@@ -2295,8 +2325,16 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         SGLD_HIP(hipGetDevice(&dev));
         SGLD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         int per[2] = {0, 0};
-        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double>, 256, 0));
-        SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float>, 256, 0));
+        // one barrier per element layer: measured 71-74 vs 75 us (FP64) and 52 vs
+        // 54 us (FP32) on 256x128x128; SGLD_KU_BARRIERS=2 selects the two-barrier step
+        if (const char *env = std::getenv("SGLD_KU_BARRIERS")) C->kbyuOneBar = std::atoi(env) != 2;
+        if (C->kbyuOneBar) {
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double, true>, 256, 0));
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float, true>, 256, 0));
+        } else {
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double, false>, 256, 0));
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float, false>, 256, 0));
+        }
         C->numCU = ncu;
         // Blocks per compute unit of the persistent grid.  A sweep of 2..6 on the
         // 256x128x128 cuboid measured 3 (FP64: 168 VGPRs, 3 waves per SIMD) and

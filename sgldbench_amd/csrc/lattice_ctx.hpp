@@ -82,6 +82,7 @@ struct sgld_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
+    bool kbyuOneBar = true;  // tiled K.u with one block barrier per element layer (else two)
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail

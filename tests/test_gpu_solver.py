@@ -141,16 +141,18 @@ def test_cfg0_fp64_against_oracle():
     assert abs(p["F"] @ U - p["F"] @ Uo) <= 1e-9 * abs(p["F"] @ Uo)
 
 
-@pytest.mark.parametrize("variant", ["dense", "tiled"])
+@pytest.mark.parametrize("variant", ["dense", "tiled", "tiled2"])
 @pytest.mark.parametrize("dims", [(38, 22, 18), (16, 8, 8)])
 def test_kbyu_kernel_variants(variant, dims, monkeypatch):
-    """Both K.u kernels (node-centric dense Ke; tiled mode-space) against the
-    oracle, on lattices that are not multiples of the 15-node tile, with voids
-    and moduli spanning 1e-9..1."""
+    """The K.u kernels (node-centric dense Ke; tiled mode-space with one or
+    two block barriers per element layer) against the oracle, on lattices
+    that are not multiples of the 15-node tile, with voids and moduli
+    spanning 1e-9..1."""
     if variant == "dense":
         monkeypatch.setenv("SGLD_KBYU", "dense")
     else:
         monkeypatch.delenv("SGLD_KBYU", raising=False)
+    monkeypatch.setenv("SGLD_KU_BARRIERS", "2" if variant == "tiled2" else "1")
     nx, ny, nz = dims
     vox = holes(nx, ny, nz, 11, nholes=4, r=3)
     F, fixed = problems.cantilever3(vox)
