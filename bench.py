@@ -84,12 +84,25 @@ def kbyu_only(a, s, n, ne):
                       "copy_bytes": src.numel() * 8, "dofs": n, "elements": ne}), flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
     """Oracle (C/OpenMP restatement of the reference algorithm) on the host:
-    `sample_its` FP64 PCG iterations of the same workload, extrapolated to the
-    GPU FP64 iteration count."""
+    `sample_its` FP64 PCG iterations of the same workload with OMP_NUM_THREADS
+    threads bound close, extrapolated to the GPU FP64 iteration count; one
+    more iteration on one thread beside it."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime starts
     from oracle import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.set_threads(threads)
     t0 = time.time()
     m = orc.Mesh(p["voxels"], p["num_levels"])
     m.set_fixed(p["fixed"])
@@ -98,6 +111,9 @@ def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
     t_setup = time.time() - t0
     t_it, t_k, t_v = m.time_pcg_iterations(p["F"], p["omega"], sample_its)
     per_it = t_it / sample_its
+    orc.set_threads(1)
+    t1, t1k, t1v = m.time_pcg_iterations(p["F"], p["omega"], 1)
+    orc.set_threads(threads)
     n = p["F"].size
     t_solve = per_it * its_fp64
     return {"value": n / t_solve / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
@@ -105,7 +121,9 @@ def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
                       "(%.2f s/iteration: K.u %.2f s, V-cycle %.2f s; setup %.1f s), "
                       "extrapolated to %d iterations" % (sample_its, per_it, t_k / sample_its,
                                                          t_v / sample_its, t_setup, its_fp64),
-            "solve_s_est": t_solve}
+            "solve_s_est": t_solve, "cpu": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "one_thread": {"s_per_iteration": t1, "kbyu_s": t1k, "vcycle_s": t1v,
+                           "solve_s_est": t1 * its_fp64}}
 
 
 def progress(msg):

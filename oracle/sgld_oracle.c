@@ -857,6 +857,12 @@ void or_mesh_unit_compliance(const or_mesh *M, const double *U, double *ce) {
     or_unit_compliance_mex(U, L->eNodMat, L->numElements, M->Ke, ce);
 }
 
+int or_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
+
 double or_mesh_time_pcg_iterations(const or_mesh *M, double omega, const double *b,
                                    int nits, double *tK, double *tV) {
     const int64_t n = M->L[0].numDOFs;

@@ -129,6 +129,10 @@ int or_mesh_pde_filter(const or_mesh *m, double rHatMin, const double *src, doub
 double or_mesh_time_pcg_iterations(const or_mesh *m, double omega, const double *b,
                                    int its, double *kbyuSeconds, double *vcycleSeconds);
 
+/* OpenMP team size of the following calls (timing at one thread and at all
+ * cores); returns the previous maximum. */
+int or_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -265,6 +265,26 @@ def test_pcg_converges_to_spsolve(dims, levels):
     assert c == pytest.approx(u @ (K @ u), rel=1e-9)  # cancellation in u_e^T Ke u_e
 
 
+def test_pcg_on_one_thread_matches_team():
+    """The CPU baseline times the oracle at one thread and at all cores: the
+    same iteration count and solution to rounding either way."""
+    vox = problems.cuboid(16, 8, 8)
+    m = orc.Mesh(vox, 3)
+    F, fixed = problems.cantilever3(vox)
+    m.set_modulus(problems.simp_modulus(problems.random_density_cuboid(16, 8, 8, seed=12)))
+    m.set_fixed(fixed)
+    m.assemble()
+    prev = orc.set_threads(4)
+    try:
+        u4, its4, _, rc4 = m.pcg(F, 0.6, 1e-8, 500)
+        assert orc.set_threads(1) == 4
+        u1, its1, _, rc1 = m.pcg(F, 0.6, 1e-8, 500)
+    finally:
+        orc.set_threads(prev)
+    assert rc1 == rc4 == 0 and abs(its1 - its4) <= 1
+    assert np.linalg.norm(u1 - u4) <= 1e-8 * np.linalg.norm(u4)
+
+
 def test_mex_level_functions_agree_with_mesh_methods():
     vox = random_voxels((8, 8, 8), 0.7, 12)
     m = orc.Mesh(vox, 3)
