@@ -237,3 +237,35 @@ def test_vcycle_tail_equals_level_chain(levels, monkeypatch):
     scale = np.abs(ref).max()
     assert np.abs(zs[0] - zs[1]).max() <= 1e-12 * scale
     assert np.abs(zs[0] - ref).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("cz", ["1", "2", "3"])
+@pytest.mark.parametrize("case", ["cuboid", "holes"])
+def test_tiled_restriction_matches_node_centric(case, cz, monkeypatch):
+    """The LDS-tiled span-2 restriction (partial row, column and z tiles, any
+    z-chunk, the tile list of an irregular domain) gives the V-cycle of the
+    node-centric restriction and of the oracle."""
+    monkeypatch.setenv("SGLD_RESTRICT_CZ", cz)
+    vox = problems.cuboid(40, 72, 20) if case == "cuboid" else holes(40, 72, 20, 5, nholes=5, r=4)
+    F, fixed = problems.cantilever3(vox)
+    x = problems.gaussian_random_density(vox.shape, 6, sigma=2.0).ravel(order="F")
+    E = problems.simp_modulus(x[vox.ravel(order="F") != 0])
+    print("case: tiled restriction", case, "cz", cz, flush=True)
+    zs = []
+    for restr in ("tiled", "node-centric"):
+        g = solver.FEASolver(vox, 3)
+        g.set_fixed(fixed)
+        g.set_modulus(E)
+        g.assemble(0.6)
+        g.set_schedule("tiled", 8, restr)
+        r = np.random.default_rng(31).standard_normal(g.numDOFs)
+        r[fixed == 1] = 0
+        zs.append(g.vcycle(r))
+    m = orc.Mesh(vox, 3)
+    m.set_fixed(fixed)
+    m.set_modulus(E)
+    m.assemble()
+    ref = m.vcycle(r, 0.6)
+    scale = np.abs(ref).max()
+    assert np.abs(zs[0] - zs[1]).max() <= 1e-12 * scale
+    assert np.abs(zs[0] - ref).max() <= 1e-11 * scale
