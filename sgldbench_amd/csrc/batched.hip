@@ -431,6 +431,99 @@ __global__ void __launch_bounds__(256) k_brestrict(const float *__restrict__ rf,
     }
 }
 
+// Span-2 batched restriction, LDS-tiled and marching up the layers.  A block
+// owns kBRR x kBRC coarse node columns (all NR cases, one thread per node and
+// case) and walks their coarse layers; for coarse layer L it stages fine
+// layers 2L and 2L+1 of the (2 kBRR + 1) x (2 kBRC + 1) fine window (whole
+// 192-byte nodes as float4, coalesced along the fine rows), forms each coarse
+// node's in-plane 3 x 3 sums (weights 1/2, 1, 1/2 per axis) from LDS, and
+// carries the half of fine layer 2L+1 that belongs to layer L+1 in registers.
+// Each fine value is read from HBM once per block (plus the window's rim),
+// where the node-centric gather re-reads the fine layer of each z-neighbour.
+constexpr int kBRR = 8, kBRC = 2;                       // coarse rows, cols per block
+constexpr int kBFR = 2 * kBRR + 1, kBFC = 2 * kBRC + 1;  // fine window rows, cols
+constexpr int kBFN = kBFR * kBFC;                        // fine nodes per staged layer
+constexpr int kBNode4 = 3 * NR / 4;                      // float4 per node (12)
+constexpr int kBPad4 = kBNode4 + 1;                      // LDS node stride in float4 (fewer bank conflicts)
+constexpr int kBStage = (kBFN * kBNode4 + 255) / 256;    // float4 staged per thread and layer
+
+__global__ void __launch_bounds__(256) k_brestrict2_tiled(const float *__restrict__ rf, Grid gf,
+                                                          float *__restrict__ rc, Grid gc,
+                                                          const uint8_t *__restrict__ cflag, int tilesR) {
+    __shared__ float4 sh[2][kBFN * kBPad4];
+    const int t = threadIdx.x, j = t & (NR - 1), cr = (t / NR) % kBRR, cc = t / (NR * kBRR);
+    const int tr = blockIdx.x % tilesR, tc = blockIdx.x / tilesR;
+    const int NYf = gf.ny + 1, NXf = gf.nx + 1, NZf = gf.nz + 1;
+    const int NYc = gc.ny + 1, NXc = gc.nx + 1, NZc = gc.nz + 1;
+    const int64_t plf = (int64_t)NYf * NXf, plc = (int64_t)NYc * NXc;
+    const int R0 = tr * kBRR, C0 = tc * kBRC, R = R0 + cr, Cn = C0 + cc;
+    const bool mine = R < NYc && Cn < NXc;
+    // staged float4 q = t + 256 i: window node k = q / 12 (row-major in the
+    // fine column), part q % 12; offset of the node within a fine layer (or -1)
+    int64_t goff[kBStage];
+    int part[kBStage], lds[kBStage];
+#pragma unroll
+    for (int i = 0; i < kBStage; ++i) {
+        const int q = t + 256 * i, k = q / kBNode4;
+        part[i] = q - k * kBNode4;
+        lds[i] = k * kBPad4 + part[i];
+        goff[i] = -1;
+        if (k < kBFN) {
+            const int rfw = 2 * R0 - 1 + k % kBFR, cfw = 2 * C0 - 1 + k / kBFR;
+            if (rfw >= 0 && rfw < NYf && cfw >= 0 && cfw < NXf) goff[i] = rfw + (int64_t)NYf * cfw;
+        } else {
+            lds[i] = -1;
+        }
+    }
+    const float4 *rf4 = reinterpret_cast<const float4 *>(rf);
+    const float *shf0 = reinterpret_cast<const float *>(sh[0]), *shf1 = reinterpret_cast<const float *>(sh[1]);
+    float carry[3] = {0.f, 0.f, 0.f};  // half of fine layer 2L-1
+    for (int L = 0; L < NZc; ++L) {
+        float4 v0[kBStage], v1[kBStage];
+#pragma unroll
+        for (int i = 0; i < kBStage; ++i) {
+            const bool ok0 = goff[i] >= 0 && 2 * L < NZf, ok1 = goff[i] >= 0 && 2 * L + 1 < NZf;
+            const int64_t o0 = ok0 ? (goff[i] + plf * (2 * L)) * kBNode4 + part[i] : 0;
+            const int64_t o1 = ok1 ? (goff[i] + plf * (2 * L + 1)) * kBNode4 + part[i] : 0;
+            v0[i] = rf4[o0];
+            v1[i] = rf4[o1];
+            if (!ok0) v0[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!ok1) v1[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();  // the previous layer's reads of LDS are done
+#pragma unroll
+        for (int i = 0; i < kBStage; ++i)
+            if (lds[i] >= 0) {
+                sh[0][lds[i]] = v0[i];
+                sh[1][lds[i]] = v1[i];
+            }
+        __syncthreads();
+        float P0[3] = {0.f, 0.f, 0.f}, P1[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dc = -1; dc <= 1; ++dc)
+#pragma unroll
+            for (int dr = -1; dr <= 1; ++dr) {
+                const float w = float((dc == 0 ? 2 : 1) * (dr == 0 ? 2 : 1)) * 0.25f;
+                const int k = (2 * cr + 1 + dr) + kBFR * (2 * cc + 1 + dc);
+                const int o = 4 * k * kBPad4 + j;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    P0[a] += w * shf0[o + a * NR];
+                    P1[a] += w * shf1[o + a * NR];
+                }
+            }
+        if (mine) {
+            const int64_t G = R + (int64_t)NYc * Cn + plc * L;
+            const bool solid = cflag[G] & kSolid;
+            float *po = rc + 3 * G * NR + j;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) po[a * NR] = solid ? carry[a] + P0[a] + 0.5f * P1[a] : 0.0f;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) carry[a] = 0.5f * P1[a];
+    }
+}
+
 // x_f = P x_c + dinv_f r_f; finest: fixed DOFs 0 and partial x_f.r_f per case.
 // Branch-free interpolation: per axis the coarse neighbours q and q + (m > 0)
 // with weights (S - m)/S and m/S, all inside the coarse lattice.
@@ -682,7 +775,11 @@ int enqueue_bvcycle_coarse(sgld_ctx *C, const float *r, double *partX2, int *nbX
         const Level &F = C->lv[k - 1], &Cc = C->lv[k];
         const int nb = bgrid(Cc.g.nodes() * NR);
         const NodeDiv dc = NodeDiv::make(Cc.g);
-        if (Cc.span == 2)
+        if (Cc.span == 2 && C->bRestrictTiled) {
+            const int tilesR = (int)((Cc.g.NY() + kBRR - 1) / kBRR), tilesC = (int)((Cc.g.NX() + kBRC - 1) / kBRC);
+            hipLaunchKernelGGL(k_brestrict2_tiled, dim3(tilesR * tilesC), dim3(256), 0, C->s, rf, F.g, B->lr[k].p,
+                               Cc.g, Cc.nflag.p, tilesR);
+        } else if (Cc.span == 2)
             hipLaunchKernelGGL(k_brestrict<2>, dim3(nb), dim3(256), 0, C->s, rf, F.g, B->lr[k].p, Cc.g, dc,
                                Cc.nflag.p);
         else

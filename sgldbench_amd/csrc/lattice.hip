@@ -2155,6 +2155,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_BRESTRICT_TILED")) C->bRestrictTiled = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
     C->nonDyadic = nonDyadic;
     C->nu = nu;

@@ -88,6 +88,7 @@ struct sgld_ctx {
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
     bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)
+    bool bRestrictTiled = true;  // batched span-2 restriction: LDS-tiled layer march (else node-centric)
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
     // what holds solid material.  dofChunks / nodeChunks list the 64-entry
