@@ -248,9 +248,11 @@ def run_multi(a):
 
 
 def run_batched(a):
-    """16 load cases on an n^3 GRF cuboid: one batched mixed-precision solve
-    (BF16 matrix-core K.u) against 16 sequential FP64 solves of the same
-    cases.  One step = all 16 cases solved to FP64 true rel-res 1e-8."""
+    """16 load cases on an n^3 GRF cuboid through the batched entry point:
+    sequential mixed-precision solves (its default), the interleaved 16-case
+    iteration (BF16 matrix-core K.u) and 16 sequential FP64 solves of the same
+    cases.  One step = all 16 cases solved to FP64 true rel-res 1e-8; the
+    line's value is the faster mixed mode."""
     import torch
     from sgldbench_amd import problems, solver
     n3 = a.n3
@@ -278,7 +280,8 @@ def run_batched(a):
         return time.perf_counter() - t, its, rr
 
     res, U = {}, {}
-    for prec, name in ((solver.PREC_FP64, "fp64_sequential"), (solver.PREC_MIXED, "batched_mixed")):
+    for prec, name in ((solver.PREC_FP64, "fp64_sequential"), (solver.PREC_MIXED, "mixed_sequential"),
+                       (solver.PREC_INTERLEAVED, "interleaved_mixed")):
         for _ in range(a.warmup):
             solve(prec)
         runs = []
@@ -293,40 +296,60 @@ def run_batched(a):
         U[name] = dU.cpu().numpy()
     F = p["F"]
     c64 = np.einsum("ij,ij->i", F, U["fp64_sequential"])
-    cmx = np.einsum("ij,ij->i", F, U["batched_mixed"])
-    urel = np.linalg.norm(U["batched_mixed"] - U["fp64_sequential"], axis=1) / np.linalg.norm(U["fp64_sequential"], axis=1)
+
+    def agreement(name):
+        c = np.einsum("ij,ij->i", F, U[name])
+        urel = np.linalg.norm(U[name] - U["fp64_sequential"], axis=1) / np.linalg.norm(U["fp64_sequential"], axis=1)
+        return {"compliance_rel_diff_max": float(np.max(np.abs(c - c64) / np.abs(c64))),
+                "u_rel_diff_max": float(urel.max())}
+
     if a.kbyu_warmup > 0:  # each batched launch is ~6 ms: a tenth of the launches covers the transient
         s.bench_kbyu_batched(max(1, a.kbyu_warmup // 10))
     ms_b = s.bench_kbyu_batched(a.kbyu_reps)
     ms64 = time_kbyu(a, s, solver.PREC_FP64)
+    ms32 = time_kbyu(a, s, solver.PREC_MIXED)
     useful = 1152.0 * k * ne                       # flops of 16 dense 24x24 element products
     executed = 6 * 2 * 32 * 32 * 16 * (ne / 2.0) * 1.0  # 6 MFMA 32x32x16 per element pair (hi/lo, padding)
     bytes_b = 2 * n * k * 4 + ne * 4
-    t_b = res["batched_mixed"]["s_per_step"]
+    bytes32 = 2 * n * 4 + ne * 4
     t_s = res["fp64_sequential"]["s_per_step"]
+    best = min(("mixed_sequential", "interleaved_mixed"), key=lambda m: res[m]["s_per_step"]
+               if res[m]["converged"] else float("inf"))
+    t_b = res[best]["s_per_step"]
+    if best == "mixed_sequential":  # dominant kernel: the single-case FP32 K.u
+        roof = {"bound": "hbm", "achieved": bytes32 / ms32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": bytes32 / ms32 / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_kbyu_tiled<float>", "bytes_per_launch": bytes32}
+    else:
+        roof = {"bound": "mfma", "achieved": useful / ms_b / 1e9, "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": useful / ms_b / 1e9 / BF16_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_bkbyu", "flops_per_launch": useful,
+                "hbm_frac": bytes_b / ms_b / 1e6 / HBM_PEAK_GBS}
     out = {
         "metric": "MG-PCG time to rel-res 1e-8 & K.u GDOF/s, FP64 vs mixed/batched, 1 MI355X",
         "value": k * n / t_b / 1e9,
         "unit": "GDOF/s (DOF x load cases solved to rel-res 1e-8 per second)",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * t_b, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3+f32+f64", "data": "synthetic (seeded Gaussian random density field, seeded load directions)",
+        "dtype": "f32+f64" if best == "mixed_sequential" else "bf16x3+f32+f64",
+        "data": "synthetic (seeded Gaussian random density field, seeded load directions)",
         "config": {"workload": ("cuboid %d^3 voxels, %d DOF, SIMP E from GRF, x=0 fixed, %d load cases "
                                 "(unit force on the x=%d face, random directions), 6 levels, omega 0.6, tol 1e-8"
-                                % (n3, n, k, n3)), "path": "batched mixed", "dofs": n, "elements": ne,
+                                % (n3, n, k, n3)),
+                   "path": "batched entry point, " + best.replace("_", " "), "dofs": n, "elements": ne,
                    "cases": k},
-        "batched": {"batched_mixed": res["batched_mixed"], "fp64_sequential": res["fp64_sequential"],
+        "batched": {"mixed_sequential": dict(res["mixed_sequential"], **agreement("mixed_sequential")),
+                    "interleaved_mixed": dict(res["interleaved_mixed"], **agreement("interleaved_mixed")),
+                    "fp64_sequential": res["fp64_sequential"],
                     "per_case_speedup": t_s / t_b,
-                    "compliance_rel_diff_max": float(np.max(np.abs(cmx - c64) / np.abs(c64))),
-                    "u_rel_diff_max": float(urel.max()), "setup_s": setup_s},
-        "kbyu": {"batched_ms": ms_b, "fp64_single_ms": ms64,
+                    "per_case_speedup_interleaved": t_s / res["interleaved_mixed"]["s_per_step"],
+                    "setup_s": setup_s},
+        "kbyu": {"batched_ms": ms_b, "fp64_single_ms": ms64, "fp32_single_ms": ms32,
                  "batched_per_case_ms": ms_b / k, "per_case_speedup_vs_fp64": ms64 * k / ms_b,
+                 "per_case_speedup_vs_fp32": ms32 * k / ms_b,
                  "batched_GBs": bytes_b / ms_b / 1e6,
                  "useful_TFLOPs": useful / ms_b / 1e9, "executed_mfma_TFLOPs": executed / ms_b / 1e9},
-        "roofline": {"bound": "mfma", "achieved": useful / ms_b / 1e9, "peak": BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": useful / ms_b / 1e9 / BF16_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_bkbyu", "flops_per_launch": useful,
-                     "hbm_frac": bytes_b / ms_b / 1e6 / HBM_PEAK_GBS},
+        "roofline": roof,
     }
     if not a.no_cpu:
         p1 = dict(p)

@@ -34,7 +34,9 @@ enum {
     SGLD_ERR_NOMEM = 6      /* device allocation failed */
 };
 
-enum { SGLD_PREC_FP64 = 0, SGLD_PREC_MIXED = 1 };
+enum { SGLD_PREC_FP64 = 0, SGLD_PREC_MIXED = 1,
+       /* sgld_ctx_pcg_batched only: the 16 cases interleaved per DOF */
+       SGLD_PREC_INTERLEAVED = 2 };
 
 const char *sgld_last_error(void);
 /* Library version string and the gfx target it was built for. */
@@ -150,10 +152,13 @@ int sgld_ctx_bench_kbyu(sgld_ctx *ctx, int precision, int reps, double *msPerLau
  * Solving_CG_GMGS.m at a time; each case follows sgld_ctx_pcg's contract.
  * F, U: numDOF x nrhs, case-major (case j at F + j numDOF); U in: initial
  * guesses, out: solutions.  its[j], relres[j]: per-case iterations and FP64
- * true relative residual.  precision SGLD_PREC_MIXED: all cases iterate
- * together (16 cases interleaved per DOF, K.u on the BF16 matrix cores as
- * hi/lo split products with FP32 accumulation, FP32 V-cycle, FP64 per-case
- * scalars and reliable updates); SGLD_PREC_FP64: nrhs sequential FP64 solves.
+ * true relative residual.  precision SGLD_PREC_FP64: nrhs sequential FP64
+ * solves; SGLD_PREC_MIXED: nrhs sequential mixed-precision solves (the
+ * fastest mode on one MI355X: 1.90 s for 16 cases of a 256^3 cuboid, against
+ * 2.29 s interleaved and 3.13 s FP64); SGLD_PREC_INTERLEAVED: all cases
+ * iterate together (16 cases interleaved per DOF, K.u on the BF16 matrix
+ * cores as hi/lo split products with FP32 accumulation, FP32 V-cycle, FP64
+ * per-case scalars and reliable updates).
  * Returns SGLD_ERR_MAXIT (solutions filled) if any case misses tol. */
 int sgld_ctx_pcg_batched(sgld_ctx *ctx, int nrhs, const double *F, double *U, double tol,
                          int maxIT, int precision, int *its, double *relres);

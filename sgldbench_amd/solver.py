@@ -13,6 +13,7 @@ from ._lib import SGLDError, check, lib, ERR_MAXIT
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_INTERLEAVED = 2  # pcg_batched only: the 16 cases interleaved per DOF, matrix-core K.u
 
 
 def _f64(a):

@@ -1,6 +1,7 @@
 """Batched multi-RHS path: the BF16 matrix-core K.u against the FP64 oracle,
-and the batched mixed-precision solve of several load cases against the
-oracle's FP64 solves of each case."""
+and the solve of several load cases (interleaved 16-case iteration, and the
+default sequential mixed-precision solves) against the oracle's FP64 solves
+of each case."""
 import numpy as np
 import pytest
 
@@ -45,9 +46,10 @@ def test_batched_kbyu_matches_oracle(dims, nrhs):
         assert np.all(Y[j][fixed != 0] == 0)
 
 
+@pytest.mark.parametrize("prec", [solver.PREC_INTERLEAVED, solver.PREC_MIXED])
 @pytest.mark.parametrize("case", ["cuboid32x16x16", "holes32x16x16", "nondyadic32x16x16"])
-def test_batched_solve_matches_oracle(case):
-    print("case: batched solve", case, flush=True)
+def test_batched_solve_matches_oracle(case, prec):
+    print("case: batched solve", case, "precision", prec, flush=True)
     if case == "holes32x16x16":
         vox, levels, nd = holes(32, 16, 16, 7), 3, 0
     else:
@@ -63,7 +65,7 @@ def test_batched_solve_matches_oracle(case):
     m.set_fixed(fixed)
     m.set_modulus(E)
     m.assemble()
-    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000)
+    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000, precision=prec)
     assert g.converged
     assert its[3] == 0 and np.all(U[3] == 0)
     for j in range(F.shape[0]):
@@ -79,7 +81,8 @@ def test_batched_solve_matches_oracle(case):
         assert its[j] <= 1.5 * its_o + 10, (j, its[j], its_o)
 
 
-def test_batched_fp64_is_sequential():
+@pytest.mark.parametrize("prec", [solver.PREC_FP64, solver.PREC_MIXED])
+def test_batched_sequential_modes_equal_single_solves(prec):
     vox = problems.cuboid(32, 16, 16)
     F, fixed, _ = problems.face_load_cases(vox, 3, seed=2)
     E = grf_E(vox, 6)
@@ -87,24 +90,40 @@ def test_batched_fp64_is_sequential():
     g.set_fixed(fixed)
     g.set_modulus(E)
     g.assemble(0.6)
-    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000, precision=solver.PREC_FP64)
+    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000, precision=prec)
+    assert g.converged
     for j in range(3):
-        u1, it1, r1 = g.pcg(F[j], tol=1e-8, maxIT=1000)
-        # FP64 K.u sums LDS face forces in run-dependent order: equal up to rounding
-        assert abs(int(its[j]) - it1) <= 1 and rr[j] <= 1e-8
-        assert np.linalg.norm(U[j] - u1) <= 1e-7 * np.linalg.norm(u1)
+        u1, it1, r1 = g.pcg(F[j], tol=1e-8, maxIT=1000, precision=prec)
+        # K.u sums LDS face forces in run-dependent order: equal up to rounding
+        # (FP64), or to the FP32 inner solve's rounding (mixed)
+        assert abs(int(its[j]) - it1) <= (1 if prec == solver.PREC_FP64 else 3) and rr[j] <= 1e-8
+        assert np.linalg.norm(U[j] - u1) <= (1e-7 if prec == solver.PREC_FP64 else 1e-6) * np.linalg.norm(u1)
 
 
-def test_batched_warm_start_and_maxit():
+@pytest.mark.parametrize("prec", [solver.PREC_INTERLEAVED, solver.PREC_MIXED])
+def test_batched_warm_start_and_maxit(prec):
     vox = problems.cuboid(32, 16, 16)
     F, fixed, _ = problems.face_load_cases(vox, 4, seed=3)
     g = solver.FEASolver(vox, 4)
     g.set_fixed(fixed)
     g.set_modulus(np.ones(int(vox.sum())))
     g.assemble(0.6)
-    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000)
+    U, its, rr = g.pcg_batched(F, tol=1e-8, maxIT=1000, precision=prec)
     assert g.converged
-    U2, its2, rr2 = g.pcg_batched(F, tol=1e-8, maxIT=1000, U0=U)
+    U2, its2, rr2 = g.pcg_batched(F, tol=1e-8, maxIT=1000, U0=U, precision=prec)
     assert np.all(its2 == 0) and np.all(rr2 <= 1e-8)
-    U3, its3, rr3 = g.pcg_batched(F, tol=1e-12, maxIT=5)
+    U3, its3, rr3 = g.pcg_batched(F, tol=1e-12, maxIT=5, precision=prec)
     assert not g.converged and np.all(its3 <= 5)
+
+
+def test_batched_rejects_unknown_precision():
+    vox = problems.cuboid(16, 8, 8)
+    F, fixed, _ = problems.face_load_cases(vox, 2, seed=4)
+    g = solver.FEASolver(vox, 3)
+    g.set_fixed(fixed)
+    g.set_modulus(np.ones(int(vox.sum())))
+    g.assemble(0.6)
+    with pytest.raises(solver.SGLDError):
+        g.pcg_batched(F, tol=1e-8, maxIT=100, precision=7)
+    with pytest.raises(solver.SGLDError):  # the interleaved mode is a batched-solve mode only
+        g.pcg(F[0], tol=1e-8, maxIT=100, precision=solver.PREC_INTERLEAVED)
