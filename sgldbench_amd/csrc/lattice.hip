@@ -1858,7 +1858,7 @@ struct SolveState {
     // mixed precision
     double scale = 1.0, normb = 0.0, delta = 0.1;
     int itsAtUpdate = 0;
-    static constexpr int kMaxStretch = 96;  // force a reliable update if FP32 stalls
+    int maxStretch = 96;  // force a reliable update after this many FP32 iterations
 };
 
 int fp64_begin(sgld_ctx *C, SolveState &S) {
@@ -1903,10 +1903,16 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
         return SGLD_OK;
     }
     S.normb = normr / res;
-    // 0.1: on the 125 M-DOF ellipsoid domain 1e-2 let the FP32 recurrence drift (681 vs 397
-    // iterations); on the 12.8 M-DOF cuboid it costs ~8 % over 1e-2.  1e-4 stagnates.
+    // Reliable update when the FP32 residual has dropped by delta = 0.1.
+    // Measured, mixed solve at delta 0.01 / 0.1: 256x128x128 cuboid 29.2 /
+    // 31.2 ms, part_R256 44.5 / 42.3 ms (its long FP32 stretches drift: true /
+    // recursive residual 1.8 at delta 0.01, 1.05 at 0.1), 125 M-DOF
+    // ellipsoids 0.893 / 0.884 s.  Starting at 0.1 and narrowing after a
+    // clean first stretch gave 30.0-31.0 ms, 42.7-43.0 ms and 0.902 s: no
+    // signal in the first stretch tells the cuboid from the ellipsoids.
     S.delta = 0.1;
     if (const char *env = std::getenv("SGLD_MIXED_DELTA")) S.delta = std::atof(env);
+    if (const char *env = std::getenv("SGLD_MIXED_STRETCH")) S.maxStretch = std::max(1, std::atoi(env));
     S.scale = normr;
     S.itsAtUpdate = 0;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
@@ -1961,11 +1967,11 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
         return SGLD_OK;
     }
     // Mixed: the FP32 Krylov iteration on the residual scaled to unit norm;
-    // when its recursive residual has dropped by delta (or after kMaxStretch
+    // when its recursive residual has dropped by delta (or after maxStretch
     // iterations), fold the FP32 correction into the FP64 solution and
     // replace the residual by the FP64 true residual (reliable update).
     S.its = st.its;
-    if (!st.done && S.its - S.itsAtUpdate < SolveState::kMaxStretch) return SGLD_OK;
+    if (!st.done && S.its - S.itsAtUpdate < S.maxStretch) return SGLD_OK;
     S.itsAtUpdate = S.its;
     hipLaunchKernelGGL(k_fold_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, (const float *)C->p32.p,
                        S.scale, V, (const Ctl *)C->ctl.p);  // with the pending x32 += lambda p32
@@ -1985,16 +1991,28 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
                        1.0 / S.scale, V);
     hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (float)f, V);
     SGLD_HIP(hipGetLastError());
+    // rho of the iteration whose direction p holds, for beta = rho' / rho of
+    // the replaced residual: after a stop that iteration's own rho; after a
+    // forced update (stretch limit, every iteration of the batch complete)
+    // the last iteration has already formed rho_{i+1} and beta_{i+1} =
+    // rho_{i+1} / rho_i, so rho_i = rho_{i+1} / beta_{i+1}.  Continuing the
+    // direction there, instead of restarting it, keeps slow stretches from
+    // compounding (125 M-DOF ellipsoids at delta 0.01: 665 -> 406 iterations).
+    double rhoDir = st.rho;
+    bool keep = true;
+    if (!st.done) {
+        keep = st.beta > 0.0;
+        if (keep) rhoDir = st.rho / st.beta;
+    }
     *C->hctl = st;
     C->hctl->done = 0;
-    C->hctl->rho = st.rho * f * f;
+    C->hctl->rho = rhoDir * f * f;
     C->hctl->lambda = 0.0;  // applied above
     C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
-    // rho = z.r and beta for the replaced residual, then continue from p.  After
-    // a forced update (FP32 stalled mid-stretch) rho already belongs to the
-    // recursive residual, so the direction restarts (beta = 0).
-    SGLD_TRY(enqueue_rho<float>(C, C->r32.p, st.done ? FIN_RHO : FIN_RHO0));
+    // rho = z.r and beta for the replaced residual, then continue from p
+    // (restart, beta = 0, only when the direction has just been restarted)
+    SGLD_TRY(enqueue_rho<float>(C, C->r32.p, keep ? FIN_RHO : FIN_RHO0));
     return SGLD_OK;
 }
 

@@ -269,3 +269,24 @@ def test_tiled_restriction_matches_node_centric(case, cz, monkeypatch):
     scale = np.abs(ref).max()
     assert np.abs(zs[0] - zs[1]).max() <= 1e-12 * scale
     assert np.abs(zs[0] - ref).max() <= 1e-11 * scale
+
+
+def test_mixed_forced_updates_continue_direction(monkeypatch):
+    """A reliable update forced by the stretch limit (here every 8 FP32
+    iterations, with a threshold the stretches never reach) keeps the search
+    direction: the solve converges in about the iterations of the default
+    schedule, where restarting the direction at each one would make it a
+    restarted CG(8)."""
+    vox = holes(32, 16, 16, 9, nholes=4, r=3)
+    g, m, F, fixed, E = build_pair(vox, 3)
+    U0, its0, rr0 = g.pcg(F, tol=1e-8, maxIT=2000, precision=solver.PREC_MIXED)
+    assert g.converged
+    monkeypatch.setenv("SGLD_MIXED_DELTA", "1e-12")
+    monkeypatch.setenv("SGLD_MIXED_STRETCH", "8")
+    U, its, rr = g.pcg(F, tol=1e-8, maxIT=2000, precision=solver.PREC_MIXED)
+    st = g.stats()
+    print("default its %d; forced every 8: its %d, updates %d" % (its0, its, st["outer"]), flush=True)
+    assert g.converged and rr <= 1e-8
+    assert st["outer"] >= its // 8 - 1
+    assert its <= 1.25 * its0 + 8
+    assert np.linalg.norm(U - U0) <= 1e-6 * np.linalg.norm(U0)
