@@ -805,6 +805,25 @@ __global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ 
     }
 }
 
+// Sum over the block returned to every thread: the lane-0 wave sums are
+// added by thread 0 in wave order and broadcast, so every block summing the
+// same values with the same mapping gets the same bits.
+__device__ __forceinline__ double block_allsum(double v) {
+    __shared__ double sh[17];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) t += sh[i];
+        sh[16] = t;
+    }
+    __syncthreads();
+    return sh[16];
+}
+
 __device__ __forceinline__ void block_sum2(double &a, double &b) {
     __shared__ double sh[2][16];
     for (int o = 32; o > 0; o >>= 1) {
@@ -834,9 +853,21 @@ template <class T>
 __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
                                                   const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
-                                                  const Ctl *ctl) {
+                                                  const double *__restrict__ partPQ, int nPQ, Ctl *ctl) {
     if (stopped(ctl)) return;
-    const T lt = (T)ctl->lambda;
+    // lambda = rho / p.q from the K.u partials, summed by every block in the
+    // same order (so every block gets the same bits); block 0 records p.q and
+    // lambda for the deferred x += lambda p.  This replaces a separate
+    // single-block finishing launch.
+    double pq = 0.0;
+    for (int i = threadIdx.x; i < nPQ; i += blockDim.x) pq += partPQ[i];
+    pq = block_allsum(pq);
+    const double lambda = ctl->rho / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->pq = pq;
+        ctl->lambda = lambda;
+    }
+    const T lt = (T)lambda;
     double rr = 0.0, dr = 0.0;
     FOR_SPAN(i, S) {
         const T rn = r[i] - lt * q[i];
@@ -1154,7 +1185,7 @@ __global__ void k_scale(T *v, T f, Span S) {
     FOR_SPAN(i, S) v[i] *= f;
 }
 
-enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_PQ, FIN_RR, FIN_RHO };
+enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_RR, FIN_RHO };  // (lambda = rho / p.q: in k_update_r)
 
 // Second reduction stage (fixed order, deterministic) and the scalar updates
 // of the PCG recurrence.
@@ -1172,10 +1203,6 @@ __global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial
         case FIN_RHO0:
             ctl->rho = s;
             ctl->beta = 0.0;
-            break;
-        case FIN_PQ:
-            ctl->pq = s;
-            ctl->lambda = ctl->rho / s;
             break;
         case FIN_RR:
             ctl->rr = s;
@@ -1829,9 +1856,8 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     SGLD_HIP(hipGetLastError());
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
-    SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
     hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, fine_dinv<T>(C), S,
-                       slot(C, 1), slot(C, 3), (const Ctl *)ctl);
+                       slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
     int nb2 = 0;
