@@ -54,6 +54,49 @@ __global__ void __launch_bounds__(256) k_density_filter(const double *__restrict
     }
 }
 
+// LDS-tiled form for R <= kFRmax: a block filters a kFBy x kFBx x kFBz
+// (row, column, layer) brick of voxels, one thread each, from its
+// neighbourhood staged in LDS (0 outside the lattice).  Every voxel walks the
+// full (2R+1)^3 window in the same order as k_density_filter; a neighbour
+// outside the lattice adds w * 0 = +0, which leaves the sum unchanged (the sum
+// starts at +0 and never becomes -0), so the results are the same bits.  The
+// weight offsets are uniform across the block (scalar loads); the per-thread
+// gathers come from LDS instead of the cache hierarchy.
+constexpr int kFBy = 8, kFBx = 8, kFBz = 4, kFRmax = 4;
+template <int R>
+__global__ void __launch_bounds__(256) k_density_filter_tiled(const double *__restrict__ xl,
+                                                              const int32_t *__restrict__ efwd, int nx, int ny,
+                                                              int nz, const double *__restrict__ w,
+                                                              double *__restrict__ Y, int bricksY, int bricksX) {
+    constexpr int W = 2 * R + 1, SY = kFBy + 2 * R, SX = kFBx + 2 * R, SZ = kFBz + 2 * R;
+    __shared__ double sh[SY * SX * SZ];
+    const int64_t nxy = (int64_t)nx * ny;
+    const int by = blockIdx.x % bricksY, bx = (blockIdx.x / bricksY) % bricksX, bz = blockIdx.x / (bricksY * bricksX);
+    const int y0 = by * kFBy - R, x0 = bx * kFBx - R, z0 = bz * kFBz - R;
+    for (int t = threadIdx.x; t < SY * SX * SZ; t += blockDim.x) {
+        const int ly = t % SY, lx = (t / SY) % SX, lz = t / (SY * SX);
+        const int gy = y0 + ly, gx = x0 + lx, gz = z0 + lz;
+        const bool in = gy >= 0 && gy < ny && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+        sh[t] = in ? xl[gz * nxy + (int64_t)gx * ny + gy] : 0.0;
+    }
+    __syncthreads();
+    const int ty = threadIdx.x % kFBy, tx = (threadIdx.x / kFBy) % kFBx, tz = threadIdx.x / (kFBy * kFBx);
+    const int jj = by * kFBy + ty, ii = bx * kFBx + tx, kk = bz * kFBz + tz;
+    if (jj >= ny || ii >= nx || kk >= nz) return;
+    const int32_t e1 = efwd[kk * nxy + (int64_t)ii * ny + jj];
+    if (!e1) return;
+    double acc = 0.0;
+    for (int dz = 0; dz < W; ++dz)
+#pragma unroll
+        for (int dx = 0; dx < W; ++dx) {
+            const double *srow = sh + ((tz + dz) * SX + (tx + dx)) * SY + ty;
+            const double *wrow = w + (dz * W + dx) * W;
+#pragma unroll
+            for (int dy = 0; dy < W; ++dy) acc += wrow[dy] * srow[dy];
+        }
+    Y[e1 - 1] = acc;
+}
+
 std::vector<double> cone_weights(double rMin, int R) {
     const int W = 2 * R + 1;
     std::vector<double> w((size_t)W * W * W);
@@ -79,8 +122,20 @@ int run_filter(hipStream_t s, const double *x_dev, double rMin, const int32_t *e
     SGLD_TRY(xl.alloc(nv));
     hipLaunchKernelGGL(k_filter_scatter, dim3(bounded_blocks(nv, 256)), dim3(256), 0, s, x_dev, efwd_dev, nv, xl.p);
     if (timed) SGLD_HIP(hipEventRecord(timed->e0, s));
-    hipLaunchKernelGGL(k_density_filter, dim3(bounded_blocks(nv, 256)), dim3(256), 0, s, xl.p, efwd_dev, resX, resY,
-                       resZ, R, dw.p, Y_dev);
+    if (R <= kFRmax) {
+        const int bY = (resY + kFBy - 1) / kFBy, bX = (resX + kFBx - 1) / kFBx, bZ = (resZ + kFBz - 1) / kFBz;
+        const dim3 grid((unsigned)((int64_t)bY * bX * bZ)), block(kFBy * kFBx * kFBz);
+        switch (R) {
+            case 0: hipLaunchKernelGGL(k_density_filter_tiled<0>, grid, block, 0, s, xl.p, efwd_dev, resX, resY, resZ, dw.p, Y_dev, bY, bX); break;
+            case 1: hipLaunchKernelGGL(k_density_filter_tiled<1>, grid, block, 0, s, xl.p, efwd_dev, resX, resY, resZ, dw.p, Y_dev, bY, bX); break;
+            case 2: hipLaunchKernelGGL(k_density_filter_tiled<2>, grid, block, 0, s, xl.p, efwd_dev, resX, resY, resZ, dw.p, Y_dev, bY, bX); break;
+            case 3: hipLaunchKernelGGL(k_density_filter_tiled<3>, grid, block, 0, s, xl.p, efwd_dev, resX, resY, resZ, dw.p, Y_dev, bY, bX); break;
+            default: hipLaunchKernelGGL(k_density_filter_tiled<4>, grid, block, 0, s, xl.p, efwd_dev, resX, resY, resZ, dw.p, Y_dev, bY, bX); break;
+        }
+    } else {
+        hipLaunchKernelGGL(k_density_filter, dim3(bounded_blocks(nv, 256)), dim3(256), 0, s, xl.p, efwd_dev, resX,
+                           resY, resZ, R, dw.p, Y_dev);
+    }
     if (timed) SGLD_HIP(hipEventRecord(timed->e1, s));
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipStreamSynchronize(s));

@@ -11,7 +11,7 @@ from tests.test_filter_pin import voids
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("rMin", [1.0, 1.5, 2.5, 3.2])
+@pytest.mark.parametrize("rMin", [1.0, 1.5, 2.5, 3.2, 4.9, 5.5])  # R = 0..3 and 4 (LDS-tiled), 5 (gather)
 @pytest.mark.parametrize("dims", [(7, 9, 6), (32, 24, 20)])
 def test_filter_bitexact(rMin, dims):
     vox, efwd = voids(*dims, seed=5)
