@@ -59,27 +59,125 @@ __global__ void __launch_bounds__(256) k_pde_setup(const double *__restrict__ sr
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// y = K u (scalar 8 x 8 element matrix, column-major), partial u.y
+// The 8 x 8 element matrix by value: with the indices known at compile time
+// its entries are uniform scalar loads.
+struct PdeKe {
+    double v[64];  // column-major, Ke[j + 8 k]
+};
+
+// y = K u (scalar 8 x 8 element matrix), partial u.y.  A thread owns a node
+// column (row, col) and a chunk of kPdeCz layers and marches up it: the 3 x 3
+// node plane above is loaded per layer, the two below are kept from the
+// previous layers, and the 4 solid flags of the element layer above likewise,
+// so a node costs 9 + 4 loads instead of 27 + 8.  Loads are unconditional
+// (indices clamped into the lattice; a clamped value only meets elements
+// outside it, whose weight is 0) and each element is weighted by 1 (solid) or
+// 0 (void or outside), so they issue together.
+constexpr int kPdeCz = 8;
 __global__ void __launch_bounds__(256) k_pde_kbyu(const double *__restrict__ u, const int32_t *__restrict__ efwd,
-                                                  Grid gd, const double *__restrict__ Ke, double *__restrict__ y,
+                                                  Grid gd, PdeKe K, double *__restrict__ y,
                                                   double *__restrict__ partial) {
-    const int64_t NY = gd.NY(), NX = gd.NX(), nN = gd.nodes(), pl = NY * NX;
+    const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
+    const int64_t pl = (int64_t)NY * NX, ple = (int64_t)gd.ny * gd.nx;
+    const int64_t cols = pl, chunks = (NZ + kPdeCz - 1) / kPdeCz;
     double acc = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nN; g += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / pl);
-        double yv = 0.0;
-        FOR_NODE_ELEMENTS(
-            for (int k = 0; k < 8; ++k) {
-                const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
-                const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
-                const int zk = (k >= 4);
-                yv += Ke[j + 8 * k] * u[(er + 1 - yk) + NY * (ec + xk) + pl * (el + zk)];
-            })
-        y[g] = yv;
-        acc += u[g] * yv;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cols * chunks;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = t % cols, ch = t / cols;
+        const int r = (int)(col % NY), c = (int)(col / NY);
+        const int l0 = (int)ch * kPdeCz, l1 = min(l0 + kPdeCz, NZ);
+        int64_t off[3][3];  // in-plane offsets of the 3 x 3 neighbourhood (clamped)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                off[dx][dy] = min(max(r - 1 + dy, 0), NY - 1) + (int64_t)NY * min(max(c - 1 + dx, 0), NX - 1);
+        // element (er, ec) columns around the node: in-plane offset and whether inside
+        int64_t eoff[2][2];
+        bool ein[2][2];
+#pragma unroll
+        for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+            for (int dr = 0; dr < 2; ++dr) {
+                const int er = r - 1 + dr, ec = c - 1 + dc;
+                ein[dc][dr] = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
+                eoff[dc][dr] = ein[dc][dr] ? er + (int64_t)gd.ny * ec : 0;
+            }
+        auto plane = [&](int lz, double (&pv)[3][3]) {
+            const int64_t base = pl * min(max(lz, 0), NZ - 1);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) pv[dx][dy] = u[base + off[dx][dy]];
+        };
+        auto solid = [&](int el, double (&w)[2][2]) {
+            const bool lin = el >= 0 && el < gd.nz;
+            const int64_t base = lin ? ple * el : 0;
+#pragma unroll
+            for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+                for (int dr = 0; dr < 2; ++dr) {
+                    const int32_t ev = efwd[base + eoff[dc][dr]];
+                    w[dc][dr] = (lin && ein[dc][dr] && ev) ? 1.0 : 0.0;
+                }
+        };
+        double P[3][3][3];   // node planes l-1, l, l+1
+        double Wl[2][2][2];  // solid weights of element layers l-1, l
+        plane(l0 - 1, P[0]);
+        plane(l0, P[1]);
+        solid(l0 - 1, Wl[0]);
+        for (int l = l0; l < l1; ++l) {
+            plane(l + 1, P[2]);
+            solid(l, Wl[1]);
+            double yv = 0.0;
+#pragma unroll
+            for (int dl = 0; dl < 2; ++dl)
+#pragma unroll
+                for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+                    for (int dr = 0; dr < 2; ++dr) {
+                        const int j = (dl ? 0 : 4) + (dr ? ((1 - dc) ? 2 : 3) : ((1 - dc) ? 1 : 0));
+                        double sEl = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+                            const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+                            const int zk = (k >= 4);
+                            sEl += K.v[j + 8 * k] * P[dl + zk][dc + xk][dr + 1 - yk];
+                        }
+                        yv += Wl[dl][dc][dr] * sEl;
+                    }
+            y[pl * l + col] = yv;
+            acc += P[1][1][1] * yv;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    P[0][dx][dy] = P[1][dx][dy];
+                    P[1][dx][dy] = P[2][dx][dy];
+                }
+#pragma unroll
+            for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+                for (int dr = 0; dr < 2; ++dr) Wl[0][dc][dr] = Wl[1][dc][dr];
+        }
     }
     const double s = psum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[0] = sum of partial[0 .. nb), out[1] = sum of partial[nb .. 2 nb)
+// (one block, fixed order), so the host copies two values instead of the
+// partials.
+__global__ void __launch_bounds__(256) k_pde_fin(const double *__restrict__ partial, int nb, int sets,
+                                                 double *__restrict__ out) {
+    for (int k = 0; k < sets; ++k) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) a += partial[(int64_t)k * nb + i];
+        const double s = psum(a);
+        if (threadIdx.x == 0) out[k] = s;
+        __syncthreads();
+    }
 }
 
 // x += lambda p; r -= lambda q; z = dinv r; partials r.r and z.r
@@ -168,20 +266,30 @@ extern "C" int sgld_ctx_pde_filter_device(sgld_ctx *C, const double *dsrc, doubl
     const int64_t n = gd.nodes(), ne = C->lv[0].numElements;
     double Ke[64];
     pde_element(rHatMin, Ke);
+    PdeKe K;
+    std::memcpy(K.v, Ke, sizeof(Ke));
     DevBuf<double> dKe, b, x, r, z, p, q, dinv, part;
     SGLD_TRY(dKe.upload(Ke, 64));
     for (DevBuf<double> *v : {&b, &x, &r, &z, &p, &q, &dinv}) SGLD_TRY(v->alloc(n));
     const int nb = grid_blocks(n);
-    SGLD_TRY(part.alloc(2 * (size_t)nb));
-    std::vector<double> hp(2 * (size_t)nb);
-    auto reduce = [&](int which, double *out) -> int {
-        SGLD_HIP(hipMemcpyAsync(hp.data(), part.p + (size_t)which * nb, sizeof(double) * nb, hipMemcpyDeviceToHost, C->s));
+    SGLD_TRY(part.alloc(2 * (size_t)nb + 2));
+    double *sums = part.p + 2 * (size_t)nb;  // k_pde_fin's two results
+    // the two sums travel through the context's pinned control-block mirror
+    // (16 bytes, one wait), not through pageable memory
+    double *hs = reinterpret_cast<double *>(C->hctl);
+    auto reduce_sets = [&](int first, int sets, double *out0, double *out1) -> int {
+        hipLaunchKernelGGL(k_pde_fin, dim3(1), dim3(256), 0, C->s, (const double *)part.p + (size_t)first * nb, nb,
+                           sets, sums);
+        SGLD_HIP(hipGetLastError());
+        SGLD_HIP(hipMemcpyAsync(hs, sums, sizeof(double) * sets, hipMemcpyDeviceToHost, C->s));
         SGLD_HIP(hipStreamSynchronize(C->s));
-        double s = 0;
-        for (int i = 0; i < nb; ++i) s += hp[i];
-        *out = s;
+        *out0 = hs[0];
+        if (out1) *out1 = hs[1];
         return SGLD_OK;
     };
+    auto reduce = [&](int which, double *out) -> int { return reduce_sets(which, 1, out, nullptr); };
+    // both partial sets of k_pde_update with one copy and one wait
+    auto reduce2 = [&](double *out0, double *out1) -> int { return reduce_sets(0, 2, out0, out1); };
     hipLaunchKernelGGL(k_pde_setup, dim3(nb), dim3(256), 0, C->s, dsrc, C->efwd.p, gd, dKe.p, b.p, dinv.p, part.p);
     SGLD_HIP(hipGetLastError());
     double bb = 0;
@@ -200,15 +308,14 @@ extern "C" int sgld_ctx_pde_filter_device(sgld_ctx *C, const double *dsrc, doubl
     int its = 0;
     if (normB > 0.0) {
         for (its = 1; its <= maxIT; ++its) {
-            hipLaunchKernelGGL(k_pde_kbyu, dim3(nb), dim3(256), 0, C->s, p.p, C->efwd.p, gd, dKe.p, q.p, part.p);
+            hipLaunchKernelGGL(k_pde_kbyu, dim3(nb), dim3(256), 0, C->s, p.p, C->efwd.p, gd, K, q.p, part.p);
             double pq = 0;
             SGLD_TRY(reduce(0, &pq));
             const double lambda = rho / pq;
             hipLaunchKernelGGL(k_pde_update, dim3(nb), dim3(256), 0, C->s, x.p, r.p, z.p, p.p, q.p, dinv.p, lambda,
                                n, part.p, part.p + nb);
             double rr = 0, rho2 = 0;
-            SGLD_TRY(reduce(0, &rr));
-            SGLD_TRY(reduce(1, &rho2));
+            SGLD_TRY(reduce2(&rr, &rho2));
             if (std::sqrt(rr) / normB < tol) break;
             hipLaunchKernelGGL(k_pde_axpby, dim3(nb), dim3(256), 0, C->s, p.p, (const double *)z.p, rho2 / rho, n);
             rho = rho2;
