@@ -62,6 +62,48 @@ def measured_traffic():
     return None
 
 
+def measured_copy_GBs(reps=10):
+    """Achievable HBM bandwidth on this GPU: a 1 GiB device-to-device copy
+    (read + write), HIP-event timed; the spec peak stays the roofline's peak."""
+    import torch
+    src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * 128 * 1024 * 1024 * 8 / ms / 1e6
+
+
+def measured_bf16_gemm_TFLOPs(n=8192, reps=10):
+    """Achievable dense BF16 matrix-core rate: an n^3 torch matmul (rocBLAS /
+    hipBLASLt), HIP-event timed."""
+    import torch
+    a = torch.randn(n, n, dtype=torch.bfloat16, device="cuda:0")
+    b = torch.randn(n, n, dtype=torch.bfloat16, device="cuda:0")
+    for _ in range(3):
+        c = a @ b
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        c = a @ b
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 2.0 * n ** 3 / ms / 1e9
+
+
 def time_kbyu(a, s, prec):
     """ms per launch of `a.kbyu_reps` back-to-back K.u launches (HIP events on
     the solver stream) after `a.kbyu_warmup` untimed ones."""
@@ -316,15 +358,19 @@ def run_batched(a):
     best = min(("mixed_sequential", "interleaved_mixed"), key=lambda m: res[m]["s_per_step"]
                if res[m]["converged"] else float("inf"))
     t_b = res[best]["s_per_step"]
+    copy_gbs = measured_copy_GBs()
+    gemm_tf = measured_bf16_gemm_TFLOPs()
     if best == "mixed_sequential":  # dominant kernel: the single-case FP32 K.u
         roof = {"bound": "hbm", "achieved": bytes32 / ms32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": bytes32 / ms32 / 1e6 / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_kbyu_tiled<float>", "bytes_per_launch": bytes32}
+                "kernel": "k_kbyu_tiled<float>", "bytes_per_launch": bytes32,
+                "measured_copy_GBs": copy_gbs, "frac_of_measured": bytes32 / ms32 / 1e6 / copy_gbs}
     else:
         roof = {"bound": "mfma", "achieved": useful / ms_b / 1e9, "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": useful / ms_b / 1e9 / BF16_PEAK_TFLOPS, "traffic": None,
                 "kernel": "k_bkbyu", "flops_per_launch": useful,
-                "hbm_frac": bytes_b / ms_b / 1e6 / HBM_PEAK_GBS}
+                "hbm_frac": bytes_b / ms_b / 1e6 / HBM_PEAK_GBS,
+                "measured_bf16_gemm_TFLOPs": gemm_tf, "frac_of_measured": useful / ms_b / 1e9 / gemm_tf}
     out = {
         "metric": "MG-PCG time to rel-res 1e-8 & K.u GDOF/s, FP64 vs mixed/batched, 1 MI355X",
         "value": k * n / t_b / 1e9,
@@ -348,7 +394,8 @@ def run_batched(a):
                  "batched_per_case_ms": ms_b / k, "per_case_speedup_vs_fp64": ms64 * k / ms_b,
                  "per_case_speedup_vs_fp32": ms32 * k / ms_b,
                  "batched_GBs": bytes_b / ms_b / 1e6,
-                 "useful_TFLOPs": useful / ms_b / 1e9, "executed_mfma_TFLOPs": executed / ms_b / 1e9},
+                 "useful_TFLOPs": useful / ms_b / 1e9, "executed_mfma_TFLOPs": executed / ms_b / 1e9,
+                 "measured_bf16_gemm_TFLOPs": gemm_tf, "measured_copy_GBs": copy_gbs},
         "roofline": roof,
     }
     if not a.no_cpu:
@@ -440,6 +487,7 @@ def main():
     # K.u roofline (FP64 and FP32), hipEvent-timed on the solver stream
     ms64 = time_kbyu(a, s, solver.PREC_FP64)
     ms32 = time_kbyu(a, s, solver.PREC_MIXED)
+    copy_gbs = measured_copy_GBs()
     bytes64 = 2 * n * 8 + ne * 8
     bytes32 = 2 * n * 4 + ne * 4
     # density filter of the same mesh (rMin 2.5: 5^3 cone), kernel time
@@ -494,7 +542,8 @@ def main():
                      "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if (a.workload == "cfg1" and (nx, ny, nz) == (256, 128, 128)
                                                        and p["num_levels"] == 6) else None,
-                     "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64},
+                     "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64,
+                     "measured_copy_GBs": copy_gbs, "frac_of_measured": bytes64 / ms64 / 1e6 / copy_gbs},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=a.cpu_its)
