@@ -76,6 +76,7 @@ struct sgld_ctx {
     sgld::DevBuf<double> b, x, r, z, p, p2, q;
     sgld::DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
     sgld::DevBuf<double> partial;
+    sgld::DevBuf<double> pdeWork;  // PDE filter work vectors, allocated on first use and kept
     sgld::DevBuf<sgld::Ctl> ctl;
     sgld::Ctl *hctl = nullptr;  // pinned host mirror
     hipStream_t s = nullptr;

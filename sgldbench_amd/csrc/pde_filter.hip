@@ -59,6 +59,14 @@ __global__ void __launch_bounds__(256) k_pde_setup(const double *__restrict__ sr
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Device-side scalars of the filter's CG (so a batch of iterations runs
+// without host waits); every iteration kernel returns at once after a stop.
+struct PCtl {
+    double rho, lambda, beta, normb, tol;
+    int its, maxit, done, pad;
+};
+__device__ __forceinline__ bool pstopped(const PCtl *c) { return c && c->done; }
+
 // The 8 x 8 element matrix by value: with the indices known at compile time
 // its entries are uniform scalar loads.
 struct PdeKe {
@@ -76,7 +84,8 @@ struct PdeKe {
 constexpr int kPdeCz = 8;
 __global__ void __launch_bounds__(256) k_pde_kbyu(const double *__restrict__ u, const int32_t *__restrict__ efwd,
                                                   Grid gd, PdeKe K, double *__restrict__ y,
-                                                  double *__restrict__ partial) {
+                                                  double *__restrict__ partial, const PCtl *ctl) {
+    if (pstopped(ctl)) return;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
     const int64_t pl = (int64_t)NY * NX, ple = (int64_t)gd.ny * gd.nx;
     const int64_t cols = pl, chunks = (NZ + kPdeCz - 1) / kPdeCz;
@@ -180,12 +189,44 @@ __global__ void __launch_bounds__(256) k_pde_fin(const double *__restrict__ part
     }
 }
 
+// Scalar steps of one CG iteration (Solving_PreconditionedConjugateGradientSolver.m:
+// 15-29) from the partial sums: op 0, lambda = rho / p.q; op 1, with r.r and
+// z.r, the stopping test on ||r|| / ||b|| (done 1; done 2 at maxit), then
+// beta = z.r / rho and rho = z.r.
+__global__ void __launch_bounds__(256) k_pde_fin_iter(const double *__restrict__ partial, int nb, int op,
+                                                      PCtl *ctl) {
+    if (ctl->done) return;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += partial[i];
+        if (op == 1) b += partial[nb + i];
+    }
+    const double sa = psum(a);
+    __syncthreads();
+    const double sb = psum(b);
+    if (threadIdx.x != 0) return;
+    if (op == 0) {
+        ctl->lambda = ctl->rho / sa;
+        return;
+    }
+    ctl->its += 1;
+    if (sqrt(sa) / ctl->normb < ctl->tol) {
+        ctl->done = 1;
+        return;
+    }
+    if (ctl->its >= ctl->maxit) ctl->done = 2;
+    ctl->beta = sb / ctl->rho;
+    ctl->rho = sb;
+}
+
 // x += lambda p; r -= lambda q; z = dinv r; partials r.r and z.r
 __global__ void __launch_bounds__(256) k_pde_update(double *__restrict__ x, double *__restrict__ r,
                                                     double *__restrict__ z, const double *__restrict__ p,
                                                     const double *__restrict__ q, const double *__restrict__ dinv,
-                                                    double lambda, int64_t n, double *__restrict__ prr,
-                                                    double *__restrict__ pzr) {
+                                                    double lambda0, int64_t n, double *__restrict__ prr,
+                                                    double *__restrict__ pzr, const PCtl *ctl) {
+    if (pstopped(ctl)) return;
+    const double lambda = ctl ? ctl->lambda : lambda0;
     double a = 0.0, bz = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         x[i] += lambda * p[i];
@@ -205,7 +246,9 @@ __global__ void __launch_bounds__(256) k_pde_update(double *__restrict__ x, doub
     }
 }
 
-__global__ void k_pde_axpby(double *__restrict__ p, const double *__restrict__ z, double beta, int64_t n) {
+__global__ void k_pde_axpby(double *__restrict__ p, const double *__restrict__ z, int64_t n, const PCtl *ctl) {
+    if (pstopped(ctl)) return;
+    const double beta = ctl->beta;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = z[i] + beta * p[i];
 }
@@ -268,11 +311,20 @@ extern "C" int sgld_ctx_pde_filter_device(sgld_ctx *C, const double *dsrc, doubl
     pde_element(rHatMin, Ke);
     PdeKe K;
     std::memcpy(K.v, Ke, sizeof(Ke));
-    DevBuf<double> dKe, b, x, r, z, p, q, dinv, part;
-    SGLD_TRY(dKe.upload(Ke, 64));
-    for (DevBuf<double> *v : {&b, &x, &r, &z, &p, &q, &dinv}) SGLD_TRY(v->alloc(n));
+    // work vectors b, x, r, z, p, q, dinv, the element matrix, partial sums
+    // and the control block, in one allocation kept by the context (the
+    // per-call allocations cost about as much as the iterations)
     const int nb = grid_blocks(n);
-    SGLD_TRY(part.alloc(2 * (size_t)nb + 2));
+    const size_t need = 7 * (size_t)n + 64 + 2 * (size_t)nb + 2 + 8;
+    if (C->pdeWork.n < need) SGLD_TRY(C->pdeWork.alloc(need));
+    struct View {
+        double *p;
+    };
+    View b{C->pdeWork.p}, x{b.p + n}, r{x.p + n}, z{r.p + n}, p{z.p + n}, q{p.p + n}, dinv{q.p + n};
+    View dKe{dinv.p + n}, part{dKe.p + 64};
+    PCtl *dctl = reinterpret_cast<PCtl *>(part.p + 2 * (size_t)nb + 2);
+    static_assert(sizeof(PCtl) <= 8 * sizeof(double), "control block slot");
+    SGLD_HIP(hipMemcpyAsync(dKe.p, Ke, sizeof(Ke), hipMemcpyHostToDevice, C->s));
     double *sums = part.p + 2 * (size_t)nb;  // k_pde_fin's two results
     // the two sums travel through the context's pinned control-block mirror
     // (16 bytes, one wait), not through pageable memory
@@ -288,8 +340,6 @@ extern "C" int sgld_ctx_pde_filter_device(sgld_ctx *C, const double *dsrc, doubl
         return SGLD_OK;
     };
     auto reduce = [&](int which, double *out) -> int { return reduce_sets(which, 1, out, nullptr); };
-    // both partial sets of k_pde_update with one copy and one wait
-    auto reduce2 = [&](double *out0, double *out1) -> int { return reduce_sets(0, 2, out0, out1); };
     hipLaunchKernelGGL(k_pde_setup, dim3(nb), dim3(256), 0, C->s, dsrc, C->efwd.p, gd, dKe.p, b.p, dinv.p, part.p);
     SGLD_HIP(hipGetLastError());
     double bb = 0;
@@ -301,26 +351,38 @@ extern "C" int sgld_ctx_pde_filter_device(sgld_ctx *C, const double *dsrc, doubl
     SGLD_HIP(hipMemcpyAsync(r.p, b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, C->s));
     SGLD_HIP(hipMemsetAsync(q.p, 0, sizeof(double) * n, C->s));
     hipLaunchKernelGGL(k_pde_update, dim3(nb), dim3(256), 0, C->s, x.p, r.p, z.p, p.p, q.p, dinv.p, 0.0, n,
-                       part.p, part.p + nb);
+                       part.p, part.p + nb, (const PCtl *)nullptr);
     SGLD_HIP(hipMemcpyAsync(p.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, C->s));
     double rho = 0;
     SGLD_TRY(reduce(1, &rho));
     int its = 0;
     if (normB > 0.0) {
-        for (its = 1; its <= maxIT; ++its) {
-            hipLaunchKernelGGL(k_pde_kbyu, dim3(nb), dim3(256), 0, C->s, p.p, C->efwd.p, gd, K, q.p, part.p);
-            double pq = 0;
-            SGLD_TRY(reduce(0, &pq));
-            const double lambda = rho / pq;
-            hipLaunchKernelGGL(k_pde_update, dim3(nb), dim3(256), 0, C->s, x.p, r.p, z.p, p.p, q.p, dinv.p, lambda,
-                               n, part.p, part.p + nb);
-            double rr = 0, rho2 = 0;
-            SGLD_TRY(reduce2(&rr, &rho2));
-            if (std::sqrt(rr) / normB < tol) break;
-            hipLaunchKernelGGL(k_pde_axpby, dim3(nb), dim3(256), 0, C->s, p.p, (const double *)z.p, rho2 / rho, n);
-            rho = rho2;
+        // batches of iterations with the scalars on the device; the host
+        // looks at the control block once per batch (through the context's
+        // pinned mirror)
+        static_assert(sizeof(PCtl) <= sizeof(Ctl), "PCtl travels through the Ctl mirror");
+        PCtl *hc = reinterpret_cast<PCtl *>(C->hctl);
+        *hc = PCtl{rho, 0.0, 0.0, normB, tol, 0, maxIT, 0, 0};
+        PCtl *dcp = dctl;
+        SGLD_HIP(hipMemcpyAsync(dcp, hc, sizeof(PCtl), hipMemcpyHostToDevice, C->s));
+        constexpr int kBatch = 8;
+        for (;;) {
+            for (int i = 0; i < kBatch; ++i) {
+                hipLaunchKernelGGL(k_pde_kbyu, dim3(nb), dim3(256), 0, C->s, p.p, C->efwd.p, gd, K, q.p, part.p,
+                                   (const PCtl *)dcp);
+                hipLaunchKernelGGL(k_pde_fin_iter, dim3(1), dim3(256), 0, C->s, (const double *)part.p, nb, 0, dcp);
+                hipLaunchKernelGGL(k_pde_update, dim3(nb), dim3(256), 0, C->s, x.p, r.p, z.p, p.p, q.p, dinv.p, 0.0,
+                                   n, part.p, part.p + nb, (const PCtl *)dcp);
+                hipLaunchKernelGGL(k_pde_fin_iter, dim3(1), dim3(256), 0, C->s, (const double *)part.p, nb, 1, dcp);
+                hipLaunchKernelGGL(k_pde_axpby, dim3(nb), dim3(256), 0, C->s, p.p, (const double *)z.p, n,
+                                   (const PCtl *)dcp);
+            }
+            SGLD_HIP(hipGetLastError());
+            SGLD_HIP(hipMemcpyAsync(hc, dcp, sizeof(PCtl), hipMemcpyDeviceToHost, C->s));
+            SGLD_HIP(hipStreamSynchronize(C->s));
+            if (hc->done) break;
         }
-        if (its > maxIT) its = maxIT;
+        its = hc->its;
     }
     hipLaunchKernelGGL(k_pde_n2e, dim3(grid_blocks(ne)), dim3(256), 0, C->s, x.p, gd,
                        C->full ? nullptr : C->eleMap.p, ne, dtar);
