@@ -110,16 +110,23 @@ std::vector<double> cone_weights(double rMin, int R) {
     return w;
 }
 
-// Filter with the voxel map on the device; x_dev null computes Hs.
+// Filter with the voxel map on the device; x_dev null computes Hs.  With a
+// context (`timed`) the lattice copy of x and the weights live in the
+// context's work buffer, allocated on first use and kept.
 int run_filter(hipStream_t s, const double *x_dev, double rMin, const int32_t *efwd_dev, int resX, int resY,
                int resZ, double *Y_dev, sgld_ctx *timed = nullptr) {
     const int R = (int)std::ceil(rMin) - 1;
     SGLD_ARG(R >= 0 && R <= 32, "sgld:arg: rMin must be in (0, 33]");
     const int64_t nv = (int64_t)resX * resY * resZ;
     std::vector<double> w = cone_weights(rMin, R);
-    DevBuf<double> dw, xl;
-    SGLD_TRY(dw.upload(w.data(), w.size()));
-    SGLD_TRY(xl.alloc(nv));
+    struct View {
+        double *p;
+    };
+    DevBuf<double> own;
+    DevBuf<double> &buf = timed ? timed->filterWork : own;
+    if (buf.n < (size_t)nv + w.size()) SGLD_TRY(buf.alloc((size_t)nv + w.size()));
+    View xl{buf.p}, dw{buf.p + nv};
+    SGLD_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_filter_scatter, dim3(bounded_blocks(nv, 256)), dim3(256), 0, s, x_dev, efwd_dev, nv, xl.p);
     if (timed) SGLD_HIP(hipEventRecord(timed->e0, s));
     if (R <= kFRmax) {

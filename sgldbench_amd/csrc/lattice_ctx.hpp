@@ -77,6 +77,7 @@ struct sgld_ctx {
     sgld::DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
     sgld::DevBuf<double> partial;
     sgld::DevBuf<double> pdeWork;  // PDE filter work vectors, allocated on first use and kept
+    sgld::DevBuf<double> filterWork;  // density filter: lattice copy of x and the weights, kept
     sgld::DevBuf<sgld::Ctl> ctl;
     sgld::Ctl *hctl = nullptr;  // pinned host mirror
     hipStream_t s = nullptr;
