@@ -942,6 +942,29 @@ __global__ void __launch_bounds__(256) k_residual_norms(const double *__restrict
     }
 }
 
+// The same for a reliable update of the mixed solve: the residual goes
+// straight to the FP32 vector, scaled by invScale (the solve's fixed power of
+// two), and only its norm is taken in FP64.
+__global__ void __launch_bounds__(256) k_residual_norms32(const double *__restrict__ b, const double *__restrict__ q,
+                                                          float *__restrict__ r32, double invScale,
+                                                          const uint8_t *__restrict__ nflag, Span S,
+                                                          double *__restrict__ partRR, double *__restrict__ partBB) {
+    double rr = 0.0, bb = 0.0;
+    FOR_SPAN(i, S) {
+        const uint32_t node = (uint32_t)(i / 3), a = (uint32_t)(i - 3 * (int64_t)node);
+        const double bi = b[i];
+        const double ri = (nflag[node] & (1u << a)) ? 0.0 : bi - q[i];
+        r32[i] = (float)(ri * invScale);
+        rr += ri * ri;
+        bb += bi * bi;
+    }
+    block_sum2(rr, bb);
+    if (threadIdx.x == 0) {
+        partRR[blockIdx.x] = rr;
+        partBB[blockIdx.x] = bb;
+    }
+}
+
 // x += lambda p with lambda of the last executed iteration (the deferred
 // update of a solve that stopped, or before a reliable update).
 template <class T>
@@ -1177,12 +1200,6 @@ __global__ void k_convert(To *__restrict__ dst, const Ti *__restrict__ src, doub
 template <class T>
 __global__ void k_fill(T *__restrict__ v, T val, Span S) {
     FOR_SPAN(i, S) v[i] = val;
-}
-
-// v *= f (in place)
-template <class T>
-__global__ void k_scale(T *v, T f, Span S) {
-    FOR_SPAN(i, S) v[i] *= f;
 }
 
 enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_RR, FIN_RHO };  // (lambda = rho / p.q: in k_update_r)
@@ -1867,6 +1884,7 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
 }
 
 int true_residual(sgld_ctx *C, double *res, double *normr);
+int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb);
 
 // A solve as a resumable state machine: pcg_begin enqueues and runs the set-up,
 // pcg_enqueue_batch queues `batch` iterations plus the copy of the device
@@ -1939,16 +1957,22 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     S.delta = 0.1;
     if (const char *env = std::getenv("SGLD_MIXED_DELTA")) S.delta = std::atof(env);
     if (const char *env = std::getenv("SGLD_MIXED_STRETCH")) S.maxStretch = std::max(1, std::atoi(env));
-    S.scale = normr;
+    // The FP32 vectors hold values divided by one power of two near the
+    // initial residual norm for the whole solve (exact, and FP32's range is
+    // ample for the 1e-8 decrease after it), so a reliable update replaces r
+    // without rescaling p, rho or the correction.
+    S.scale = std::ldexp(1.0, (int)std::lround(std::log2(normr)));
     S.itsAtUpdate = 0;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
                        1.0 / S.scale, V);
     hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
     SGLD_HIP(hipGetLastError());
     *C->hctl = Ctl{};
-    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
+    // the FP32 recursion's stop: its residual, relative to the stretch's
+    // start (normb, in scaled units), below delta, or at the solve's tolerance
+    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / normr);
     C->hctl->maxit = S.maxit;
-    C->hctl->normb = 1.0;
+    C->hctl->normb = normr / S.scale;
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
     SGLD_TRY(enqueue_rho<float>(C, C->r32.p, FIN_RHO0));
     hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, V);  // beta 0 x p
@@ -1992,18 +2016,20 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
         }
         return SGLD_OK;
     }
-    // Mixed: the FP32 Krylov iteration on the residual scaled to unit norm;
-    // when its recursive residual has dropped by delta (or after maxStretch
-    // iterations), fold the FP32 correction into the FP64 solution and
-    // replace the residual by the FP64 true residual (reliable update).
+    // Mixed: the FP32 Krylov iteration (values divided by the solve's fixed
+    // scale); when its recursive residual has dropped by delta since the
+    // stretch began (or after maxStretch iterations), fold the FP32
+    // correction into the FP64 solution and replace the residual by the FP64
+    // true residual (reliable update).
     S.its = st.its;
     if (!st.done && S.its - S.itsAtUpdate < S.maxStretch) return SGLD_OK;
     S.itsAtUpdate = S.its;
     hipLaunchKernelGGL(k_fold_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, (const float *)C->p32.p,
                        S.scale, V, (const Ctl *)C->ctl.p);  // with the pending x32 += lambda p32
     SGLD_HIP(hipGetLastError());
-    double res = 0, normr = 0;
-    SGLD_TRY(true_residual(C, &res, &normr));
+    double rr = 0, bb = 0;
+    SGLD_TRY(residual_to32(C, 1.0 / S.scale, &rr, &bb));
+    const double normr = std::sqrt(rr), res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     ++S.updates;
     S.res = res;
     if (res < S.tol || st.done == 2 || !(normr > 0.0)) {
@@ -2011,12 +2037,6 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
         S.done = res < S.tol ? 1 : 2;
         return SGLD_OK;
     }
-    const double f = S.scale / normr;
-    S.scale = normr;
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
-                       1.0 / S.scale, V);
-    hipLaunchKernelGGL(k_scale<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, (float)f, V);
-    SGLD_HIP(hipGetLastError());
     // rho of the iteration whose direction p holds, for beta = rho' / rho of
     // the replaced residual: after a stop that iteration's own rho; after a
     // forced update (stretch limit, every iteration of the batch complete)
@@ -2032,9 +2052,10 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     }
     *C->hctl = st;
     C->hctl->done = 0;
-    C->hctl->rho = rhoDir * f * f;
+    C->hctl->rho = rhoDir;
     C->hctl->lambda = 0.0;  // applied above
-    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / S.scale);
+    C->hctl->normb = normr / S.scale;  // the new stretch's start
+    C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / normr);
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
     // rho = z.r and beta for the replaced residual, then continue from p
     // (restart, beta = 0, only when the direction has just been restarted)
@@ -2082,6 +2103,29 @@ int pcg_solve(sgld_ctx *C, SolveState &S) {
         SGLD_HIP(hipStreamSynchronize(C->s));
         SGLD_TRY(pcg_after_batch(C, S));
     }
+    return SGLD_OK;
+}
+
+// Reliable update of the mixed solve: r32 = (b - K x) * invScale (FP64 K.u
+// and difference, FP32 store) with rr = ||b - K x||^2 and bb = ||b||^2;
+// host-synchronous.
+int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb) {
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
+    SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
+    hipLaunchKernelGGL(k_residual_norms32, dim3(nbv), dim3(256), 0, C->s, C->b.p, (const double *)C->q.p, C->r32.p,
+                       invScale, C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1));
+    SGLD_HIP(hipGetLastError());
+    std::vector<double> part(2 * kRedBlocks);
+    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    double a = 0, c = 0;
+    for (int i = 0; i < nbv; ++i) {
+        a += part[i];
+        c += part[kRedBlocks + i];
+    }
+    *rr = a;
+    *bb = c;
     return SGLD_OK;
 }
 
