@@ -290,3 +290,18 @@ def test_mixed_forced_updates_continue_direction(monkeypatch):
     assert st["outer"] >= its // 8 - 1
     assert its <= 1.25 * its0 + 8
     assert np.linalg.norm(U - U0) <= 1e-6 * np.linalg.norm(U0)
+
+
+@pytest.mark.parametrize("factor", [1e-30, 1e30])
+def test_mixed_load_magnitude(factor):
+    """The FP32 vectors of the mixed solve are divided by one power of two near
+    the initial residual norm, so loads of any magnitude FP64 holds solve to
+    the same relative accuracy, and the solution scales with the load."""
+    vox = holes(32, 16, 16, 5, nholes=3, r=3)
+    g, m, F, fixed, E = build_pair(vox, 3)
+    U1, its1, rr1 = g.pcg(F, tol=1e-8, maxIT=1000, precision=solver.PREC_MIXED)
+    U, its, rr = g.pcg(F * factor, tol=1e-8, maxIT=1000, precision=solver.PREC_MIXED)
+    print("factor %g: its %d (unit load %d), relres %.2e" % (factor, its, its1, rr), flush=True)
+    assert g.converged and rr <= 1e-8
+    assert abs(its - its1) <= 2
+    assert np.linalg.norm(U - factor * U1) <= 1e-7 * np.linalg.norm(factor * U1)
