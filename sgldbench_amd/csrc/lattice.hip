@@ -2435,8 +2435,10 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             ts.tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
             ts.tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
             ts.cap = (int)std::min<int64_t>((int64_t)ncu * C->kOcc[q], kRedBlocks);
-            // z-chunks of 8 node layers: measured best for both precisions on 256x128x128
-            SGLD_TRY(set_kchunk(C.get(), q, 8));
+            // z-chunks of 12 node layers, measured (FP64 / FP32 us, chunk 8 / 12 / 16):
+            // 256x128x128 72 / 65 / 73 and 50 / 47 / 52; 256^3 320 / 301 / 295 and
+            // 196 / 188 / 184; 125 M-DOF ellipsoids 918 / 887 / 894 and 538 / 517 / 521
+            SGLD_TRY(set_kchunk(C.get(), q, 12));
         }
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
@@ -2830,7 +2832,7 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
     SGLD_TRY(use_constants(C));
     const int variant0 = C->kbyuVariant;
-    const int czs[3] = {8, 16, 32};
+    const int czs[4] = {8, 12, 16, 32};
     double tiled[2] = {1e300, 1e300};
     int bestCz[2] = {C->ksched[0].tm.cz, C->ksched[1].tm.cz};
     if (variant0 == 1) {
