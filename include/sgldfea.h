@@ -52,6 +52,17 @@ int sgld_kbyu(const double *U, int64_t numDOFs, const int32_t *eNodMat, int64_t 
               const double *Ke, int64_t keRows, int64_t keCols, const double *E, int64_t numE,
               const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
 
+/* Replaces Solving_KbyU_MatrixFree8x8_mex
+ * (src/MEXfuncs/Solving_KbyU_MatrixFree8x8_mex.cpp:6-121), called at
+ * src/TopOpti_PDEFiltering_matrixFree.m:46 (the PDE filter's CG).  One scalar
+ * DOF per node: U, Y have numNodes entries, Ke is the 8x8 col-major element
+ * matrix (no per-element factor); colours as for sgld_kbyu, any count.
+ * Elements in no colour contribute nothing.  Two elements of one colour that
+ * share a node race on it, as they do in the reference. */
+int sgld_kbyu8x8(const double *U, int64_t numNodes, const int32_t *eNodMat, int64_t numElements,
+                 const double *Ke, int64_t keRows, int64_t keCols, const int32_t *colors,
+                 const int64_t *colorOffsets, int numColors, double *Y);
+
 /* Replaces Solving_Restriction_MatrixFree_mex
  * (src/MEXfuncs/Solving_Restriction_MatrixFree_mex.cpp:6-191), called at
  * src/Solving_RestrictResidual.m:6-8.  rF: nF x 3, RIT: 8 x m (m = 27 or 125),

@@ -34,6 +34,7 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         L.or_element_stiffness.argtypes = [C.c_double, C.c_double, _dp]
         L.or_kbyu_mex.argtypes = [_dp, _i64, _ip, _i64, _dp, _dp, _ip, _lp, _dp]
+        L.or_kbyu8x8_mex.argtypes = [_dp, _i64, _ip, _i64, _dp, _ip, _lp, C.c_int, _dp]
         L.or_restrict_mex.argtypes = [_dp, _i64, _ip, C.c_int, _dp, _ip, _i64, _i64, _i64,
                                       _ip, _dp, _ip, _lp, _dp]
         L.or_interpolate_mex.argtypes = [_dp, _i64, _ip, C.c_int, _dp, _ip, _i64, _i64, _i64,
@@ -118,6 +119,21 @@ def kbyu_mex(U, eNodMat, Ke, E, colors):
     Y = np.zeros_like(U)
     lib().or_kbyu_mex(U, U.size, eN, eNodMat.shape[0], f64(np.asarray(Ke).ravel(order="F")),
                       f64(E), flat, off, Y)
+    return Y
+
+
+def kbyu8x8_mex(U, eNodMat, Ke, colors):
+    """Solving_KbyU_MatrixFree8x8_mex(U, eNodMat, Ke, colors): any number of colours."""
+    U = f64(U)
+    eN = i32(np.asfortranarray(eNodMat).ravel(order="F"))
+    off = np.zeros(len(colors) + 1, dtype=np.int64)
+    for i, c in enumerate(colors):
+        off[i + 1] = off[i] + len(c)
+    flat = i32(np.concatenate([np.asarray(c, dtype=np.int32).ravel() for c in colors])
+               if off[-1] else np.zeros(1))
+    Y = np.zeros_like(U)
+    lib().or_kbyu8x8_mex(U, U.size, eN, eNodMat.shape[0], f64(np.asarray(Ke).ravel(order="F")),
+                         flat, off, len(colors), Y)
     return Y
 
 

@@ -150,6 +150,34 @@ void or_kbyu_mex(const double *U, int64_t numDOFs, const int32_t *eNodMat,
     }
 }
 
+/* Solving_KbyU_MatrixFree8x8_mex.cpp:50-89 with apply_Ke_Ue_8 (:92-121, the
+ * column-sweep branch): one scalar DOF per node, one uniform 8x8 Ke, no
+ * per-element factor; elements in no colour contribute nothing. */
+void or_kbyu8x8_mex(const double *U, int64_t numNodes, const int32_t *eNodMat,
+                    int64_t numElements, const double *Ke, const int32_t *colors,
+                    const int64_t *colorOff, int numColors, double *Y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < numNodes; ++i) Y[i] = 0.0;
+    for (int c = 0; c < numColors; ++c) {
+        const int64_t n = colorOff[c + 1] - colorOff[c];
+        const int32_t *ids = colors + colorOff[c];
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t e = ids[k] - 1;
+            int64_t nod[8];
+            double Ue[8], Ye[8];
+            for (int j = 0; j < 8; ++j) nod[j] = eNodMat[e + j * numElements] - 1;
+            for (int j = 0; j < 8; ++j) Ue[j] = U[nod[j]];
+            for (int j = 0; j < 8; ++j) Ye[j] = 0.0;
+            for (int kk = 0; kk < 8; ++kk) {
+                const double uk = Ue[kk];
+                for (int j = 0; j < 8; ++j) Ye[j] += Ke[kk * 8 + j] * uk;
+            }
+            for (int j = 0; j < 8; ++j) Y[nod[j]] += Ye[j];
+        }
+    }
+}
+
 /* Solving_Restriction_MatrixFree_mex.cpp:63-132 (efficiency-prior branch). */
 void or_restrict_mex(const double *rF, int64_t nF, const int32_t *transferMat,
                      int m, const double *RIT, const int32_t *eNodMatC,

@@ -27,6 +27,13 @@ void or_kbyu_mex(const double *U, int64_t numDOFs, const int32_t *eNodMat,
                  int64_t numElements, const double *Ke, const double *E,
                  const int32_t *colors, const int64_t *colorOff, double *Y);
 
+/* Solving_KbyU_MatrixFree8x8_mex.cpp:6-121 (called at
+ * TopOpti_PDEFiltering_matrixFree.m:46).  U, Y: numNodes; Ke: 8x8 col-major;
+ * colors: concatenated 1-based element ids of numColors colours. */
+void or_kbyu8x8_mex(const double *U, int64_t numNodes, const int32_t *eNodMat,
+                    int64_t numElements, const double *Ke, const int32_t *colors,
+                    const int64_t *colorOff, int numColors, double *Y);
+
 /* Solving_Restriction_MatrixFree_mex.cpp:6-191. rF: nF x 3 col-major, Y: nC x 3. */
 void or_restrict_mex(const double *rF, int64_t nF, const int32_t *transferMat,
                      int m, const double *RIT, const int32_t *eNodMatC,

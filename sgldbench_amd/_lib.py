@@ -22,6 +22,7 @@ SIGNATURES = {
     "sgld_last_error": [],
     "sgld_version": [],
     "sgld_kbyu": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _dp, _i64, _ip, _lp, C.c_int, _dp],
+    "sgld_kbyu8x8": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _lp, C.c_int, _dp],
     "sgld_restrict": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,
                       _ip, _lp, C.c_int, _dp],
     "sgld_interpolate": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,

@@ -1,4 +1,5 @@
-// Stateless replacements of the six MEX kernels on the reference's solve path.
+// Stateless replacements of the six MEX kernels on the reference's solve path
+// and of the PDE filter's scalar K.u (Solving_KbyU_MatrixFree8x8_mex).
 //
 // They take the same arrays as the MEX calls (1-based int32 ids, column-major
 // matrices) as host pointers, validate every index on the host before any
@@ -54,6 +55,39 @@ __global__ void __launch_bounds__(256) k_kbyu_colour(const double *__restrict__ 
     for (int j = 0; j < 8; ++j)
 #pragma unroll
         for (int a = 0; a < 3; ++a) Y[3 * nod[j] + a] += Ye[3 * j + a];
+}
+
+// --- scalar K.u of the PDE filter (Solving_KbyU_MatrixFree8x8_mex.cpp:50-121)
+// One DOF per node, one uniform 8x8 Ke, no element factor.
+__global__ void __launch_bounds__(256) k_kbyu8_colour(const double *__restrict__ U,
+                                                      const int32_t *__restrict__ eNodMat,
+                                                      int64_t numElements,
+                                                      const double *__restrict__ Ke,
+                                                      const int32_t *__restrict__ ids, int64_t n,
+                                                      double *__restrict__ Y) {
+#pragma clang fp contract(off)
+    __shared__ double sK[64];
+    if (threadIdx.x < 64) sK[threadIdx.x] = Ke[threadIdx.x];
+    __syncthreads();
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t e = ids[k] - 1;
+    int64_t nod[8];
+    double Ue[8], Ye[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) nod[j] = (int64_t)eNodMat[e + j * numElements] - 1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Ue[j] = U[nod[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Ye[j] = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        const double uk = Ue[kk];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Ye[j] += sK[kk * 8 + j] * uk;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Y[nod[j]] += Ye[j];
 }
 
 // --- restriction (Solving_Restriction_MatrixFree_mex.cpp:72-131) -----------
@@ -312,10 +346,10 @@ extern "C" int sgld_kbyu(const double *U, int64_t numDOFs, const int32_t *eNodMa
     DevBuf<int32_t> dN, dC;
     SGLD_TRY(dU.upload(U, numDOFs));
     SGLD_TRY(dK.upload(Ke, 576));
-    SGLD_TRY(dE.upload(E, std::max<int64_t>(numE, 1)));
-    SGLD_TRY(dN.upload(eNodMat, std::max<int64_t>(8 * numElements, 1)));
+    SGLD_TRY(dE.upload(E, numE));
+    SGLD_TRY(dN.upload(eNodMat, 8 * numElements));
     const int64_t nIds = numColors ? colorOffsets[numColors] : 0;
-    SGLD_TRY(dC.upload(colors, std::max<int64_t>(nIds, 1)));
+    SGLD_TRY(dC.upload(colors, nIds));
     SGLD_TRY(dY.alloc(numDOFs));
     SGLD_TRY(dY.zero());
     for (int c = 0; c < numColors; ++c) {
@@ -326,6 +360,40 @@ extern "C" int sgld_kbyu(const double *U, int64_t numDOFs, const int32_t *eNodMa
         SGLD_HIP(hipGetLastError());
     }
     SGLD_HIP(hipMemcpy(Y, dY.p, sizeof(double) * numDOFs, hipMemcpyDeviceToHost));
+    return SGLD_OK;
+}
+
+extern "C" int sgld_kbyu8x8(const double *U, int64_t numNodes, const int32_t *eNodMat,
+                            int64_t numElements, const double *Ke, int64_t keRows,
+                            int64_t keCols, const int32_t *colors, const int64_t *colorOffsets,
+                            int numColors, double *Y) {
+    SGLD_ARG(keRows == 8 && keCols == 8,
+             "MATLAB:mexFunction:inputNot8x8: Input Ke must be a 8x8 double matrix.");
+    SGLD_ARG(Ke, "MATLAB:mexFunction:inputNot8x8: Ke is null");
+    SGLD_ARG(numNodes >= 0 && (numNodes == 0 || (U && Y)),
+             "MATLAB:mexFunction:inputNotVector: Input U must be a column vector.");
+    SGLD_ARG(numElements >= 0 && (numElements == 0 || eNodMat),
+             "MATLAB:mexFunction:inputNotMatrix: Input eNodMat must be an Mx8 int32 matrix.");
+    SGLD_TRY(check_ids(eNodMat, 8 * numElements, numNodes, "eNodMat"));
+    SGLD_TRY(check_colours(colors, colorOffsets, numColors, numElements));
+    if (numNodes == 0) return SGLD_OK;
+    DevBuf<double> dU, dK, dY;
+    DevBuf<int32_t> dN, dC;
+    SGLD_TRY(dU.upload(U, numNodes));
+    SGLD_TRY(dK.upload(Ke, 64));
+    SGLD_TRY(dN.upload(eNodMat, 8 * numElements));
+    const int64_t nIds = numColors ? colorOffsets[numColors] : 0;
+    SGLD_TRY(dC.upload(colors, nIds));
+    SGLD_TRY(dY.alloc(numNodes));
+    SGLD_TRY(dY.zero());
+    for (int c = 0; c < numColors; ++c) {
+        const int64_t n = colorOffsets[c + 1] - colorOffsets[c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_kbyu8_colour, dim3(sgld::blocks_for(n, 256)), dim3(256), 0, 0, dU.p,
+                           dN.p, numElements, dK.p, dC.p + colorOffsets[c], n, dY.p);
+        SGLD_HIP(hipGetLastError());
+    }
+    SGLD_HIP(hipMemcpy(Y, dY.p, sizeof(double) * numNodes, hipMemcpyDeviceToHost));
     return SGLD_OK;
 }
 
@@ -365,12 +433,12 @@ extern "C" int sgld_restrict(const double *rF, int64_t nF, const int32_t *transf
     DevBuf<int32_t> dT, dN, dM, dC;
     SGLD_TRY(drF.upload(rF, 3 * nF));
     SGLD_TRY(dR.upload(RIT, 8 * m));
-    SGLD_TRY(dCo.upload(coeffi, std::max<int64_t>(nGhost, 1)));
-    SGLD_TRY(dT.upload(transferMat, std::max<int64_t>(m * numElesC, 1)));
-    SGLD_TRY(dN.upload(eNodMatC, std::max<int64_t>(8 * numElesC, 1)));
-    SGLD_TRY(dM.upload(map, std::max<int64_t>(nF, 1)));
+    SGLD_TRY(dCo.upload(coeffi, nGhost));
+    SGLD_TRY(dT.upload(transferMat, m * numElesC));
+    SGLD_TRY(dN.upload(eNodMatC, 8 * numElesC));
+    SGLD_TRY(dM.upload(map, nF));
     const int64_t nIds = numColors ? off[numColors] : 0;
-    SGLD_TRY(dC.upload(colors, std::max<int64_t>(nIds, 1)));
+    SGLD_TRY(dC.upload(colors, nIds));
     SGLD_TRY(dG.alloc(3 * std::max<int64_t>(nGhost, 1)));
     SGLD_TRY(dG.zero());
     SGLD_TRY(dY.alloc(3 * nC));
@@ -406,14 +474,14 @@ extern "C" int sgld_interpolate(const double *xC, int64_t nC, const int32_t *tra
     if (nF == 0) return SGLD_OK;
     DevBuf<double> dX, dR, dCo, dG, dY;
     DevBuf<int32_t> dT, dN, dM, dC;
-    SGLD_TRY(dX.upload(xC, std::max<int64_t>(3 * nC, 1)));
+    SGLD_TRY(dX.upload(xC, 3 * nC));
     SGLD_TRY(dR.upload(RI, 8 * m));
-    SGLD_TRY(dCo.upload(coeffi, std::max<int64_t>(nGhost, 1)));
-    SGLD_TRY(dT.upload(transferMat, std::max<int64_t>(m * numElesC, 1)));
-    SGLD_TRY(dN.upload(eNodMatC, std::max<int64_t>(8 * numElesC, 1)));
+    SGLD_TRY(dCo.upload(coeffi, nGhost));
+    SGLD_TRY(dT.upload(transferMat, m * numElesC));
+    SGLD_TRY(dN.upload(eNodMatC, 8 * numElesC));
     SGLD_TRY(dM.upload(map, nF));
     const int64_t nIds = numColors ? off[numColors] : 0;
-    SGLD_TRY(dC.upload(colors, std::max<int64_t>(nIds, 1)));
+    SGLD_TRY(dC.upload(colors, nIds));
     SGLD_TRY(dG.alloc(3 * std::max<int64_t>(nGhost, 1)));
     SGLD_TRY(dG.zero());
     SGLD_TRY(dY.alloc(3 * nF));
@@ -457,7 +525,7 @@ extern "C" int sgld_assemble_finest(const double *Ke, int64_t keRows, int64_t ke
     DevBuf<double> dH, dE, dKs;
     DevBuf<int32_t> dU;
     SGLD_TRY(dH.upload(H.data(), H.size()));
-    SGLD_TRY(dE.upload(E, std::max<int64_t>(numE, 1)));
+    SGLD_TRY(dE.upload(E, numE));
     SGLD_TRY(dU.upload(upMap, nq * nC));
     SGLD_TRY(dKs.alloc(576 * nC));
     hipLaunchKernelGGL(k_assemble_finest, dim3(sgld::bounded_blocks(576 * nC, 256)), dim3(256), 0, 0,
@@ -485,7 +553,7 @@ extern "C" int sgld_assemble_nonfinest(const double *KsPrev, int64_t numPrev,
     }
     DevBuf<double> dKp, dP, dKs;
     DevBuf<int32_t> dU;
-    SGLD_TRY(dKp.upload(KsPrev, std::max<int64_t>(576 * numPrev, 1)));
+    SGLD_TRY(dKp.upload(KsPrev, 576 * numPrev));
     SGLD_TRY(dP.upload(P.data(), P.size()));
     SGLD_TRY(dU.upload(upMap, nq * nC));
     SGLD_TRY(dKs.alloc(576 * nC));

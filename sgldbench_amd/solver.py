@@ -60,6 +60,19 @@ def Solving_KbyU_MatrixFree_mex(U, eNodMat, Ke, E, colors):
     return Y
 
 
+def Solving_KbyU_MatrixFree8x8_mex(U, eNodMat, Ke, colors):
+    """src/MEXfuncs/Solving_KbyU_MatrixFree8x8_mex.cpp:6-121 (one DOF per node)."""
+    U = _f64(U).ravel()
+    eNodMat = np.asarray(eNodMat)
+    Ke = np.asarray(Ke, dtype=np.float64)
+    flat, off, nc = _colors(colors)
+    Y = np.zeros_like(U)
+    kr, kc = (Ke.shape + (1,))[:2] if Ke.ndim == 2 else (Ke.size, 1)
+    check(lib().sgld_kbyu8x8(U, U.size, _fortran_i32(eNodMat), eNodMat.shape[0], _fortran_f64(Ke),
+                             kr, kc, flat, off, nc, Y))
+    return Y
+
+
 def Solving_Restriction_MatrixFree_mex(rF, transferMat, RIT, eNodMat, numNodesCoarse,
                                        numNodesFineGhost, solidNodeMapCoarser2Finer,
                                        transferMatCoeffi, colors):

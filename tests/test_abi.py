@@ -54,6 +54,24 @@ def test_kbyu_rejects_out_of_range_node_and_colour_ids():
     assert "colors" in str(e.value)
 
 
+@pytest.mark.parametrize("shape", [(24, 24), (8, 7)])
+def test_kbyu8x8_rejects_bad_ke(shape):
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree8x8_mex(np.zeros(8), np.arange(1, 9, dtype=np.int32)[None, :],
+                                              np.zeros(shape), [np.array([1], np.int32)])
+    assert solver.identifier(e.value) == "MATLAB:mexFunction:inputNot8x8"
+
+
+def test_kbyu8x8_rejects_out_of_range_node_and_colour_ids():
+    eN = np.arange(1, 9, dtype=np.int32)[None, :]
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree8x8_mex(np.zeros(7), eN, np.zeros((8, 8)), [np.array([1], np.int32)])
+    assert "eNodMat" in str(e.value)
+    with pytest.raises(_lib.SGLDError) as e:
+        solver.Solving_KbyU_MatrixFree8x8_mex(np.zeros(8), eN, np.zeros((8, 8)), [np.array([0], np.int32)])
+    assert "colors" in str(e.value)
+
+
 def test_restrict_rejects_bad_transfer_shapes():
     with pytest.raises(_lib.SGLDError) as e:
         solver.Solving_Restriction_MatrixFree_mex(np.zeros((8, 3)), np.ones((26, 1), np.int32),

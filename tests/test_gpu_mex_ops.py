@@ -34,6 +34,23 @@ def test_kbyu_bitexact(mesh):
     assert np.array_equal(solver.Solving_KbyU_MatrixFree_mex(*args), orc.kbyu_mex(*args))
 
 
+@pytest.mark.parametrize("ke", ["pde", "random"])
+def test_kbyu8x8_bitexact(mesh, ke):
+    info = mesh.info[0]
+    rng = np.random.default_rng(11)
+    Ke = orc.pde_element(8.0) if ke == "pde" else rng.standard_normal((8, 8))
+    U = rng.standard_normal(info["numNodes"])
+    cols = mesh.colors(1)
+    print("case: kbyu8x8", ke, info, flush=True)
+    args = (U, mesh.get(1, "eNodMat"), Ke, cols)
+    assert np.array_equal(solver.Solving_KbyU_MatrixFree8x8_mex(*args), orc.kbyu8x8_mex(*args))
+    # 17 cells (each colour split in two, an empty one), one element in none
+    split = [h for c in cols for h in np.array_split(c, 2)] + [np.array([], np.int32)]
+    split[0] = split[0][1:]
+    args = (U, mesh.get(1, "eNodMat"), Ke, split)
+    assert np.array_equal(solver.Solving_KbyU_MatrixFree8x8_mex(*args), orc.kbyu8x8_mex(*args))
+
+
 def test_restrict_interpolate_bitexact(mesh):
     rng = np.random.default_rng(2)
     for ii in range(2, mesh.num_levels + 1):

@@ -111,6 +111,51 @@ def test_kbyu_mex_vs_scipy_global_K():
     assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+def scalar_global_K(Ke, eNodMat, num_nodes, elements):
+    """Assembled sum over the given 0-based elements of scatter(Ke), one DOF per node."""
+    n = eNodMat[elements].astype(np.int64) - 1
+    rows = np.repeat(n, 8, axis=1).ravel()
+    cols = np.tile(n, (1, 8)).ravel()
+    vals = np.tile(np.asarray(Ke).ravel(), len(elements))
+    return brute.sp.csr_matrix((vals, (rows, cols)), shape=(num_nodes, num_nodes))
+
+
+@pytest.mark.parametrize("ke", ["pde", "random"])
+def test_kbyu8x8_mex_vs_scipy_global_K(ke):
+    """Non-symmetric Ke catches a transposed operand; elements left out of
+    every colour must contribute nothing; the colour count is free."""
+    vox = random_voxels((6, 8, 4), 0.7, 4)
+    m = orc.Mesh(vox, 2)
+    info = m.info[0]
+    eN = m.get(1, "eNodMat")
+    rng = np.random.default_rng(5)
+    Ke = orc.pde_element(8.0) if ke == "pde" else rng.standard_normal((8, 8))
+    cols = [c[c != 3] for c in m.colors(1)]  # element 3 in no colour
+    kept = np.concatenate(cols) - 1
+    u = rng.standard_normal(info["numNodes"])
+    y = orc.kbyu8x8_mex(u, eN, Ke, cols)
+    ref = scalar_global_K(Ke, eN, info["numNodes"], kept) @ u
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    # any colour count: each colour split in three cells (still independent
+    # sets, so every node sums in the same order) gives the same bits
+    split = [h for c in cols for h in np.array_split(c, 3)]
+    assert np.array_equal(orc.kbyu8x8_mex(u, eN, Ke, split), y)
+
+
+def test_kbyu8x8_mex_pde_element_mass_and_constant():
+    """PDE filter matrix on a constant field: the gradient part vanishes and
+    the mass part integrates to the cell volume, so sum(K 1) = #elements and
+    K 1 at a node = (number of its elements) / 8."""
+    vox = random_voxels((4, 6, 6), 0.6, 6)
+    m = orc.Mesh(vox, 2)
+    info = m.info[0]
+    eN = m.get(1, "eNodMat")
+    y = orc.kbyu8x8_mex(np.ones(info["numNodes"]), eN, orc.pde_element(3.0), m.colors(1))
+    assert abs(y.sum() - info["numElements"]) <= 1e-12 * info["numElements"]
+    valence = np.bincount(eN.ravel() - 1, minlength=info["numNodes"])
+    assert np.allclose(y, valence / 8.0, rtol=0, atol=1e-14)
+
+
 def test_colours_are_independent_sets():
     vox = random_voxels((8, 6, 4), 0.8, 3)
     m = orc.Mesh(vox, 2)
