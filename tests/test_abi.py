@@ -72,6 +72,11 @@ def test_kbyu8x8_rejects_out_of_range_node_and_colour_ids():
     assert "colors" in str(e.value)
 
 
+def test_kbyu8x8_zero_nodes_returns_empty():
+    y = solver.Solving_KbyU_MatrixFree8x8_mex(np.zeros(0), np.zeros((0, 8), np.int32), np.zeros((8, 8)), [])
+    assert y.shape == (0,)
+
+
 def test_restrict_rejects_bad_transfer_shapes():
     with pytest.raises(_lib.SGLDError) as e:
         solver.Solving_Restriction_MatrixFree_mex(np.zeros((8, 3)), np.ones((26, 1), np.int32),

@@ -104,3 +104,28 @@ def test_empty_colour_cells_and_zero_elements():
                                            [np.array([], np.int32)] * 6)
     assert np.array_equal(y, orc.kbyu_mex(U, eN, Ke, np.array([2.0]), [np.array([1], np.int32)] +
                                           [np.array([], np.int32)] * 7))
+
+
+def test_null_arrays_with_zero_elements():
+    """No elements and no colours: the id and colour arrays may be null and
+    nothing is read from them; Y = 0."""
+    import ctypes as C
+    from sgldbench_amd import _lib
+    _lib.lib()  # binds to torch's HIP runtime first, as the library loader does
+    L = C.CDLL(_lib.LIB_PATH)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    vp, i64 = C.c_void_p, C.c_int64
+    k8 = L.sgld_kbyu8x8
+    k8.argtypes = [dp, i64, vp, i64, dp, i64, i64, vp, vp, C.c_int, dp]
+    k8.restype = C.c_int
+    y = np.full(8, 7.0)
+    print("case: kbyu8x8 with null arrays", flush=True)
+    assert k8(np.ones(8), 8, None, 0, np.eye(8), 8, 8, None, None, 0, y) == 0
+    assert np.all(y == 0)
+    k24 = L.sgld_kbyu
+    k24.argtypes = [dp, i64, vp, i64, dp, i64, i64, vp, i64, vp, vp, C.c_int, dp]
+    k24.restype = C.c_int
+    y = np.full(24, 7.0)
+    print("case: kbyu with null arrays", flush=True)
+    assert k24(np.ones(24), 24, None, 0, np.eye(24), 24, 24, None, 0, None, None, 0, y) == 0
+    assert np.all(y == 0)
