@@ -1,10 +1,14 @@
 """Benchmark: MG-PCG time to rel-res 1e-8 and K.u GDOF/s on one MI355X.
 
-Workload: 256x128x128 voxel cuboid (12,830,211 DOF), SIMP modulus from a
-seeded Gaussian random field, cantilever boundary conditions, 6 multigrid
-levels (span 2), weighted Jacobi omega 0.6, tolerance 1e-8.  One step = one
-full solve from U = 0 to FP64 true relative residual 1e-8, with F and U
-resident in HBM.  Prints one JSON line.
+Default workload: the ~125 M-DOF irregular domain (union of 24 seeded
+ellipsoids in a 512^3 voxel box, fill 0.30; nodes of voxels below z = 8
+fixed; 2000 unit loads on surface nodes above z = 400; E = 1; 7 multigrid
+levels, span 2; weighted Jacobi omega 0.6; tolerance 1e-8), the largest
+single-GPU configuration.  One step = one full solve from U = 0 to FP64
+true relative residual 1e-8 with F and U resident in HBM; the line's value
+is the mixed-precision solve (FP32 inner PCG, V-cycle and K.u; FP64 reliable
+updates, scalars and coarsest solve), with the plain FP64 solve of the same
+problem beside it.  Prints one JSON line.
 """
 import argparse
 import json
@@ -25,8 +29,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "part_r256"],
-                    help="cfg1: 256x128x128 GRF cuboid (default); cfg2: ~125 M-DOF ellipsoid union in 512^3; "
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "part_r256"],
+                    help="cfg2: ~125 M-DOF ellipsoid union in 512^3 (default); cfg0: 64x32x32 solid cuboid; "
+                         "cfg1: 256x128x128 GRF cuboid; "
                          "cfg3: 16 load cases on a 256^3 GRF cuboid, batched mixed vs 16 sequential FP64 "
                          "solves; cfg4: 32 uneven cuboids in one replayed graph plus sensitivities; "
                          "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
@@ -36,8 +41,14 @@ def parse():
                          "default: the runtime's (4, measured fastest: 0.126 s vs 0.187 s with 8, 0.175 s with 16)")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
-    ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample")
+    ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample (1 above 50 M DOF)")
+    ap.add_argument("--cpu-full", type=int, default=0, help="also time this many whole oracle solves (median)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ref-case", action="store_true",
+                    help="also time the 64x32x32 reference case (GPU and oracle whole solves) beside the workload")
+    ap.add_argument("--probe-its", type=int, default=0,
+                    help="only a fixed-length solve of this many iterations per precision plus a 256 MiB "
+                         "torch copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--kbyu-reps", type=int, default=100, help="timed back-to-back K.u launches")
     ap.add_argument("--kbyu-warmup", type=int, default=300,
                     help="untimed K.u launches first: after idle time the FP64 kernel runs a ~30 ms "
@@ -45,21 +56,6 @@ def parse():
     ap.add_argument("--kbyu-only", action="store_true",
                     help="only K.u launches plus a 256 MiB torch copy (for rocprofv3 --pmc runs)")
     return ap.parse_args()
-
-
-TRAFFIC_MD = os.path.join(ROOT, "profiles", "kbyu_traffic_cuboid256x128x128.md")
-
-
-def measured_traffic():
-    """HBM bytes per FP64 K.u launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    try:
-        for line in open(TRAFFIC_MD):
-            if line.startswith("traffic_bytes_per_launch_fp64:"):
-                return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
 
 
 def measured_copy_GBs(reps=10):
@@ -136,36 +132,69 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(p, its_fp64, t_gpu_unused=None, sample_its=3):
+def cpu_threads():
+    """Threads the oracle runs with (OMP_NUM_THREADS, else the CPUs this
+    process may run on) and that affinity count."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return int(os.environ.get("OMP_NUM_THREADS", aff)), aff
+
+
+def cpu_baseline(p, its_fp64, sample_its=3, one_thread=True, lean=False, full_solves=0):
     """Oracle (C/OpenMP restatement of the reference algorithm) on the host:
     `sample_its` FP64 PCG iterations of the same workload with OMP_NUM_THREADS
-    threads bound close, extrapolated to the GPU FP64 iteration count; one
-    more iteration on one thread beside it."""
+    threads bound close (after its set-up and the initial z = M r), scaled to
+    the GPU FP64 iteration count; optionally one iteration on one thread and
+    `full_solves` complete solves (median)."""
     os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime starts
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads, aff = cpu_threads()
     orc.set_threads(threads)
     t0 = time.time()
     m = orc.Mesh(p["voxels"], p["num_levels"])
     m.set_fixed(p["fixed"])
     m.set_modulus(p["E"])
-    m.assemble()
+    m.assemble(lean=lean)
     t_setup = time.time() - t0
+    progress("oracle set-up %.1f s (%d threads); timing %d iteration(s)" % (t_setup, threads, sample_its))
     t_it, t_k, t_v = m.time_pcg_iterations(p["F"], p["omega"], sample_its)
+    progress("oracle: %.2f s per iteration" % (t_it / sample_its))
     per_it = t_it / sample_its
-    orc.set_threads(1)
-    t1, t1k, t1v = m.time_pcg_iterations(p["F"], p["omega"], 1)
-    orc.set_threads(threads)
     n = p["F"].size
     t_solve = per_it * its_fp64
-    return {"value": n / t_solve / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
-            "sample": "%d FP64 PCG iterations of the same workload on the oracle "
-                      "(%.2f s/iteration: K.u %.2f s, V-cycle %.2f s; setup %.1f s), "
-                      "extrapolated to %d iterations" % (sample_its, per_it, t_k / sample_its,
-                                                         t_v / sample_its, t_setup, its_fp64),
-            "solve_s_est": t_solve, "cpu": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "one_thread": {"s_per_iteration": t1, "kbyu_s": t1k, "vcycle_s": t1v,
-                           "solve_s_est": t1 * its_fp64}}
+    out = {"value": n / t_solve / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
+           "sample": "%d FP64 PCG iteration(s) of the same workload on the oracle, %d threads "
+                     "(%.2f s/iteration: K.u %.2f s, V-cycle %.2f s; set-up %.1f s not counted), "
+                     "scaled to the GPU's %d FP64 iterations" % (sample_its, threads, per_it, t_k / sample_its,
+                                                                 t_v / sample_its, t_setup, its_fp64),
+           "solve_s_est": t_solve, "s_per_iteration": per_it, "kbyu_s_per_iteration": t_k / sample_its,
+           "vcycle_s_per_iteration": t_v / sample_its, "setup_s": t_setup,
+           "cpu": cpu_model(), "affinity_cpus": aff, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "lean_level2_storage": bool(lean)}
+    if one_thread:
+        orc.set_threads(1)
+        t1, t1k, t1v = m.time_pcg_iterations(p["F"], p["omega"], 1)
+        orc.set_threads(threads)
+        out["one_thread"] = {"s_per_iteration": t1, "kbyu_s": t1k, "vcycle_s": t1v, "solve_s_est": t1 * its_fp64}
+    if full_solves:
+        ts, its = [], None
+        for _ in range(full_solves + 1):  # the first is a warm-up
+            t = time.time()
+            _, its, _, rc = m.pcg(p["F"], p["omega"], p["tol"], p.get("max_it", 500))
+            ts.append(time.time() - t)
+        med = float(np.median(ts[1:]))
+        out["full_solves"] = {"count": full_solves, "median_s": med, "all_s": ts[1:], "its": its,
+                              "converged": rc == 0, "gdofs": n / med / 1e9}
+    return out
+
+
+def provenance(p):
+    """SHA-256 (first 16 hex digits) of the generated workload arrays."""
+    import hashlib
+    h = {}
+    for k in ("voxels", "E", "F", "fixed"):
+        a = np.ascontiguousarray(np.asarray(p[k]))
+        h[k] = hashlib.sha256(a.tobytes()).hexdigest()[:16]
+    return h
 
 
 def progress(msg):
@@ -408,42 +437,118 @@ def run_batched(a):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is described in DESIGN.md, not run." % a.gpus, file=sys.stderr)
-        sys.exit(2)
-    if a.workload == "cfg3":
-        run_batched(a)
-        return
-    if a.workload == "cfg4":
-        run_multi(a)
-        return
-    import torch
-    from sgldbench_amd import problems, solver
+def kernel_bytes(key, n, n2, ne):
+    """Algorithmic bytes of one launch of a finest-level pass (each input read
+    once, each output written once; node flags and chunk lists excluded):
+    n DOFs of level 1, n2 of level 2, ne elements."""
+    s = 8 if key.endswith("64") else 4
+    dw = 8 if s == 8 else 2  # Jacobi weights: FP64, or bfloat16 in the FP32 inner solve
+    if key.startswith("dir"):       # read x_2, r, dinv, p, x; write p, x
+        return n * (5 * s + dw) + n2 * s
+    if key.startswith("kbyu"):      # read u, E; write y
+        return 2 * n * s + ne * s
+    if key.startswith("update_r"):  # read r, q, dinv; write r
+        return n * (3 * s + dw)
+    if key.startswith("restrict"):  # read r_1; write r_2
+        return (n + n2) * s
+    raise KeyError(key)
 
+
+TRAFFIC_FILES = {"cfg1": "kbyu_traffic_cuboid256x128x128.md", "cfg2": "traffic_ellipsoids512.md"}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel: lines "traffic_bytes_per_launch[<kernel>]: <bytes>"), or None."""
+    name = TRAFFIC_FILES.get(workload)
+    if not name:
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch[%s]:" % kernel):
+                return float(line.split("]:")[1])
+    except OSError:
+        pass
+    return None
+
+
+def load_workload(a):
+    """(problem dict, workload description, data description)."""
+    from sgldbench_amd import problems
     if a.workload == "cfg2":
         p = problems.cfg2()
         ny, nx, nz = p["voxels"].shape
-        a.no_cpu = True  # the oracle's per-element Galerkin storage does not fit this size on the host
-    elif a.workload == "part_r256":
+        desc = ("ellipsoid union in %dx%dx%d box (fill %.3f), %d DOF, E=1, z<8 fixed, 2000 unit loads, "
+                "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, float(p["voxels"].mean()), p["F"].size,
+                                                    p["num_levels"]))
+        return p, desc, "synthetic (seeded ellipsoid union)"
+    if a.workload == "part_r256":
         from sgldbench_amd import topvoxel
-        here = os.path.dirname(os.path.abspath(__file__))
-        p = topvoxel.setup(topvoxel.load_packed(os.path.join(here, "tests", "golden", "part_R256.npz")),
+        p = topvoxel.setup(topvoxel.load_packed(os.path.join(ROOT, "tests", "golden", "part_R256.npz")),
                            num_levels=6)
         ny, nx, nz = p["voxels"].shape
-    elif a.workload == "cfg0":  # the reference's own CPU-sized case: E = 1, 4 levels
+        desc = ("voxel model part_R256 (256x160x112, padded %dx%dx%d), %d DOF, E=1, its 13042 fixations "
+                "and 7788 loads, %d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, p["F"].size, p["num_levels"]))
+        return p, desc, "the reference's voxel model data/part_R256.TopVoxel (packed copy)"
+    if a.workload == "cfg0":  # the reference's own CPU-sized case: E = 1, 4 levels
         p = problems.cfg0()
         ny, nx, nz = p["voxels"].shape
-    else:
-        nx, ny, nz = (int(v) for v in a.size.split("x"))
-        p = problems.cfg1(nx, ny, nz, a.levels)
+        desc = ("solid cuboid %dx%dx%d voxels, %d DOF, E=1, cantilever, %d levels, omega 0.6, tol 1e-8"
+                % (nx, ny, nz, p["F"].size, p["num_levels"]))
+        return p, desc, "synthetic (solid cuboid, E = 1)"
+    nx, ny, nz = (int(v) for v in a.size.split("x"))
+    p = problems.cfg1(nx, ny, nz, a.levels)
+    desc = ("cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, %d levels, omega 0.6, tol 1e-8"
+            % (nx, ny, nz, p["F"].size, p["num_levels"]))
+    return p, desc, "synthetic (seeded Gaussian random density field)"
+
+
+def reference_case(a):
+    """The reference's CPU-runnable case (64x32x32 solid cantilever, 4
+    levels): median of 5 whole solves on the GPU (FP64 and mixed) and on the
+    oracle's host threads."""
+    import torch
+    from sgldbench_amd import problems, solver
+    p = problems.cfg0()
+    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s.set_fixed(p["fixed"])
+    s.set_modulus(p["E"])
+    s.assemble(p["omega"])
+    dF = torch.from_numpy(p["F"]).to("cuda:0")
+    dU = torch.zeros_like(dF)
+    out = {"workload": "solid cuboid 64x32x32, 212355 DOF, E=1, cantilever, 4 levels, omega 0.6, tol 1e-8"}
+    for prec, name in ((solver.PREC_FP64, "gpu_fp64"), (solver.PREC_MIXED, "gpu_mixed")):
+        ts = []
+        for _ in range(6):
+            dU.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], 500, prec)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        out[name] = {"median_ms": 1e3 * float(np.median(ts[1:])), "its": its, "relres": rr}
+    s.close()
+    if not a.no_cpu:
+        cb = cpu_baseline(p, out["gpu_fp64"]["its"], sample_its=3, one_thread=False, full_solves=5)
+        out["cpu_oracle"] = {"median_s": cb["full_solves"]["median_s"], "its": cb["full_solves"]["its"],
+                             "threads": cb["cores"], "s_per_iteration": cb["s_per_iteration"]}
+    return out
+
+
+def run_single(a):
+    """One-RHS workloads (cfg0, cfg1, cfg2, part_r256): FP64 and mixed
+    MG-PCG solves from U = 0 to FP64 true rel-res 1e-8 with F, U in HBM."""
+    import torch
+    from sgldbench_amd import solver
+    t_gen = time.time()
+    p, desc, data = load_workload(a)
     n = p["F"].size
     ne = int(p["E"].size)
+    prov = provenance(p)
+    progress("problem %s: %d DOF, %d elements (generated in %.1f s)" % (a.workload, n, ne, time.time() - t_gen))
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
-    progress("problem %s: %d DOF, %d elements" % (a.workload, n, ne))
     s = solver.FEASolver(p["voxels"], p["num_levels"])
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
@@ -453,22 +558,35 @@ def main():
     t0 = time.time()
     s.assemble(p["omega"])
     setup_s = time.time() - t0
-    progress("assembled in %.2f s" % setup_s)
+    n2 = int(s.levels[1]["numDOFs"])
+    progress("assembled in %.2f s (level 2: %d DOF)" % (setup_s, n2))
     dF = torch.from_numpy(p["F"]).to(dev)
     dU = torch.zeros(n, dtype=torch.float64, device=dev)
 
-    def solve(prec):
+    def solve(prec, max_it=5000):
         dU.zero_()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], 5000, prec)
+        its, rr = s.pcg_device(dF.data_ptr(), dU.data_ptr(), p["tol"], max_it, prec)
         torch.cuda.synchronize()
         return time.perf_counter() - t, its, rr, s.stats()
 
-    res = {}
+    if a.probe_its:  # a short, fixed-length solve per precision for counter runs
+        for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+            solve(prec, a.probe_its)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe_its": a.probe_its, "copy_bytes": src.numel() * 8, "dofs": n}), flush=True)
+        return
+
+    res, ktimes = {}, {}
     for prec, name in ((solver.PREC_FP64, "fp64"), (solver.PREC_MIXED, "mixed")):
         for _ in range(a.warmup):
             solve(prec)
+        s.kernel_timing(True)
         torch.cuda.synchronize()
         t = time.perf_counter()
         runs = []
@@ -477,33 +595,59 @@ def main():
             progress("%s solve: %.3f s, %d its, rel-res %.2e" % (name, runs[-1][0], runs[-1][1], runs[-1][2]))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
-        res[name] = dict(ms_per_step=1e3 * wall / a.steps, its=runs[-1][1], relres=max(r[2] for r in runs),
-                         outer=runs[-1][3]["outer"], converged=all(r[2] <= p["tol"] for r in runs))
+        ktimes[name] = s.kernel_times()
+        s.kernel_timing(False)
+        its = runs[-1][1]
+        res[name] = dict(ms_per_step=1e3 * wall / a.steps, its=its, relres=max(r[2] for r in runs),
+                         outer=runs[-1][3]["outer"], converged=all(r[2] <= p["tol"] for r in runs),
+                         gdofs=n / (wall / a.steps) / 1e9, gdofs_x_its=n * its / (wall / a.steps) / 1e9)
+    # the same mixed solves without the kernel events
+    plain = [solve(solver.PREC_MIXED)[0] for _ in range(2)]
+    res["mixed"]["ms_uninstrumented"] = 1e3 * float(np.mean(plain))
     U = dU.cpu().numpy()
     comp_mixed = float(p["F"] @ U)
     solve(solver.PREC_FP64)
     U64 = dU.cpu().numpy()
     comp_fp64 = float(p["F"] @ U64)
-    # K.u roofline (FP64 and FP32), hipEvent-timed on the solver stream
     ms64 = time_kbyu(a, s, solver.PREC_FP64)
     ms32 = time_kbyu(a, s, solver.PREC_MIXED)
     copy_gbs = measured_copy_GBs()
-    bytes64 = 2 * n * 8 + ne * 8
-    bytes32 = 2 * n * 4 + ne * 4
-    # density filter of the same mesh (rMin 2.5: 5^3 cone), kernel time
-    dxf = torch.rand(ne, dtype=torch.float64, device=dev)
-    dyf = torch.zeros_like(dxf)
-    fms = []
-    for _ in range(4):
-        s.density_filter_device(dxf.data_ptr(), 2.5, dyf.data_ptr())
-        fms.append(s.stats()["filter_ms"])
-    filt = {"rMin": 2.5, "ms": min(fms[1:]), "GB_s_algorithmic": 2 * ne * 8 / min(fms[1:]) / 1e6}
+    filt = None
+    if a.workload != "cfg2":  # density filter of the same mesh (rMin 2.5: 5^3 cone), kernel time
+        dxf = torch.rand(ne, dtype=torch.float64, device=dev)
+        dyf = torch.zeros_like(dxf)
+        fms = []
+        for _ in range(4):
+            s.density_filter_device(dxf.data_ptr(), 2.5, dyf.data_ptr())
+            fms.append(s.stats()["filter_ms"])
+        filt = {"rMin": 2.5, "ms": min(fms[1:]), "GB_s_algorithmic": 2 * ne * 8 / min(fms[1:]) / 1e6}
     # the line reports the faster of the converged paths (mixed on the large
     # workloads; FP64 on the 212 k-DOF case, where reliable updates and launch
     # overhead outweigh the cheaper FP32 passes)
     best = min((m for m in ("mixed", "fp64") if res[m]["converged"]), key=lambda m: res[m]["ms_per_step"],
                default="fp64")
     t_best = res[best]["ms_per_step"] / 1e3
+    # per-kernel figures of the timed solves; the roofline names the kernel
+    # with the largest share of the headline path's time
+    kern = {}
+    for path in ("fp64", "mixed"):
+        kern[path] = {}
+        for key, v in ktimes[path].items():
+            if not v["work_launches"]:
+                continue
+            avg = v["work_ms"] / v["work_launches"]
+            b = kernel_bytes(key, n, n2, ne)
+            kern[path][key] = {"kernel": v["kernel"], "launches": v["launches"], "work_launches": v["work_launches"],
+                               "avg_us": 1e3 * avg, "avg_us_all_launches": 1e3 * v["ms"] / v["launches"],
+                               "bytes_per_launch": b, "GB_s": b / avg / 1e6, "frac": b / avg / 1e6 / HBM_PEAK_GBS,
+                               "share_of_solve": v["work_ms"] / (res[path]["ms_per_step"] * a.steps)}
+    dom = max(kern[best], key=lambda k: kern[best][k]["avg_us"] * kern[best][k]["work_launches"])
+    d = kern[best][dom]
+    roof = {"bound": "hbm", "achieved": d["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+            "traffic": measured_traffic(a.workload, d["kernel"]), "kernel": d["kernel"], "path": best,
+            "bytes_per_launch": d["bytes_per_launch"], "avg_us": d["avg_us"], "launches": d["work_launches"],
+            "measured_copy_GBs": copy_gbs, "frac_of_measured": d["GB_s"] / copy_gbs}
+    bytes64, bytes32 = kernel_bytes("kbyu64", n, n2, ne), kernel_bytes("kbyu32", n, n2, ne)
     out = {
         "metric": "MG-PCG time to rel-res 1e-8 & K.u GDOF/s, FP64 vs mixed/batched, 1 MI355X",
         "value": n / t_best / 1e9,
@@ -512,42 +656,47 @@ def main():
         "ms_per_step": res[best]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if best == "mixed" else "f64",
-        "data": ("the reference's voxel model data/part_R256.TopVoxel (packed copy)" if a.workload == "part_r256"
-                 else "synthetic (seeded Gaussian random density field)" if a.workload == "cfg1"
-                 else "synthetic (solid cuboid, E = 1)" if a.workload == "cfg0"
-                 else "synthetic (seeded ellipsoid union)"),
-        "config": {"workload": ("cuboid %dx%dx%d voxels, %d DOF, SIMP E from GRF, cantilever, "
-                                "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]))
-                   if a.workload == "cfg1" else
-                   ("solid cuboid %dx%dx%d voxels, %d DOF, E=1, cantilever, %d levels, omega 0.6, tol 1e-8"
-                    % (nx, ny, nz, n, p["num_levels"]))
-                   if a.workload == "cfg0" else
-                   ("voxel model part_R256 (256x160x112, padded %dx%dx%d), %d DOF, E=1, its 13042 fixations "
-                    "and 7788 loads, %d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, n, p["num_levels"]))
-                   if a.workload == "part_r256" else
-                   ("ellipsoid union in %dx%dx%d box (fill %.3f), %d DOF, E=1, z<8 fixed, 2000 unit loads, "
-                    "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, float(p["voxels"].mean()), n, p["num_levels"])),
-                   "path": best, "dofs": n, "elements": ne},
+        "data": data,
+        "config": {"workload": desc, "path": best, "dofs": n, "elements": ne, "level2_dofs": n2,
+                   "sha256_16": prov},
         "solve": {"fp64": res["fp64"], "mixed": res["mixed"],
                   "speedup_mixed_vs_fp64": res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"],
                   "compliance_fp64": comp_fp64, "compliance_mixed": comp_mixed,
                   "compliance_rel_diff": abs(comp_mixed - comp_fp64) / abs(comp_fp64),
                   "u_rel_diff": float(np.linalg.norm(U - U64) / np.linalg.norm(U64)),
                   "setup_s": setup_s},
-        "density_filter": filt,
+        "kernels": kern,
         "kbyu": {"fp64_ms": ms64, "fp32_ms": ms32, "fp64_gdofs": n / ms64 / 1e6,
                  "fp32_gdofs": n / ms32 / 1e6, "fp64_GBs": bytes64 / ms64 / 1e6,
-                 "fp32_GBs": bytes32 / ms32 / 1e6},
-        "roofline": {"bound": "hbm", "achieved": bytes64 / ms64 / 1e6, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if (a.workload == "cfg1" and (nx, ny, nz) == (256, 128, 128)
-                                                       and p["num_levels"] == 6) else None,
-                     "kernel": "k_kbyu_tiled<double>", "bytes_per_launch": bytes64,
-                     "measured_copy_GBs": copy_gbs, "frac_of_measured": bytes64 / ms64 / 1e6 / copy_gbs},
+                 "fp32_GBs": bytes32 / ms32 / 1e6, "note": "back-to-back launches, HIP events"},
+        "roofline": roof,
     }
+    if filt:
+        out["density_filter"] = filt
+    s.close()
+    del dF, dU
+    torch.cuda.empty_cache()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=a.cpu_its)
+        big = n > 50_000_000
+        out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=1 if big else a.cpu_its,
+                                           one_thread=not big, lean=big, full_solves=a.cpu_full)
+    if a.workload != "cfg0" and a.ref_case:
+        out["reference_case"] = reference_case(a)
     print(json.dumps(out), flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
+              "multi-GPU sharding is described in DESIGN.md, not run." % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    if a.workload == "cfg3":
+        run_batched(a)
+    elif a.workload == "cfg4":
+        run_multi(a)
+    else:
+        run_single(a)
 
 
 if __name__ == "__main__":

@@ -281,6 +281,28 @@ int sgld_ctx_pde_filter_device(sgld_ctx *ctx, const double *dsrc, double rHatMin
  * SGLD_PREC_MIXED [3] counts reliable updates. */
 int sgld_ctx_stats(const sgld_ctx *ctx, double *out);
 
+/* Per-kernel timing of the finest-level passes of the following
+ * sgld_ctx_pcg[_device] solves (no counterpart in the reference, whose
+ * zTmp_Benchmarking_SpMV.m:73-100 times the SpMV and preconditioner calls
+ * from MATLAB): with on != 0 every launch of the kernels below is bracketed
+ * by HIP events on the context stream.  Resets the counters.  Kernel ids: */
+enum { SGLD_KT_DIR64 = 0, /* direction update p = z + beta p (finest V-cycle level fused), FP64 */
+       SGLD_KT_DIR32,     /* the same, FP32 inner solve of SGLD_PREC_MIXED */
+       SGLD_KT_KU64,      /* K.u, FP64 (incl. the residuals of reliable updates) */
+       SGLD_KT_KU32,      /* K.u, FP32 */
+       SGLD_KT_UPD64,     /* residual update r -= lambda q, FP64 */
+       SGLD_KT_UPD32,     /* the same, FP32 */
+       SGLD_KT_RES64,     /* first restriction of the V-cycle, FP64 */
+       SGLD_KT_RES32,     /* the same, FP32 */
+       SGLD_KT_COUNT = 8 };
+int sgld_ctx_kernel_timing(sgld_ctx *ctx, int on);
+/* out[4 k + 0..3] for kernel id k < SGLD_KT_COUNT: launches, their total
+ * milliseconds, launches that did work (not in an iteration queued after the
+ * solve's stop), their total milliseconds. */
+int sgld_ctx_kernel_times(const sgld_ctx *ctx, double *out);
+/* Name of the kernel the last timed launch of id ran ("" if none). */
+const char *sgld_ctx_kernel_name(const sgld_ctx *ctx, int id);
+
 #ifdef __cplusplus
 }
 #endif

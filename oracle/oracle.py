@@ -55,6 +55,7 @@ def lib():
         L.or_mesh_set_fixed.argtypes = [C.c_void_p, _bp]
         L.or_mesh_set_modulus.argtypes = [C.c_void_p, _dp]
         L.or_mesh_assemble.argtypes = [C.c_void_p]
+        L.or_mesh_set_lean.argtypes = [C.c_void_p, C.c_int]
         L.or_mesh_kbyu.argtypes = [C.c_void_p, _dp, _dp]
         L.or_mesh_restrict.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.or_mesh_interpolate.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -271,7 +272,9 @@ class Mesh:
     def set_modulus(self, E):
         lib().or_mesh_set_modulus(self._h, f64(E))
 
-    def assemble(self, check=True):
+    def assemble(self, check=True, lean=False):
+        """lean: keep no level-2 Galerkin matrices (same results, less memory)."""
+        lib().or_mesh_set_lean(self._h, 1 if lean else 0)
         rc = lib().or_mesh_assemble(self._h)
         if rc != 0 and check:
             raise RuntimeError("coarsest matrix not SPD")

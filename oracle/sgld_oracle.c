@@ -310,8 +310,7 @@ static void galerkin_child_add(const double p[8][8], const double *K, double sca
 
 /* Solving_AssembleCmptStencilFromFinestLevel.cpp:71-169: templates
  * H_q = P_q^T Ke P_q, then Ks_e = sum_q E_{child q} H_q. */
-void or_assemble_finest(const double *Ke, const double *E, const int32_t *upMap,
-                        int64_t nC, int s, double *Ks) {
+static double *finest_templates(const double *Ke, int s) {
     const int nq = s * s * s;
     double *H = (double *)calloc((size_t)nq * 576, sizeof(double));
     for (int q = 0; q < nq; ++q) {
@@ -319,6 +318,27 @@ void or_assemble_finest(const double *Ke, const double *E, const int32_t *upMap,
         child_prolongation(s, q, p);
         galerkin_child_add(p, Ke, 1.0, H + 576 * q);
     }
+    return H;
+}
+
+/* Level-2 matrix of coarse element i (0-based) from the templates, in the
+ * order of or_assemble_finest. */
+static void finest_element(const double *H, const double *E, const int32_t *upMap, int64_t nC,
+                           int nq, int64_t i, double *acc) {
+    for (int t = 0; t < 576; ++t) acc[t] = 0.0;
+    for (int q = 0; q < nq; ++q) {
+        int32_t child = upMap[i + q * nC];
+        if (child == 0) continue;
+        double Ev = E[child - 1];
+        const double *Hq = H + 576 * q;
+        for (int t = 0; t < 576; ++t) acc[t] += Ev * Hq[t];
+    }
+}
+
+void or_assemble_finest(const double *Ke, const double *E, const int32_t *upMap,
+                        int64_t nC, int s, double *Ks) {
+    const int nq = s * s * s;
+    double *H = finest_templates(Ke, s);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nC; ++i) {
         double acc[576];
@@ -394,7 +414,7 @@ typedef struct {
 } or_level;
 
 struct or_mesh {
-    int numLevels, nonDyadic;
+    int numLevels, nonDyadic, lean;
     double nu, cellSize;
     double Ke[576];
     double *eleModulus;
@@ -700,15 +720,70 @@ static int dense_cholesky(double *A, int64_t n) {
     return 0;
 }
 
+/* Lean storage (or_mesh_set_lean): with three levels or more the level-2
+ * Galerkin matrices (24 x 24 per coarse element, 23 GB for 5 M coarse
+ * elements) are not kept.  Their diagonals are summed from the templates'
+ * diagonal entries, and each level-3 element rebuilds its children's matrices
+ * on the fly; every value is formed by the same operations in the same order
+ * as in the stored path, so the results are identical. */
+void or_mesh_set_lean(or_mesh *M, int lean) { M->lean = lean; }
+
+static void assemble_level3_lean(const or_mesh *M, const double *H2, or_level *C) {
+    const or_level *C2 = &M->L[1];
+    const int s = C->span, nq = s * s * s, nq2 = C2->span * C2->span * C2->span;
+    double (*P)[8][8] = malloc(sizeof(double) * 64 * nq);
+    for (int q = 0; q < nq; ++q) child_prolongation(s, q, P[q]);
+    const int64_t nC = C->numElements;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nC; ++i) {
+        double acc[576], K2[576];
+        for (int t = 0; t < 576; ++t) acc[t] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            int32_t child = C->elementUpwardMap[i + q * nC];
+            if (child <= 0) continue;
+            finest_element(H2, M->eleModulus, C2->elementUpwardMap, C2->numElements, nq2, child - 1, K2);
+            galerkin_child_add(P[q], K2, 1.0, acc);
+        }
+        memcpy(C->Ks + 576 * i, acc, sizeof(acc));
+    }
+    free(P);
+}
+
 /* Solving_AssembleFEAstencil.m:9-135 */
 int or_mesh_assemble(or_mesh *M) {
     const int Lc = M->numLevels;
+    const int lean2 = M->lean && Lc >= 3;
+    double *H2 = NULL;
     for (int ii = 2; ii <= Lc; ++ii) {
         or_level *C = &M->L[ii - 1];
         free(C->Ks);
+        C->Ks = NULL;
+        if (ii == 2 && lean2) { /* diagonal only, :82-99 order */
+            const int nq = C->span * C->span * C->span;
+            H2 = finest_templates(M->Ke, C->span);
+            free(C->diagK);
+            C->diagK = (double *)calloc((size_t)C->numDOFs, sizeof(double));
+            for (int64_t e = 0; e < C->numElements; ++e) {
+                double d[24];
+                for (int t = 0; t < 24; ++t) d[t] = 0.0;
+                for (int q = 0; q < nq; ++q) {
+                    int32_t child = C->elementUpwardMap[e + q * C->numElements];
+                    if (child == 0) continue;
+                    double Ev = M->eleModulus[child - 1];
+                    for (int t = 0; t < 24; ++t) d[t] += Ev * H2[576 * q + 25 * t];
+                }
+                for (int j = 0; j < 8; ++j) {
+                    int64_t nd = C->eNodMat[e + j * C->numElements] - 1;
+                    for (int a = 0; a < 3; ++a) C->diagK[3 * nd + a] += d[3 * j + a];
+                }
+            }
+            continue;
+        }
         C->Ks = (double *)malloc(sizeof(double) * 576 * (C->numElements + 1));
         if (ii == 2)
             or_assemble_finest(M->Ke, M->eleModulus, C->elementUpwardMap, C->numElements, C->span, C->Ks);
+        else if (ii == 3 && lean2)
+            assemble_level3_lean(M, H2, C);
         else
             or_assemble_nonfinest(M->L[ii - 2].Ks, C->elementUpwardMap, C->numElements, C->span, C->Ks);
         if (ii < Lc) { /* :82-99 diag accumulated in ascending element order */
@@ -762,6 +837,7 @@ int or_mesh_assemble(or_mesh *M) {
         }
     }
     free(pos);
+    free(H2);
     /* Intermediate-level Ks are no longer needed except for inspection. */
     return dense_cholesky(M->cholL, nf);
 }
@@ -897,11 +973,12 @@ double or_mesh_time_pcg_iterations(const or_mesh *M, double omega, const double 
     double *y = calloc((size_t)n, sizeof(double));
     double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
     double *p = malloc(sizeof(double) * n);
-    double t0 = omp_get_wtime(), sK = 0, sV = 0;
+    double sK = 0, sV = 0;
     memcpy(r, b, sizeof(double) * n);
     or_mesh_vcycle(M, omega, r, z);
     memcpy(p, z, sizeof(double) * n);
     double x1 = dot(z, r, n);
+    const double t0 = omp_get_wtime(); /* the iterations only (not the initial z = M r) */
     for (int it = 0; it < nits; ++it) {
         double a = omp_get_wtime();
         or_mesh_kbyu(M, p, z);

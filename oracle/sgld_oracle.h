@@ -90,6 +90,10 @@ int64_t or_mesh_get(const or_mesh *m, int level, const char *name, void *out);
 int or_mesh_set_fixed(or_mesh *m, const uint8_t *fixedDOFs);
 /* eleModulus of level 1 (numElements). */
 int or_mesh_set_modulus(or_mesh *m, const double *E);
+/* lean != 0: the following or_mesh_assemble keeps no level-2 Galerkin
+ * matrices (diagonals only; level 3 rebuilds them per element), with the
+ * same results; for meshes whose level-2 matrices do not fit host memory. */
+void or_mesh_set_lean(or_mesh *m, int lean);
 /* Solving_AssembleFEAstencil.m:1-137: Galerkin Ks, diagK, coarsest Cholesky.
  * Returns 0, or 1 if the coarsest matrix is not SPD. */
 int or_mesh_assemble(or_mesh *m);
@@ -132,7 +136,8 @@ int or_mesh_pde_filter(const or_mesh *m, double rHatMin, const double *src, doub
                        double *tar);
 
 /* Timing helper for the CPU baseline: runs `its` PCG iterations exactly
- * (no early exit) and returns seconds spent; setup excluded. */
+ * (no early exit) and returns the seconds they took; setup and the initial
+ * z = M r excluded. */
 double or_mesh_time_pcg_iterations(const or_mesh *m, double omega, const double *b,
                                    int its, double *kbyuSeconds, double *vcycleSeconds);
 

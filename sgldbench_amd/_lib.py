@@ -67,8 +67,11 @@ SIGNATURES = {
     "sgld_ctx_pde_filter": [_vp, _dp, C.c_double, C.c_double, C.c_int, _dp, C.POINTER(C.c_int)],
     "sgld_ctx_pde_filter_device": [_vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],
     "sgld_ctx_stats": [_vp, _dp],
+    "sgld_ctx_kernel_timing": [_vp, C.c_int],
+    "sgld_ctx_kernel_times": [_vp, _dp],
+    "sgld_ctx_kernel_name": [_vp, C.c_int],
 }
-_RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p}
+_RESTYPE = {"sgld_last_error": C.c_char_p, "sgld_version": C.c_char_p, "sgld_ctx_kernel_name": C.c_char_p}
 
 _lib = None
 

@@ -648,21 +648,41 @@ __global__ void __launch_bounds__(256) k_bprolong_dir(const float *__restrict__ 
     }
 }
 
-// Coarsest level: x = Minv r for 16 cases; one block per row, threads split
-// the columns 16 ways.
-__global__ void __launch_bounds__(256) k_bcoarse(const double *__restrict__ Minv, int64_t n,
-                                                 const float *__restrict__ r, float *__restrict__ x) {
+// Coarsest level for 16 cases: y = W r(free), then x(free) = W^T y with the
+// packed inverse Cholesky factor (see k_coarse_lower / k_coarse_upper); one
+// block per row, threads split the row 16 ways x 16 cases.
+__global__ void __launch_bounds__(256) k_bcoarse_lower(const double *__restrict__ Wrow,
+                                                       const int32_t *__restrict__ freeIdx, int64_t n,
+                                                       const float *__restrict__ r, double *__restrict__ y) {
     __shared__ double sh[16][NR];
     const int j = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const int64_t row = blockIdx.x;
+    const double *w = Wrow + tri_row_off(row);
     double acc = 0.0;
-    for (int64_t k = grp; k < n; k += 16) acc += Minv[row * n + k] * (double)r[k * NR + j];
+    for (int64_t k = grp; k <= row; k += 16) acc += w[k] * (double)r[(int64_t)freeIdx[k] * NR + j];
     sh[grp][j] = acc;
     __syncthreads();
     if (threadIdx.x < NR) {
         double s = 0.0;
         for (int i = 0; i < 16; ++i) s += sh[i][j];
-        x[row * NR + j] = (float)s;
+        y[row * NR + j] = s;
+    }
+}
+__global__ void __launch_bounds__(256) k_bcoarse_upper(const double *__restrict__ Wcol,
+                                                       const int32_t *__restrict__ freeIdx, int64_t n,
+                                                       const double *__restrict__ y, float *__restrict__ x) {
+    __shared__ double sh[16][NR];
+    const int j = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int64_t col = blockIdx.x;
+    const double *w = Wcol + tri_col_off(col, n);
+    double acc = 0.0;
+    for (int64_t t = grp; t < n - col; t += 16) acc += w[t] * y[(col + t) * NR + j];
+    sh[grp][j] = acc;
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double s = 0.0;
+        for (int i = 0; i < 16; ++i) s += sh[i][j];
+        x[(int64_t)freeIdx[col] * NR + j] = (float)s;
     }
 }
 
@@ -797,6 +817,13 @@ struct BatchState {
 
 void batch_free(BatchState *b) { delete b; }
 
+int batch_clear_coarse(sgld_ctx *C) {
+    if (!C->batch) return SGLD_OK;
+    DevBuf<float> &x = C->batch->lx[C->L - 1];
+    SGLD_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * x.n, C->s));
+    return SGLD_OK;
+}
+
 namespace {
 
 constexpr int kBRedBlocks = 4096;  // partial slots per reduction (>= every reducing grid)
@@ -818,6 +845,8 @@ int batch_init(sgld_ctx *C) {
         SGLD_TRY(B->lr[k].alloc(nk));
         SGLD_TRY(B->lx[k].alloc(nk));
     }
+    // the coarsest x is written at free DOFs only
+    SGLD_HIP(hipMemsetAsync(B->lx[C->L - 1].p, 0, sizeof(float) * B->lx[C->L - 1].n, C->s));
     SGLD_TRY(B->b64.alloc(n16));
     SGLD_TRY(B->x64.alloc(n16));
     SGLD_TRY(B->r64.alloc(n16));
@@ -884,8 +913,11 @@ int enqueue_bvcycle_coarse(sgld_ctx *C, const float *r, double *partX2, int *nbX
         SGLD_HIP(hipGetLastError());
         rf = B->lr[k].p;
     }
-    hipLaunchKernelGGL(k_bcoarse, dim3((int)C->nCoarseDof), dim3(256), 0, C->s, C->Minv.p, C->nCoarseDof,
-                       B->lr[L - 1].p, B->lx[L - 1].p);
+    const int64_t nf = C->nCoarseFree;
+    hipLaunchKernelGGL(k_bcoarse_lower, dim3((int)nf), dim3(256), 0, C->s, C->cWrow.p, C->cFree.p, nf,
+                       (const float *)B->lr[L - 1].p, C->cY.p);
+    hipLaunchKernelGGL(k_bcoarse_upper, dim3((int)nf), dim3(256), 0, C->s, C->cWcol.p, C->cFree.p, nf,
+                       (const double *)C->cY.p, B->lx[L - 1].p);
     SGLD_HIP(hipGetLastError());
     for (int k = L - 1; k >= 2; --k) {
         const Level &F = C->lv[k - 1], &Cc = C->lv[k];

@@ -491,6 +491,188 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
 }
 
 // ----------------------------------------------------------------------------
+// K.u, tiled mode-space form with owner-computes node sums (no atomics, the
+// same result in every run).
+//
+// Work decomposition, node-layer staging and element arithmetic are those of
+// k_kbyu_tiled.  After the inverse (x, y) transform, element column (ei, ej)
+// holds the forces of its lower face's corners A = node (ei+1, ej),
+// B = (ei+1, ej+1), C = (ei, ej), D = (ei, ej+1).  Thread (ei, ej) owns node
+// (ei, ej) and sums, always in this order,
+//   y = (C(ei, ej) + A(ei-1, ej)) + (D(ei, ej-1) + B(ei-1, ej-1)).
+// The lanes of one 16-lane DPP row are the rows ei of one patch column, so
+// the (ei-1) terms arrive by a row shift; the (ej-1) pair comes through one
+// LDS exchange per element layer, double-buffered so the layer's single block
+// barrier orders every write and read.  Outputs are stored from registers.
+// ----------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T row_shr1(T v);  // lane - 1 of the 16-lane row; 0 in its lane 0
+template <> __device__ __forceinline__ float row_shr1<float>(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+}
+template <> __device__ __forceinline__ double row_shr1<double>(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x111, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x111, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(const T *__restrict__ u, T *__restrict__ y,
+                                                       const T *__restrict__ E,
+                                                       const uint8_t *__restrict__ nflag, Grid gd,
+                                                       TileMap tm, const int32_t *__restrict__ wlist,
+                                                       int64_t wn, double *__restrict__ partial,
+                                                       const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    constexpr int kSlots = 4;
+    __shared__ T us[kSlots][kTN * kTN * 3];
+    __shared__ T xch[2][3][256];  // D + B(ei-1) of every element column, two element layers
+    const int t = threadIdx.x;
+    const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
+    const int64_t pl = (int64_t)NY * NX;       // nodes per layer
+    const int64_t ple = (int64_t)gd.ny * gd.nx;  // elements per layer
+    const int ei = t % kTE, ej = t / kTE;
+    const int64_t W = wlist ? wn : tm.work();
+    const int G = gridDim.x;
+    const int nxcd = G < kXCD ? G : kXCD;
+    const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
+    const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
+    const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
+    const int f00 = 3 * ((ei + 1) + kTN * ej), f10 = 3 * ((ei + 1) + kTN * (ej + 1));
+    const int f01 = 3 * (ei + kTN * ej), f11 = 3 * (ei + kTN * (ej + 1));
+    (void)f00; (void)f10; (void)f11;
+    uint32_t lOff[kStageSlots];
+    double dotacc = 0.0;
+    for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
+        int64_t rem = wlist ? wlist[w] : w;
+        const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
+        rem -= (int64_t)tz * tm.tilesR * tm.tilesC;
+        const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
+        const int R0 = tr * kTO, C0 = tc * kTO, Z0 = tz * tm.cz;
+        const int Zend = min(Z0 + tm.cz, NZ);
+        const int er = R0 - 1 + ei, ec = C0 - 1 + ej;
+        const bool einside = er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx;
+#pragma unroll
+        for (int q = 0; q < kStageSlots; ++q) {
+            const int j = t + 256 * q, col = j / (3 * kTN), off = j - 3 * kTN * col, row = off / 3;
+            const int gr = R0 - 1 + row, gcl = C0 - 1 + col;
+            const bool ok = j < 3 * kTN * kTN && gr >= 0 && gr < NY && gcl >= 0 && gcl < NX;
+            lOff[q] = ok ? (uint32_t)sizeof(T) * (uint32_t)(3 * NY * col + off) : kOob;
+        }
+        // this thread's output node (ei, ej) of the staged window, if an
+        // output node (ei, ej >= 1) inside the lattice
+        const bool onode = ei >= 1 && ej >= 1 && er < NY && ec < NX;
+        const uint32_t fOff = onode ? (uint32_t)(ei + NY * ej) : kOob;
+        const uint32_t yOff = onode ? (uint32_t)sizeof(T) * (uint32_t)(3 * (ei + NY * ej)) : kOob;
+        const int64_t sBase = 3 * ((int64_t)(R0 - 1) + (int64_t)NY * (C0 - 1));
+        const uint32_t vBytes = (uint32_t)sizeof(T) * (uint32_t)(3 * NY * (kTN - 1) + 3 * kTN);
+        const uint32_t fBytes = (uint32_t)(NY * (kTN - 1) + kTN);
+        auto uRsrc = [&](int L) { return buf_rsrc(u + sBase + 3 * pl * L, (L >= 0 && L < NZ) ? vBytes : 0u); };
+        const T *pE = E + (einside ? er + (int64_t)gd.ny * ec + ple * Z0 : 0);  // layer el+1
+        T pf[kStageSlots];
+        auto loadL = [&](__amdgpu_buffer_rsrc_t r, T (&q)[kStageSlots]) {
+#pragma unroll
+            for (int k = 0; k < kStageSlots; ++k) q[k] = buf_load<T>(r, lOff[k]);
+        };
+        auto stage = [&](T *dst, const T (&q)[kStageSlots]) {
+#pragma unroll
+            for (int k = 0; k < kStageSlots; ++k)
+                if (k < kStageSlots - 1 || t + 256 * k < 3 * kTN * kTN) dst[t + 256 * k] = q[k];
+        };
+        T *sPrev = us[0], *sCur = us[1], *sNext = us[2], *sFree = us[3];
+        {
+            T p0[kStageSlots], p1[kStageSlots];
+            loadL(uRsrc(Z0 - 1), p0);
+            loadL(uRsrc(Z0), p1);
+            loadL(uRsrc(Z0 + 1), pf);
+            __syncthreads();  // previous work item done with LDS
+            stage(sPrev, p0);
+            stage(sCur, p1);
+        }
+        __syncthreads();
+        T Flo[3][4], Alo[3][4];
+        face_modes<T>(sPrev, ei, ej, Flo);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
+        T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
+        bool eokCur = true;
+        auto step = [&](int el, T (&ready)[kStageSlots], T (&spare)[kStageSlots]) {
+            const bool out = el >= Z0;
+            const __amdgpu_buffer_rsrc_t fr = buf_rsrc(nflag + sBase / 3 + pl * el, out ? fBytes : 0u);
+            const uint32_t fl = __builtin_amdgcn_raw_buffer_load_b8(fr, (int)fOff, 0, 0);
+            const bool eok = einside && el + 1 < gd.nz;
+            const T Enext = *(eok ? pE : E);  // raw; zeroed where it is used
+            loadL(uRsrc(el + 3), spare);
+            T m[3][8];
+            {
+                T Fhi[3][4];
+                face_modes<T>(sCur, ei, ej, Fhi);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        m[a][q] = Flo[a][q] + Fhi[a][q];
+                        m[a][q + 4] = Fhi[a][q] - Flo[a][q];
+                        Flo[a][q] = Fhi[a][q];
+                    }
+            }
+            mode_forces<T>(m, eokCur ? Ecur : T(0));
+            T left[3] = {T(0), T(0), T(0)};
+            T *xw = &xch[el & 1][0][0];
+            if (out) {
+                // lower face complete: inverse (x, y) transform to its corners
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T A0 = Alo[a][0] - m[a][4], A1 = Alo[a][1] + (m[a][1] - m[a][5]);
+                    const T A2 = Alo[a][2] + (m[a][2] - m[a][6]), A3 = Alo[a][3] + (m[a][3] - m[a][7]);
+                    const T t0 = A0 - A2, t1 = A1 - A3, t2 = A0 + A2, t3 = A1 + A3;
+                    const T cA = t0 - t1, cB = t0 + t1, cC = t2 - t3, cD = t2 + t3;
+                    left[a] = cC + row_shr1<T>(cA);
+                    xw[256 * a + t] = cD + row_shr1<T>(cB);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                Alo[a][0] = m[a][4];
+#pragma unroll
+                for (int q = 1; q < 4; ++q) Alo[a][q] = m[a][q] + m[a][q + 4];
+            }
+            Ecur = Enext;
+            eokCur = eok;
+            stage(sNext, ready);  // layer el+2 into the slot of layer el-2
+            __syncthreads();      // corner sums of layer el exchanged
+            if (out) {
+                const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * el, vBytes);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T v0 = left[a] + (ej >= 1 ? xw[256 * a + t - kTE] : T(0));
+                    const T v = ((fl & kSolid) && !(fl & (1u << a))) ? v0 : T(0);
+                    buf_store(v, yr, yOff + (uint32_t)sizeof(T) * (uint32_t)a);
+                    dotacc += (double)sPrev[f01 + a] * (double)v;
+                }
+            }
+            T *tmp = sPrev;
+            sPrev = sCur;
+            sCur = sNext;
+            sNext = sFree;
+            sFree = tmp;
+            pE += ple;
+        };
+        T pn[kStageSlots];
+        for (int el = Z0 - 1; el < Zend; el += 2) {
+            step(el, pf, pn);
+            if (el + 1 < Zend) step(el + 1, pn, pf);
+        }
+    }
+    if (partial) {
+        const double s = block_sum(dotacc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Restriction r_c = P^T r_f with trilinear P of span S: coarse node N gathers
 // fine nodes S N + d, |d_i| < S, weight prod (1 - |d_i|/S).
 // ----------------------------------------------------------------------------
@@ -1123,20 +1305,43 @@ __global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__r
     }
 }
 
-// Coarsest level: x = Minv r with the dense inverse of K_L on free DOFs
-// (zero rows/columns elsewhere); one wavefront per row.
+// Coarsest level (Solving_Vcycle.m:29-31): x(free) = L^-T L^-1 r(free) with
+// K_L(free, free) = L L^T, applied as two triangular products with the
+// inverse factor W = L^-1 (the device-parallel form of the two triangular
+// solves; the applied operator W^T W stays symmetric positive definite in
+// floating point).  FP64 in both precisions; one wavefront per row.
+// y = W r(free): row i of W holds entries 0..i.
 template <class T>
-__global__ void __launch_bounds__(256) k_coarse_gemv(const double *__restrict__ Minv, int64_t n,
-                                                     const T *__restrict__ r, T *__restrict__ x,
-                                                     const Ctl *ctl) {
+__global__ void __launch_bounds__(256) k_coarse_lower(const double *__restrict__ Wrow,
+                                                      const int32_t *__restrict__ freeIdx, int64_t n,
+                                                      const T *__restrict__ r, double *__restrict__ y,
+                                                      const Ctl *ctl) {
     if (stopped(ctl)) return;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
+    const double *w = Wrow + tri_row_off(row);
     double acc = 0.0;
-    for (int64_t j = lane; j < n; j += 64) acc += Minv[row * n + j] * (double)r[j];
+    for (int64_t j = lane; j <= row; j += 64) acc += w[j] * (double)r[freeIdx[j]];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if (lane == 0) x[row] = (T)acc;
+    if (lane == 0) y[row] = acc;
+}
+// x(free) = W^T y: row j of W^T (column j of W) holds entries j..n-1.  Fixed
+// and void DOFs of x are never written (zero since the assembly).
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_upper(const double *__restrict__ Wcol,
+                                                      const int32_t *__restrict__ freeIdx, int64_t n,
+                                                      const double *__restrict__ y, T *__restrict__ x,
+                                                      const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    const int64_t j = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const double *w = Wcol + tri_col_off(j, n);
+    double acc = 0.0;
+    for (int64_t t = lane; t < n - j; t += 64) acc += w[t] * y[j + t];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) x[freeIdx[j]] = (T)acc;
 }
 
 // ----------------------------- PCG vector kernels ---------------------------
@@ -1363,23 +1568,51 @@ __global__ void __launch_bounds__(576) k_galerkin_from_Ks(const double *__restri
 // other threads still read it.  Step k: B = A - a_col a_row / a_kk off the
 // pivot row/col, B_kj = a_kj / a_kk, B_ik = -a_ik / a_kk, B_kk = 1 / a_kk.
 // A non-positive pivot sets *bad (to 1 + k; first one wins).
-__global__ void k_gauss_jordan_step(const double *__restrict__ A, double *__restrict__ B, int64_t n,
-                                    int64_t k, int64_t *bad) {
-    const double piv = A[k * n + k];
-    if (!(piv > 0.0)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS((unsigned long long *)bad, 0ull,
-                                                           (unsigned long long)(k + 1));
+// Dense Cholesky of the coarsest free-DOF matrix (column-major n x n, lower
+// triangle used), right-looking, one launch per pivot k: the trailing lower
+// triangle takes the Schur update A(i,j) -= A(i,k) A(j,k) / A(k,k) for
+// i >= j > k, in place (column k is only read in step k).  A 16 x 16 tile of
+// (i, j) per block, i fastest; tiles above the diagonal exit.  A pivot that
+// is not positive is reported (1-based) in *bad.
+__global__ void __launch_bounds__(256) k_chol_step(double *__restrict__ A, int64_t n, int64_t k,
+                                                   unsigned long long *bad) {
+    const double piv = A[k + n * k];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && !(piv > 0.0))
+        atomicCAS(bad, 0ull, (unsigned long long)(k + 1));
+    const int64_t i0 = k + 1 + 16 * (int64_t)blockIdx.y, j0 = k + 1 + 16 * (int64_t)blockIdx.x;
+    if (i0 + 15 < j0) return;
+    const int64_t i = i0 + (threadIdx.x & 15), j = j0 + (threadIdx.x >> 4);
+    if (i >= n || j >= n || i < j) return;
+    A[i + n * j] -= A[i + n * k] * (A[j + n * k] / piv);
+}
+// L (row-major, lower) from the Schur-updated columns: L(i,k) = A(i,k) / sqrt(A(k,k)).
+__global__ void k_chol_finish(const double *__restrict__ A, int64_t n, double *__restrict__ L) {
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx / n, k = idx % n;
+        L[idx] = i < k ? 0.0 : (i == k ? sqrt(A[k + n * k]) : A[i + n * k] / sqrt(A[k + n * k]));
     }
-    const double inv = 1.0 / piv;
+}
+// Row k of W = L^-1 (row-major) by forward substitution of e_k, from the
+// rows 0..k-1 already formed: W(k,c) = (delta_kc - sum_{c<=j<k} L(k,j) W(j,c)) / L(k,k).
+__global__ void __launch_bounds__(256) k_trinv_row(const double *__restrict__ L, double *__restrict__ W,
+                                                   int64_t n, int64_t k) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > k) return;
+    const double *lk = L + k * n;
+    double acc = c == k ? 1.0 : 0.0;
+    for (int64_t j = c; j < k; ++j) acc -= lk[j] * W[j * n + c];
+    W[k * n + c] = acc / lk[k];
+}
+// Packed rows and columns of the lower-triangular W (tri_row_off / tri_col_off).
+__global__ void k_tri_pack(const double *__restrict__ W, int64_t n, double *__restrict__ Wrow,
+                           double *__restrict__ Wcol) {
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
          idx += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = idx / n, j = idx % n;
-        double v;
-        if (i == k && j == k) v = inv;
-        else if (i == k) v = A[idx] * inv;
-        else if (j == k) v = -A[idx] * inv;
-        else v = A[idx] - A[i * n + k] * A[k * n + j] * inv;
-        B[idx] = v;
+        if (j > i) continue;
+        Wrow[tri_row_off(i) + j] = W[idx];
+        Wcol[tri_col_off(j, n) + (i - j)] = W[idx];
     }
 }
 
@@ -1567,6 +1800,51 @@ template <> const float *ctx_E<float>(sgld_ctx *C) { return C->E32.p; }
 
 double *slot(sgld_ctx *C, int k) { return C->partial.p + (int64_t)k * kRedBlocks; }
 
+// Kernel timer (KernelTimer): events around one launch on the context stream.
+enum { KT_DIR64 = 0, KT_DIR32, KT_KU64, KT_KU32, KT_UPD64, KT_UPD32, KT_RES64, KT_RES32 };
+int kt_begin(sgld_ctx *C, int id, const char *name) {
+    KernelTimer &T = C->kt;
+    if (!T.on || !T.inSolve) return SGLD_OK;
+    T.name[id] = name;
+    KernelTimer::Rec r{id, T.curIter, nullptr, nullptr};
+    for (hipEvent_t *e : {&r.a, &r.b}) {
+        if (!T.pool.empty()) {
+            *e = T.pool.back();
+            T.pool.pop_back();
+        } else {
+            SGLD_HIP(hipEventCreate(e));
+        }
+    }
+    SGLD_HIP(hipEventRecord(r.a, C->s));
+    T.pending.push_back(r);
+    return SGLD_OK;
+}
+int kt_end(sgld_ctx *C) {
+    KernelTimer &T = C->kt;
+    if (!T.on || !T.inSolve) return SGLD_OK;
+    SGLD_HIP(hipEventRecord(T.pending.back().b, C->s));
+    return SGLD_OK;
+}
+// After the stream has drained: launches of batch iterations >= `worked`
+// ran past the stop flag.
+int kt_harvest(sgld_ctx *C, int worked) {
+    KernelTimer &T = C->kt;
+    for (const KernelTimer::Rec &r : T.pending) {
+        float ms = 0.0f;
+        SGLD_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        T.ms[r.id] += ms;
+        T.n[r.id] += 1;
+        if (r.iter < worked) {
+            T.msWork[r.id] += ms;
+            T.nWork[r.id] += 1;
+        }
+        T.pool.push_back(r.a);
+        T.pool.push_back(r.b);
+    }
+    T.pending.clear();
+    return SGLD_OK;
+}
+
 // y = K u; with zin != null first pout = zin + beta u (the PCG direction
 // update of the previous iteration) and y = K pout.
 template <class T>
@@ -1581,11 +1859,21 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
         SGLD_HIP(hipGetLastError());
         u = pout;
     }
+    constexpr bool f64 = sizeof(T) == 8;
+    SGLD_TRY(kt_begin(C, f64 ? KT_KU64 : KT_KU32,
+                      C->kbyuVariant != 1 ? (f64 ? "k_kbyu<double>" : "k_kbyu<float>")
+                      : C->kbyuOwner ? (f64 ? "k_kbyu_owner<double>" : "k_kbyu_owner<float>")
+                      : C->kbyuOneBar ? (f64 ? "k_kbyu_tiled<double, true>" : "k_kbyu_tiled<float, true>")
+                                      : (f64 ? "k_kbyu_tiled<double, false>" : "k_kbyu_tiled<float, false>")));
     if (C->kbyuVariant == 1) {
-        const int q = sizeof(T) == 8 ? 0 : 1;
+        const int q = f64 ? 0 : 1;
         const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
-        if (C->kbyuOneBar)
+        if (C->kbyuOwner)
+            hipLaunchKernelGGL(k_kbyu_owner<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p, l0.g,
+                               ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial,
+                               ctl);
+        else if (C->kbyuOneBar)
             hipLaunchKernelGGL((k_kbyu_tiled<T, true>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
                                l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
                                C->nKwork[q], partial, ctl);
@@ -1600,6 +1888,7 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
                            l0.g, S, partial, ctl);
     }
     SGLD_HIP(hipGetLastError());
+    SGLD_TRY(kt_end(C));
     if (nbOut) *nbOut = nb;
     return SGLD_OK;
 }
@@ -1788,7 +2077,9 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
     const int kLast = t0 > 0 ? t0 : L - 1;  // last level reached by single restrictions
     const T *rf = r;
     for (int k = 1; k <= kLast; ++k) {
+        if (k == 1) SGLD_TRY(kt_begin(C, sizeof(T) == 8 ? KT_RES64 : KT_RES32, "first restriction"));
         SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
+        if (k == 1) SGLD_TRY(kt_end(C));
         rf = lvl_r<T>(C->lv[k]);
     }
     TailArgs<T> A{};
@@ -1798,9 +2089,11 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
         SGLD_HIP(hipGetLastError());
     }
     Level &Lc = C->lv[L - 1];
-    const int64_t n = C->nCoarseDof;
-    hipLaunchKernelGGL(k_coarse_gemv<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->Minv.p, n,
-                       lvl_r<T>(Lc), lvl_x<T>(Lc), ctl);
+    const int64_t n = C->nCoarseFree;
+    hipLaunchKernelGGL(k_coarse_lower<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->cWrow.p, C->cFree.p, n,
+                       (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
+    hipLaunchKernelGGL(k_coarse_upper<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->cWcol.p, C->cFree.p, n,
+                       (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
     SGLD_HIP(hipGetLastError());
     if (t0 > 0) {
         const int nb = grid_blocks(3 * A.g0.nodes());
@@ -1858,7 +2151,13 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     // either way, and on irregular domains whole column pairs cost more than
     // the 64-entry chunks of the DOF pass (cfg 2: 1.86 -> 2.51 s).
     const size_t colBytes = sizeof(T) * 4 * 3 * (size_t)l1.g.NY();  // staged coarse runs
-    if (l1.span == 2 && C->dirCols && sizeof(T) == 8 && C->full && colBytes <= 48 * 1024) {
+    constexpr bool f64 = sizeof(T) == 8;
+    const bool cols = l1.span == 2 && C->dirCols && f64 && C->full && colBytes <= 48 * 1024;
+    SGLD_TRY(kt_begin(C, f64 ? KT_DIR64 : KT_DIR32,
+                      cols ? "k_prolong_dir_cols<double>"
+                      : l1.span == 2 ? (f64 ? "k_prolong_dir<double, 2>" : "k_prolong_dir<float, 2>")
+                                     : (f64 ? "k_prolong_dir<double, 4>" : "k_prolong_dir<float, 4>")));
+    if (cols) {
         const int64_t items = ((l0.g.NX() + 1) / 2) * l0.g.NZ();
         const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(items, 65536));
         hipLaunchKernelGGL(k_prolong_dir_cols<T>, dim3(nb), dim3(256), colBytes, C->s, (const T *)lvl_x<T>(l1),
@@ -1871,11 +2170,14 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
         hipLaunchKernelGGL((k_prolong_dir<T, 4>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
                            (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
     SGLD_HIP(hipGetLastError());
+    SGLD_TRY(kt_end(C));
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
+    SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, f64 ? "k_update_r<double>" : "k_update_r<float>"));
     hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, fine_dinv<T>(C), S,
                        slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl);
     SGLD_HIP(hipGetLastError());
+    SGLD_TRY(kt_end(C));
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
     int nb2 = 0;
     SGLD_TRY(enqueue_vcycle_coarse<T>(C, r, slot(C, 2), &nb2, ctl));
@@ -1990,11 +2292,13 @@ int pcg_begin(sgld_ctx *C, SolveState &S) {
 // when the solve stops or before a reliable update.
 int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
     for (int i = 0; i < S.batch; ++i) {
+        C->kt.curIter = i;
         if (S.precision == SGLD_PREC_FP64)
             SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->p.p, C->q.p));
         else
             SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->p32.p, C->q32.p));
     }
+    C->kt.curIter = -1;
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
     return SGLD_OK;
 }
@@ -2095,15 +2399,50 @@ int upload_flags(Level &l) {
     return SGLD_OK;
 }
 
-// One solve on lattice vectors b -> x (x holds x0).
-int pcg_solve(sgld_ctx *C, SolveState &S) {
+// One solve on lattice vectors b -> x (x holds x0).  Up to S.batch
+// iterations are queued between host checks.  Once two checks within one
+// stretch (FP64: the solve; mixed: between reliable updates) have measured
+// the per-iteration decrease f of the recursive residual, the next batch is
+// cut to the iterations predicted to reach that stretch's stop plus one, so
+// few iterations run past the device-side stop flag (their ~20 launches
+// return at once but still cost the dispatch).
+int pcg_solve_loop(sgld_ctx *C, SolveState &S) {
+    const int full = S.batch;
     SGLD_TRY(pcg_begin(C, S));
+    double res0 = -1.0;
+    int its0 = 0, upd0 = -1;
     while (!S.finished) {
+        const int before = C->hctl->its;
         SGLD_TRY(pcg_enqueue_batch(C, S));
         SGLD_HIP(hipStreamSynchronize(C->s));
+        const Ctl st = *C->hctl;
+        SGLD_TRY(kt_harvest(C, st.its - before));
         SGLD_TRY(pcg_after_batch(C, S));
+        if (S.finished) break;
+        S.batch = full;
+        if (upd0 == S.updates && res0 > 0.0 && st.res > 0.0 && st.its > its0 && st.tol > 0.0) {
+            const double f = std::pow(st.res / res0, 1.0 / (st.its - its0));
+            if (f > 0.0 && f < 1.0) {
+                const double k = std::ceil(std::log(st.tol / st.res) / std::log(f));
+                if (k >= 0.0 && k + 1.0 < full) S.batch = std::max(1, (int)k + 1);
+            }
+        }
+        // a reliable update starts a new stretch (new reference norm)
+        res0 = st.res;
+        its0 = st.its;
+        upd0 = S.updates;
+        if (S.precision != SGLD_PREC_FP64 && C->hctl->normb != st.normb) upd0 = -1;
     }
-    return SGLD_OK;
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return kt_harvest(C, 1 << 30);
+}
+
+int pcg_solve(sgld_ctx *C, SolveState &S) {
+    C->kt.inSolve = true;
+    const int rc = pcg_solve_loop(C, S);
+    C->kt.inSolve = false;
+    C->kt.curIter = -1;
+    return rc;
 }
 
 // Reliable update of the mixed solve: r32 = (b - K x) * invScale (FP64 K.u
@@ -2417,7 +2756,13 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         // one barrier per element layer: measured 71-74 vs 75 us (FP64) and 52 vs
         // 54 us (FP32) on 256x128x128; SGLD_KU_BARRIERS=2 selects the two-barrier step
         if (const char *env = std::getenv("SGLD_KU_BARRIERS")) C->kbyuOneBar = std::atoi(env) != 2;
-        if (C->kbyuOneBar) {
+        // owner-computes node sums (no LDS atomics, run-to-run identical);
+        // SGLD_KU_OWNER=0 selects the atomic-accumulation kernel
+        if (const char *env = std::getenv("SGLD_KU_OWNER")) C->kbyuOwner = std::atoi(env) != 0;
+        if (C->kbyuOwner) {
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_owner<double>, 256, 0));
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_owner<float>, 256, 0));
+        } else if (C->kbyuOneBar) {
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double, true>, 256, 0));
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float, true>, 256, 0));
         } else {
@@ -2428,9 +2773,12 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         // Blocks per compute unit of the persistent grid.  A sweep of 2..6 on the
         // 256x128x128 cuboid measured 3 (FP64: 168 VGPRs, 3 waves per SIMD) and
         // 4 (FP32: 114 VGPRs) fastest, one more than the occupancy query gives.
-        const int measured[2] = {3, 4};
+        int measured[2] = {3, 4};
+        if (const char *env = std::getenv("SGLD_KU_BPCU64")) measured[0] = std::max(1, std::atoi(env));
+        if (const char *env = std::getenv("SGLD_KU_BPCU32")) measured[1] = std::max(1, std::atoi(env));
         for (int q = 0; q < 2; ++q) {
-            C->kOcc[q] = std::max(per[q], measured[q]);
+            C->kOcc[q] = std::getenv(q ? "SGLD_KU_BPCU32" : "SGLD_KU_BPCU64") ? measured[q]
+                                                                                : std::max(per[q], measured[q]);
             TiledSchedule &ts = C->ksched[q];
             ts.tm.tilesR = (int)((g0.NY() + kTO - 1) / kTO);
             ts.tm.tilesC = (int)((g0.NX() + kTO - 1) / kTO);
@@ -2452,6 +2800,11 @@ extern "C" int sgld_ctx_destroy(sgld_ctx *C) {
     if (C->hctl) (void)hipHostFree(C->hctl);
     if (C->e0) (void)hipEventDestroy(C->e0);
     if (C->e1) (void)hipEventDestroy(C->e1);
+    for (hipEvent_t e : C->kt.pool) (void)hipEventDestroy(e);
+    for (const KernelTimer::Rec &r : C->kt.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
     batch_free(C->batch);
     hipStream_t s = C->s;
     delete C;
@@ -2523,7 +2876,8 @@ extern "C" int sgld_ctx_set_modulus(sgld_ctx *C, const double *E) {
     return SGLD_OK;
 }
 
-extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
+namespace {
+int assemble_impl(sgld_ctx *C, double omega) {
     SGLD_ARG(C, "sgld:arg: null context");
     SGLD_ARG(omega > 0 && omega < 2, "sgld:arg: omega must be in (0, 2)");
     SGLD_TRY(use_constants(C));  // the finest diagonal reads Ke
@@ -2576,22 +2930,32 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
         }
         KsPrev = std::move(KsCur);
     }
-    // Coarsest: assemble K_L on its lattice DOFs, keep solid non-fixed ones,
-    // factor, invert (Solving_AssembleFEAstencil.m:123-135, Solving_Vcycle.m:29-31).
+    // Coarsest: K_L on its free DOFs (solid, not fixed), dense FP64 Cholesky
+    // K_L = L L^T on the device and the inverse factor W = L^-1 for the two
+    // triangular products of the V-cycle (Solving_AssembleFEAstencil.m:123-135,
+    // Solving_Vcycle.m:29-31).
     Level &lc = C->lv[C->L - 1];
-    const int64_t ne = lc.g.elems(), nn = lc.g.nodes(), nd = 3 * nn;
-    std::vector<double> Ks(576 * ne);
-    SGLD_HIP(hipMemcpyAsync(Ks.data(), KsPrev.p, sizeof(double) * Ks.size(), hipMemcpyDeviceToHost, C->s));
-    SGLD_HIP(hipStreamSynchronize(C->s));
-    KsPrev.release();
-    std::vector<int64_t> pos(nd, -1), freeIdx;
+    const int64_t ne = lc.g.elems(), nn = lc.g.nodes();
+    std::vector<int32_t> freeIdx;
+    std::vector<int64_t> pos(3 * nn, -1);
     for (int64_t g = 0; g < nn; ++g)
         for (int a = 0; a < 3; ++a)
             if ((lc.hflag[g] & kSolid) && !(lc.hflag[g] & (1 << a))) {
                 pos[3 * g + a] = (int64_t)freeIdx.size();
-                freeIdx.push_back(3 * g + a);
+                freeIdx.push_back((int32_t)(3 * g + a));
             }
     const int64_t nf = (int64_t)freeIdx.size();
+    if (nf > kCoarseMaxFree) {
+        set_error("sgld:arg: the coarsest level (level %d, %dx%dx%d elements) has %lld free DOFs; its dense "
+                  "Cholesky factor supports at most %lld: use more levels", C->L, lc.g.nx, lc.g.ny, lc.g.nz,
+                  (long long)nf, (long long)kCoarseMaxFree);
+        return SGLD_ERR_ARG;
+    }
+    SGLD_ARG(nf > 0, "sgld:arg: the coarsest level has no free DOF");
+    std::vector<double> Ks(576 * ne);
+    SGLD_HIP(hipMemcpyAsync(Ks.data(), KsPrev.p, sizeof(double) * Ks.size(), hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    KsPrev.release();
     std::vector<double> A((size_t)nf * nf, 0.0);
     const int64_t NY = lc.g.NY(), NX = lc.g.NX();
     for (int64_t e = 0; e < ne; ++e) {
@@ -2608,23 +2972,19 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
                 if (dof[r] >= 0) A[dof[r] + nf * dof[c]] += Ks[576 * e + r + 24 * c];
         }
     }
-    // invert on the device (row-major = col-major: A is symmetric)
-    DevBuf<double> dA, dB;
-    DevBuf<int64_t> dBad;
+    DevBuf<double> dA, dL, dW;
+    DevBuf<unsigned long long> dBad;
     SGLD_TRY(dA.upload(A.data(), A.size()));
-    SGLD_TRY(dB.alloc(A.size()));
     SGLD_TRY(dBad.alloc(1));
     SGLD_TRY(dBad.zero(C->s));
     for (int64_t k = 0; k < nf; ++k) {
-        hipLaunchKernelGGL(k_gauss_jordan_step, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s,
-                           dA.p, dB.p, nf, k, dBad.p);
-        std::swap(dA.p, dB.p);
+        const unsigned t = (unsigned)((nf - k - 1 + 15) / 16);
+        hipLaunchKernelGGL(k_chol_step, dim3(std::max(1u, t), std::max(1u, t)), dim3(256), 0, C->s, dA.p, nf, k,
+                           dBad.p);
     }
     SGLD_HIP(hipGetLastError());
-    int64_t bad = 0;
-    SGLD_HIP(hipMemcpyAsync(&bad, dBad.p, sizeof(int64_t), hipMemcpyDeviceToHost, C->s));
-    std::vector<double> Ainv(A.size());
-    SGLD_HIP(hipMemcpyAsync(Ainv.data(), dA.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, C->s));
+    unsigned long long bad = 0;
+    SGLD_HIP(hipMemcpyAsync(&bad, dBad.p, sizeof(bad), hipMemcpyDeviceToHost, C->s));
     SGLD_HIP(hipStreamSynchronize(C->s));
     if (bad) {
         const int64_t dof = freeIdx[bad - 1], g = dof / 3;
@@ -2634,15 +2994,41 @@ extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
                   (long long)(g / (NY * NX)), (long long)(dof % 3), A[(bad - 1) * (nf + 1)]);
         return SGLD_ERR_NOT_SPD;
     }
-    std::vector<double> M((size_t)nd * nd, 0.0);
-    for (int64_t j = 0; j < nf; ++j)
-        for (int64_t i = 0; i < nf; ++i) M[freeIdx[i] * nd + freeIdx[j]] = Ainv[i + nf * j];
-    SGLD_TRY(C->Minv.upload(M.data(), M.size()));
-    C->nCoarseDof = nd;
+    SGLD_TRY(dL.alloc(A.size()));
+    hipLaunchKernelGGL(k_chol_finish, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s, dA.p, nf, dL.p);
+    dA.release();
+    SGLD_TRY(dW.alloc(A.size()));
+    for (int64_t k = 0; k < nf; ++k)
+        hipLaunchKernelGGL(k_trinv_row, dim3(blocks_for(k + 1, 256)), dim3(256), 0, C->s, dL.p, dW.p, nf, k);
+    SGLD_HIP(hipGetLastError());
+    SGLD_TRY(C->cWrow.alloc(tri_row_off(nf)));
+    SGLD_TRY(C->cWcol.alloc(tri_row_off(nf)));
+    hipLaunchKernelGGL(k_tri_pack, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s, dW.p, nf, C->cWrow.p,
+                       C->cWcol.p);
+    SGLD_HIP(hipGetLastError());
+    SGLD_TRY(C->cFree.upload(freeIdx.data(), freeIdx.size()));
+    SGLD_TRY(C->cY.alloc(16 * nf));
+    C->nCoarseFree = nf;
+    // the coarsest x is written at free DOFs only: clear the rest (a new mask
+    // may have fixed DOFs that were free before)
+    SGLD_TRY(lc.x.zero(C->s));
+    SGLD_HIP(hipMemsetAsync(lc.x32.p, 0, sizeof(float) * lc.x32.n, C->s));
+    SGLD_TRY(batch_clear_coarse(C));
     SGLD_HIP(hipStreamSynchronize(C->s));
     C->stats[4] = now_s() - t0;
     C->assembled = true;
     return SGLD_OK;
+}
+}  // namespace
+
+extern "C" int sgld_ctx_assemble(sgld_ctx *C, double omega) {
+    try {
+        if (C) C->assembled = false;
+        return assemble_impl(C, omega);
+    } catch (const std::bad_alloc &) {
+        set_error("sgld:nomem: host allocation failed during assembly");
+        return SGLD_ERR_NOMEM;
+    }
 }
 
 extern "C" int sgld_ctx_kbyu(sgld_ctx *C, const double *u, double *y) {
@@ -2789,6 +3175,29 @@ extern "C" int sgld_ctx_bench_kbyu(sgld_ctx *C, int precision, int reps, double 
 extern "C" int sgld_ctx_stats(const sgld_ctx *C, double *out) {
     SGLD_ARG(C && out, "sgld:arg: null argument");
     for (int i = 0; i < 8; ++i) out[i] = C->stats[i];
+    return SGLD_OK;
+}
+
+extern "C" int sgld_ctx_kernel_timing(sgld_ctx *C, int on) {
+    SGLD_ARG(C, "sgld:arg: null context");
+    C->kt.on = on != 0;
+    C->kt.reset();
+    return SGLD_OK;
+}
+
+extern "C" const char *sgld_ctx_kernel_name(const sgld_ctx *C, int id) {
+    if (!C || id < 0 || id >= KernelTimer::kN || !C->kt.name[id]) return "";
+    return C->kt.name[id];
+}
+
+extern "C" int sgld_ctx_kernel_times(const sgld_ctx *C, double *out) {
+    SGLD_ARG(C && out, "sgld:arg: null argument");
+    for (int k = 0; k < KernelTimer::kN; ++k) {
+        out[4 * k + 0] = (double)C->kt.n[k];
+        out[4 * k + 1] = C->kt.ms[k];
+        out[4 * k + 2] = (double)C->kt.nWork[k];
+        out[4 * k + 3] = C->kt.msWork[k];
+    }
     return SGLD_OK;
 }
 

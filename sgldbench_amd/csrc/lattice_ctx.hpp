@@ -57,6 +57,27 @@ struct Level {
 struct BatchState;             // batched.hip
 void batch_free(BatchState *b);
 
+// HIP-event timing of the finest-level passes of a solve, per kernel and
+// precision (ids in the order of SGLD_KT_* in sgldfea.h).  Each recorded
+// launch carries its iteration's position in the queued batch (-1 outside
+// the iterations); once the batch has completed, launches of iterations after
+// the stopping one (their kernels see the device-side stop flag and return
+// at once) are told apart from the ones that did the work.
+struct KernelTimer {
+    static constexpr int kN = 8;
+    bool on = false, inSolve = false;
+    int curIter = -1;
+    struct Rec { int id, iter; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[kN] = {}, msWork[kN] = {};
+    int64_t n[kN] = {}, nWork[kN] = {};
+    const char *name[kN] = {};  // kernel of the last timed launch per id
+    void reset() {
+        for (int k = 0; k < kN; ++k) { ms[k] = msWork[k] = 0.0; n[k] = nWork[k] = 0; }
+    }
+};
+
 }  // namespace sgld
 
 struct sgld_ctx {
@@ -71,8 +92,11 @@ struct sgld_ctx {
     sgld::DevBuf<double> stressT;  // element nodal-stress operator (48 x 24), built on first use
     sgld::DevBuf<double> E;
     sgld::DevBuf<float> E32;
-    sgld::DevBuf<double> Minv;
-    int64_t nCoarseDof = 0;
+    // coarsest level: packed rows / columns of W = L^-1 (K_L(free, free) = L L^T),
+    // its free lattice DOFs (ascending) and the intermediate W r (16 cases wide)
+    sgld::DevBuf<double> cWrow, cWcol, cY;
+    sgld::DevBuf<int32_t> cFree;
+    int64_t nCoarseFree = 0;
     sgld::DevBuf<double> b, x, r, z, p, p2, q;
     sgld::DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
     sgld::DevBuf<double> partial;
@@ -85,6 +109,7 @@ struct sgld_ctx {
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
     bool kbyuOneBar = true;  // tiled K.u with one block barrier per element layer (else two)
+    bool kbyuOwner = false;  // tiled K.u with owner-computes node sums (else LDS atomics)
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
@@ -104,6 +129,7 @@ struct sgld_ctx {
     int kOcc[2] = {1, 1};           // resident tiled K.u blocks per compute unit
     int numCU = 0;
     double stats[8] = {0};
+    sgld::KernelTimer kt;
     sgld::BatchState *batch = nullptr;  // multi-RHS buffers, allocated on first use
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }
 };
@@ -133,5 +159,7 @@ int check_ready(sgld_ctx *C);
 // rr = r.r and bb = b.b; host-synchronous.
 int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb);
 int ensure_efwd(sgld_ctx *C);  // voxel -> 1-based element id map of the finest level
+// Clears the batched coarsest-level x (written at free DOFs only) if allocated.
+int batch_clear_coarse(sgld_ctx *C);
 
 }  // namespace sgld

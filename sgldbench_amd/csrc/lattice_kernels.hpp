@@ -89,6 +89,15 @@ struct Span {
     }
 };
 
+// Packed storage of the coarsest level's inverse Cholesky factor W = L^-1 over
+// its n free DOFs: row i of W (entries 0..i) at tri_row_off(i), column j
+// (entries j..n-1, a row of W^T) at tri_col_off(j, n).
+__host__ __device__ inline int64_t tri_row_off(int64_t i) { return i * (i + 1) / 2; }
+__host__ __device__ inline int64_t tri_col_off(int64_t j, int64_t n) { return j * n - j * (j - 1) / 2; }
+// Largest coarsest free-DOF count of the dense factor (8 n^2 bytes packed;
+// factorisation about n^3 / 3 flops at set-up).
+constexpr int64_t kCoarseMaxFree = 8192;
+
 constexpr int kRedBlocks = 8192;  // partial-sum slots of one reduction (max grid of a reducing kernel)
 constexpr int kVecBlocks = 2048;  // grid of the grid-stride vector kernels
 constexpr int kRedSlots = 4;      // independent reductions per solve

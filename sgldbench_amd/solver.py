@@ -15,6 +15,10 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_INTERLEAVED = 2  # pcg_batched only: the 16 cases interleaved per DOF, matrix-core K.u
 
+# Kernel timer ids (SGLD_KT_* of include/sgldfea.h), in order.
+KERNEL_TIMER_IDS = ("dir64", "dir32", "kbyu64", "kbyu32", "update_r64", "update_r32",
+                    "restrict64", "restrict32")
+
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
@@ -378,6 +382,20 @@ class FEASolver:
         check(lib().sgld_ctx_stats(self._h, out))
         return dict(zip(["solve_s", "its", "true_relres", "outer", "setup_s", "filter_ms",
                          "kbyu_ms_fp64", "kbyu_ms_fp32"], out.tolist()))
+
+    def kernel_timing(self, on=True):
+        """HIP-event timing of the finest-level passes of the following solves
+        (resets the counters)."""
+        check(lib().sgld_ctx_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kernel: dict(launches, ms, work_launches, work_ms)} of the timed solves."""
+        out = np.zeros(4 * len(KERNEL_TIMER_IDS))
+        check(lib().sgld_ctx_kernel_times(self._h, out))
+        return {name: dict(launches=int(out[4 * k]), ms=float(out[4 * k + 1]),
+                           work_launches=int(out[4 * k + 2]), work_ms=float(out[4 * k + 3]),
+                           kernel=lib().sgld_ctx_kernel_name(self._h, k).decode())
+                for k, name in enumerate(KERNEL_TIMER_IDS)}
 
 
 def multi_pcg_device(solvers, dF_ptrs, dU_ptrs, tol=1e-8, maxIT=500, precision=PREC_MIXED, batch=8):

@@ -305,3 +305,50 @@ def test_mixed_load_magnitude(factor):
     assert g.converged and rr <= 1e-8
     assert abs(its - its1) <= 2
     assert np.linalg.norm(U - factor * U1) <= 1e-7 * np.linalg.norm(factor * U1)
+
+
+def test_coarsest_size_guard():
+    """A coarsest level beyond the dense factor's limit is an argument error
+    that names the limit (no host or device allocation of its n^2 arrays)."""
+    vox = problems.cuboid(64, 64, 64)
+    F, fixed = problems.cantilever3(vox)
+    g = solver.FEASolver(vox, 2)  # coarsest 32^3 elements: 107,811 lattice DOFs
+    g.set_fixed(fixed)
+    g.set_modulus(np.ones(int(vox.sum())))
+    with pytest.raises(solver.SGLDError) as e:
+        g.assemble(0.6)
+    assert e.value.code == 1 and "at most 8192" in str(e.value), str(e.value)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["cuboid", "holes"])
+def test_coarsest_solve_is_dense_cholesky_solve(case):
+    """Two-level V-cycle minus its smoothing term isolates P K_L^-1 P^T r:
+    with the coarsest level's free-DOF matrix assembled densely by the oracle
+    (Galerkin Ks) and solved by numpy, the GPU's triangular products agree to
+    FP64 rounding; the operator is symmetric."""
+    vox = problems.cuboid(16, 8, 8) if case == "cuboid" else holes(16, 8, 8, 3, nholes=2, r=2)
+    g, m, F, fixed, E = build_pair(vox, 2)
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal(F.size) * (1 - fixed)
+    d1 = m.get(1, "diagK")
+    smooth = np.where(fixed != 0, 0.0, 2 * 0.6 * r / d1)
+    coarse_gpu = g.vcycle(r) - smooth
+    # dense K_L(free, free) from the oracle's level-2 element matrices
+    info = m.info[1]
+    eN = m.get(2, "eNodMat")
+    Ks = m.get(2, "Ks")
+    n2 = info["numDOFs"]
+    K = np.zeros((n2, n2))
+    for e in range(info["numElements"]):
+        d = (3 * (eN[e] - 1)[:, None] + np.arange(3)[None, :]).ravel()
+        K[np.ix_(d, d)] += Ks[:, :, e]
+    free = m.get(2, "fixedDOFs") == 0
+    rc = m.restrict(2, r)
+    xc = np.zeros(n2)
+    xc[free] = np.linalg.solve(K[np.ix_(free, free)], rc[free])
+    coarse_ref = m.interpolate(2, xc) * (1 - fixed)
+    assert np.max(np.abs(coarse_gpu - coarse_ref)) <= 1e-11 * np.max(np.abs(coarse_ref))
+    s = rng.standard_normal(F.size) * (1 - fixed)
+    a, b = s @ g.vcycle(r), r @ g.vcycle(s)
+    assert abs(a - b) <= 1e-12 * abs(a)
