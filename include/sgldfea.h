@@ -73,6 +73,20 @@ int sgld_restrict(const double *rF, int64_t nF, const int32_t *transferMat, int6
                   const int32_t *solidNodeMapCoarser2Finer, const double *transferMatCoeffi,
                   const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
 
+/* sgld_restrict with every array in device memory except colorOffsets (host),
+ * and no allocation: no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The ghost-node array the reference
+ * mxMallocs per call lives in a fixed 32 MiB device array that each GPU holds
+ * until the process ends, so nGhost <= 1,398,090; larger calls return
+ * SGLD_ERR_ARG.  Ids are checked on the device before use.  Synchronous on the
+ * null stream; calls that overlap on one GPU share the scratch and race. */
+int sgld_restrict_noalloc(const double *rF, int64_t nF, const int32_t *transferMat, int64_t m,
+                          const double *RIT, int64_t ritRows, int64_t ritCols,
+                          const int32_t *eNodMatC, int64_t numElesC, int64_t nC, int64_t nGhost,
+                          const int32_t *solidNodeMapCoarser2Finer, const double *transferMatCoeffi,
+                          const int32_t *colors, const int64_t *colorOffsets, int numColors, double *Y);
+
 /* Replaces Solving_Interpolation_MatrixFree_mex
  * (src/MEXfuncs/Solving_Interpolation_MatrixFree_mex.cpp:6-190), called at
  * src/Solving_InterpolationDeviation.m:7-9.  xC: nC x 3, RI: m x 8, Y: nF x 3. */

@@ -25,6 +25,8 @@ SIGNATURES = {
     "sgld_kbyu8x8": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _lp, C.c_int, _dp],
     "sgld_restrict": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,
                       _ip, _lp, C.c_int, _dp],
+    "sgld_restrict_noalloc": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _vp,
+                              _vp, _lp, C.c_int, _vp],
     "sgld_interpolate": [_dp, _i64, _ip, _i64, _dp, _i64, _i64, _ip, _i64, _i64, _i64, _ip, _dp,
                          _ip, _lp, C.c_int, _dp],
     "sgld_assemble_finest": [_dp, _i64, _i64, _dp, _i64, _ip, _i64, _i64, _dp],

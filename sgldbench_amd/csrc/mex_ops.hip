@@ -11,6 +11,7 @@
 // algorithm's bit for bit (tests compare them to the oracle with ==).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -586,5 +587,127 @@ extern "C" int sgld_unit_compliance(const double *U, int64_t numDOFs, const int3
                        dU.p, dN.p, numElements, dK.p, dce.p);
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpy(ce, dce.p, sizeof(double) * numElements, hipMemcpyDeviceToHost));
+    return SGLD_OK;
+}
+
+// ===================== allocation-free restriction =====================
+// sgld_restrict_noalloc: the restriction of sgld_restrict on device arrays,
+// for callers that cannot allocate device memory.  Its only scratch -- the
+// ghost-node array the reference mxMallocs per call
+// (Solving_Restriction_MatrixFree_mex.cpp:72,190), 3 x nGhost doubles, and a
+// validation word -- lives in the fixed device array `scratch`, which every
+// GPU the library runs on holds until the process ends.  Calls that overlap
+// on one GPU share it and race.
+namespace {
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;  // 32 MiB
+constexpr size_t kNoallocHeader = 256;                     // validation word, then the ghost array
+constexpr int64_t kNoallocMaxGhost = (int64_t)((kNoallocScratchBytes - kNoallocHeader) / (3 * sizeof(double)));
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+unsigned char *g_scratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy at its own address).
+int scratch_address(unsigned char **out) {
+    int dev = 0;
+    SGLD_HIP(hipGetDevice(&dev));
+    SGLD_ARG(dev >= 0 && dev < kMaxDevices, "sgld:arg: device %d beyond the %d this library tracks", dev,
+             kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void *p = nullptr;
+        SGLD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratchPtr[dev];
+    return SGLD_OK;
+}
+
+__global__ void k_fill_f64(double *__restrict__ v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = 0.0;
+}
+__global__ void k_fill_i32(int *__restrict__ v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = 0;
+}
+// flag = code if any id lies outside 1..hi (the first failing check wins).
+__global__ void k_check_ids(const int32_t *__restrict__ a, int64_t n, int64_t hi, int code, int *flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (a[i] < 1 || a[i] > hi) atomicCAS(flag, 0, code);
+}
+
+}  // namespace
+
+extern "C" int sgld_restrict_noalloc(const double *rF, int64_t nF, const int32_t *transferMat, int64_t m,
+                                     const double *RIT, int64_t ritRows, int64_t ritCols,
+                                     const int32_t *eNodMatC, int64_t numElesC, int64_t nC, int64_t nGhost,
+                                     const int32_t *map, const double *coeffi, const int32_t *colors,
+                                     const int64_t *off, int numColors, double *Y) {
+    SGLD_ARG(ritRows == 8 && ritCols == m && RIT,
+             "MATLAB:mexFunction:inputNotMx8: Input RIT must be a 8x 125/27 double matrix.");
+    SGLD_ARG(m == 27 || m == 125,
+             "MATLAB:mexFunction:inputNotMatrix: Input transferMat must be an 27xM or 125xM int32 matrix.");
+    SGLD_ARG(nF >= 0 && nC >= 0 && nGhost >= 0 && numElesC >= 0, "sgld:arg: negative size");
+    SGLD_ARG(nGhost <= kNoallocMaxGhost,
+             "sgld:arg: nGhost = %lld exceeds the %lld ghost nodes of the allocation-free restriction's "
+             "32 MiB scratch", (long long)nGhost, (long long)kNoallocMaxGhost);
+    SGLD_ARG(numElesC == 0 || (transferMat && eNodMatC),
+             "MATLAB:mexFunction:inputNotMatrix: Input eNodMat must be an Mx8 int32 matrix.");
+    SGLD_ARG(nF == 0 || (map && coeffi && rF),
+             "MATLAB:mexFunction:inputNotVector: Input solidNodeMapCoarser2Finer must be a column vector.");
+    SGLD_ARG(nC == 0 || Y, "sgld:arg: Y is null");
+    SGLD_ARG(numColors >= 0, "Solving_KbyU:colorArray: colour count must be >= 0");
+    if (numColors) {
+        SGLD_ARG(off && off[0] == 0, "Solving_KbyU:colorCell: colorOffsets[0] must be 0");
+        for (int c = 0; c < numColors; ++c)
+            SGLD_ARG(off[c + 1] >= off[c], "Solving_KbyU:colorCell: colorOffsets must not decrease");
+        SGLD_ARG(off[numColors] == 0 || colors, "Solving_KbyU:colorCell: colors is null");
+    }
+    if (nC == 0) return SGLD_OK;
+    unsigned char *base = nullptr;
+    SGLD_TRY(scratch_address(&base));
+    int *flag = (int *)base;
+    double *ghost = (double *)(base + kNoallocHeader);
+    // what must start at zero: the flag, the ghost array, the output
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, 0, flag, 64);
+    if (nGhost)
+        hipLaunchKernelGGL(k_fill_f64, dim3(sgld::bounded_blocks(3 * nGhost, 256)), dim3(256), 0, 0, ghost,
+                           3 * nGhost);
+    hipLaunchKernelGGL(k_fill_f64, dim3(sgld::bounded_blocks(3 * nC, 256)), dim3(256), 0, 0, Y, 3 * nC);
+    // every id checked on the device before any kernel indexes with it
+    const int64_t nIds = numColors ? off[numColors] : 0;
+    struct Chk { const int32_t *a; int64_t n, hi; } chk[4] = {
+        {transferMat, m * numElesC, nGhost}, {eNodMatC, 8 * numElesC, nC}, {map, nF, nGhost},
+        {colors, nIds, numElesC}};
+    for (int k = 0; k < 4; ++k)
+        if (chk[k].n > 0)
+            hipLaunchKernelGGL(k_check_ids, dim3(sgld::bounded_blocks(chk[k].n, 256)), dim3(256), 0, 0, chk[k].a,
+                               chk[k].n, chk[k].hi, k + 1, flag);
+    SGLD_HIP(hipGetLastError());
+    int bad = 0;
+    SGLD_HIP(hipMemcpy(&bad, flag, sizeof(int), hipMemcpyDeviceToHost));
+    static const char *const names[4] = {"transferMat", "eNodMat", "solidNodeMapCoarser2Finer", "colors"};
+    SGLD_ARG(bad == 0, "sgld:arg: %s holds an id outside 1..%lld", names[bad > 0 ? bad - 1 : 0],
+             (long long)(bad > 0 ? chk[bad - 1].hi : 0));
+    if (nF) {
+        hipLaunchKernelGGL(k_ghost_fill, dim3(sgld::blocks_for(nF, 256)), dim3(256), 0, 0, rF, nF, map, coeffi,
+                           nGhost, ghost);
+        SGLD_HIP(hipGetLastError());
+    }
+    for (int c = 0; c < numColors; ++c) {
+        const int64_t n = off[c + 1] - off[c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_restrict_colour, dim3(sgld::blocks_for(n, 256)), dim3(256), 0, 0, ghost, nGhost,
+                           transferMat, (int)m, RIT, eNodMatC, numElesC, nC, colors + off[c], n, Y);
+        SGLD_HIP(hipGetLastError());
+    }
+    SGLD_HIP(hipStreamSynchronize(nullptr));
     return SGLD_OK;
 }
