@@ -15,18 +15,9 @@
 namespace sgld {
 namespace {
 
-__constant__ double cKe64[576];
-__constant__ float cKe32[576];
-// Mode-space element stiffness of the isotropic cube (see mode_forces).
-__constant__ double cKm64[9];
-__constant__ float cKm32[9];
-
-template <class T> __device__ __forceinline__ T ke(int i);
-template <> __device__ __forceinline__ double ke<double>(int i) { return cKe64[i]; }
-template <> __device__ __forceinline__ float ke<float>(int i) { return cKe32[i]; }
-template <class T> __device__ __forceinline__ T km9(int i);
-template <> __device__ __forceinline__ double km9<double>(int i) { return cKm64[i]; }
-template <> __device__ __forceinline__ float km9<float>(int i) { return cKm32[i]; }
+// The element matrix is per context: the dense 24 x 24 Ke by pointer to the
+// context's device copy, the nine mode-space coefficients (ModeK, see
+// mode_forces) by value as a kernel argument.
 
 // Local node j of the corner (x, y, z) bits: node order of FEA_GaussianIntegral.m:2-9.
 __host__ __device__ constexpr int corner_index(int x, int y, int z) {
@@ -84,7 +75,7 @@ template <> struct FineW<float> {
 // ----------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__restrict__ y,
-                                              const T *__restrict__ E,
+                                              const T *__restrict__ Ke, const T *__restrict__ E,
                                               const uint8_t *__restrict__ nflag, Grid gd, Span S,
                                               double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
@@ -116,15 +107,15 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
                             const int zk = (k >= 4);
                             const int64_t gk = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
                             const T u0 = u[3 * gk], u1 = u[3 * gk + 1], u2 = u[3 * gk + 2];
-                            s0 += ke<T>((3 * j + 0) + 24 * (3 * k + 0)) * u0 +
-                                  ke<T>((3 * j + 0) + 24 * (3 * k + 1)) * u1 +
-                                  ke<T>((3 * j + 0) + 24 * (3 * k + 2)) * u2;
-                            s1 += ke<T>((3 * j + 1) + 24 * (3 * k + 0)) * u0 +
-                                  ke<T>((3 * j + 1) + 24 * (3 * k + 1)) * u1 +
-                                  ke<T>((3 * j + 1) + 24 * (3 * k + 2)) * u2;
-                            s2 += ke<T>((3 * j + 2) + 24 * (3 * k + 0)) * u0 +
-                                  ke<T>((3 * j + 2) + 24 * (3 * k + 1)) * u1 +
-                                  ke<T>((3 * j + 2) + 24 * (3 * k + 2)) * u2;
+                            s0 += Ke[(3 * j + 0) + 24 * (3 * k + 0)] * u0 +
+                                  Ke[(3 * j + 0) + 24 * (3 * k + 1)] * u1 +
+                                  Ke[(3 * j + 0) + 24 * (3 * k + 2)] * u2;
+                            s1 += Ke[(3 * j + 1) + 24 * (3 * k + 0)] * u0 +
+                                  Ke[(3 * j + 1) + 24 * (3 * k + 1)] * u1 +
+                                  Ke[(3 * j + 1) + 24 * (3 * k + 2)] * u2;
+                            s2 += Ke[(3 * j + 2) + 24 * (3 * k + 0)] * u0 +
+                                  Ke[(3 * j + 2) + 24 * (3 * k + 1)] * u1 +
+                                  Ke[(3 * j + 2) + 24 * (3 * k + 2)] * u2;
                         }
                         acc[0] += Ee * s0;
                         acc[1] += Ee * s1;
@@ -149,7 +140,7 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
 // (m = x + 2y + 4z difference bits) of component a on entry, the force modes
 // E K' uh on exit (in place).
 template <class T>
-__device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
+__device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee, const ModeK<T> &km) {
     // K' is block diagonal: modes couple only within {(x,1),(y,2),(z,4)},
     // {(x,6),(y,5),(z,3)}, the pairs {(a,e_b),(b,e_a)} and {(a,e_a+e_c),
     // (b,e_b+e_c)}, and (a,7) alone; mode 0 (translation) carries no force.
@@ -157,9 +148,8 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee) {
     // numbers define it (km9: own-axis linear, cross-axis linear, bilinear
     // with own axis, bilinear without, trilinear, then the four couplings).
     // E scales the nine coefficients (9 multiplies instead of 21 on the modes)
-    const T dLo = Ee * km9<T>(0), dLx = Ee * km9<T>(1), dBo = Ee * km9<T>(2), dBx = Ee * km9<T>(3),
-            dT = Ee * km9<T>(4);
-    const T c0 = Ee * km9<T>(5), c1 = Ee * km9<T>(6), c2 = Ee * km9<T>(7), c3 = Ee * km9<T>(8);
+    const T dLo = Ee * km.v[0], dLx = Ee * km.v[1], dBo = Ee * km.v[2], dBx = Ee * km.v[3], dT = Ee * km.v[4];
+    const T c0 = Ee * km.v[5], c1 = Ee * km.v[6], c2 = Ee * km.v[7], c3 = Ee * km.v[8];
     {
         const T x = uh[0][1], yv = uh[1][2], z = uh[2][4];
         uh[0][1] = dLo * x + c0 * (yv + z);
@@ -287,7 +277,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                                                        const uint8_t *__restrict__ nflag, Grid gd,
                                                        TileMap tm, const int32_t *__restrict__ wlist,
                                                        int64_t wn, double *__restrict__ partial,
-                                                       const Ctl *ctl) {
+                                                       const Ctl *ctl, ModeK<T> km) {
     if (stopped(ctl)) return;
     constexpr int kSlots = OneBar ? 4 : 3, kAcc = OneBar ? 3 : 2;
     __shared__ T us[kSlots][kTN * kTN * 3];
@@ -407,7 +397,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_tiled(cons
                         Flo[a][q] = Fhi[a][q];
                     }
             }
-            mode_forces<T>(m, eokCur ? Ecur : T(0));
+            mode_forces<T>(m, eokCur ? Ecur : T(0), km);
             if (el >= Z0) {
                 // lower face complete: inverse (x, y) transform, scatter 4 nodes
                 // (force mode 0 is zero: written out, since x + 0 is not folded)
@@ -516,17 +506,23 @@ template <> __device__ __forceinline__ double row_shr1<double>(double v) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-template <class T>
+// StageOut: node sums go through a double-buffered 16 x 16-node LDS layer and
+// are stored one step later with contiguous full-wavefront stores (value
+// t + 256 q of the layer); otherwise every thread stores its node's three
+// values itself.
+template <class T, bool StageOut, bool SkipVoid>
 __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
                                                        TileMap tm, const int32_t *__restrict__ wlist,
                                                        int64_t wn, double *__restrict__ partial,
-                                                       const Ctl *ctl) {
+                                                       const Ctl *ctl, ModeK<T> km) {
     if (stopped(ctl)) return;
     constexpr int kSlots = 4;
     __shared__ T us[kSlots][kTN * kTN * 3];
     __shared__ T xch[2][3][256];  // D + B(ei-1) of every element column, two element layers
+    constexpr int kON = 3 * kTE * kTE;       // staged output values per node layer
+    __shared__ T ost[StageOut ? 2 : 1][StageOut ? kON : 1];
     const int t = threadIdx.x;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
     const int64_t pl = (int64_t)NY * NX;       // nodes per layer
@@ -564,6 +560,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
         const bool onode = ei >= 1 && ej >= 1 && er < NY && ec < NX;
         const uint32_t fOff = onode ? (uint32_t)(ei + NY * ej) : kOob;
         const uint32_t yOff = onode ? (uint32_t)sizeof(T) * (uint32_t)(3 * (ei + NY * ej)) : kOob;
+        // staged outputs: value j = t + 256 q of a layer is node (row, col) =
+        // ((j % 48) / 3, j / 48) of the 16 x 16 window, component j % 3
+        uint32_t oOff[kON / 256];
+#pragma unroll
+        for (int q = 0; q < kON / 256; ++q) {
+            const int j = t + 256 * q, col = j / (3 * kTE), off = j - 3 * kTE * col, row = off / 3;
+            const bool ok = row >= 1 && col >= 1 && R0 - 1 + row < NY && C0 - 1 + col < NX;
+            oOff[q] = ok ? (uint32_t)sizeof(T) * (uint32_t)(3 * NY * col + off) : kOob;
+        }
         const int64_t sBase = 3 * ((int64_t)(R0 - 1) + (int64_t)NY * (C0 - 1));
         const uint32_t vBytes = (uint32_t)sizeof(T) * (uint32_t)(3 * NY * (kTN - 1) + 3 * kTN);
         const uint32_t fBytes = (uint32_t)(NY * (kTN - 1) + kTN);
@@ -598,6 +603,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
             for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
         T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
         bool eokCur = true;
+        // stores staged node layer L (its values are in ost[L & 1])
+        auto flush = [&](int L) {
+            const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * L, vBytes);
+#pragma unroll
+            for (int q = 0; q < kON / 256; ++q) buf_store(ost[L & 1][t + 256 * q], yr, oOff[q]);
+        };
         auto step = [&](int el, T (&ready)[kStageSlots], T (&spare)[kStageSlots]) {
             const bool out = el >= Z0;
             const __amdgpu_buffer_rsrc_t fr = buf_rsrc(nflag + sBase / 3 + pl * el, out ? fBytes : 0u);
@@ -618,7 +629,17 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
                         Flo[a][q] = Fhi[a][q];
                     }
             }
-            mode_forces<T>(m, eokCur ? Ecur : T(0));
+            // a wavefront whose 64 elements of this layer are all void (E = 0,
+            // or outside the lattice) has zero forces: skip their arithmetic
+            const T Eel = eokCur ? Ecur : T(0);
+            if (!SkipVoid || __builtin_amdgcn_ballot_w64(Eel != T(0)) != 0) {
+                mode_forces<T>(m, Eel, km);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) m[a][q] = T(0);
+            }
             T left[3] = {T(0), T(0), T(0)};
             T *xw = &xch[el & 1][0][0];
             if (out) {
@@ -643,13 +664,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
             eokCur = eok;
             stage(sNext, ready);  // layer el+2 into the slot of layer el-2
             __syncthreads();      // corner sums of layer el exchanged
+            if (StageOut && el - 1 >= Z0) flush(el - 1);  // staged in the previous step
             if (out) {
                 const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * el, vBytes);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const T v0 = left[a] + (ej >= 1 ? xw[256 * a + t - kTE] : T(0));
                     const T v = ((fl & kSolid) && !(fl & (1u << a))) ? v0 : T(0);
-                    buf_store(v, yr, yOff + (uint32_t)sizeof(T) * (uint32_t)a);
+                    if constexpr (StageOut) ost[el & 1][3 * t + a] = v;
+                    else buf_store(v, yr, yOff + (uint32_t)sizeof(T) * (uint32_t)a);
                     dotacc += (double)sPrev[f01 + a] * (double)v;
                 }
             }
@@ -664,6 +687,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
         for (int el = Z0 - 1; el < Zend; el += 2) {
             step(el, pf, pn);
             if (el + 1 < Zend) step(el + 1, pn, pf);
+        }
+        if constexpr (StageOut) {
+            __syncthreads();  // the last layer staged
+            if (Zend - 1 >= Z0) flush(Zend - 1);
         }
     }
     if (partial) {
@@ -1443,7 +1470,8 @@ __global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial
 // ------------------------------- setup kernels ------------------------------
 // Finest diagonal: diag = sum_e E_e Ke_jj (Solving_AssembleFEAstencil.m:102-121);
 // dinv = 2 omega / diag on solid nodes.
-__global__ void k_diag0(const double *__restrict__ E, Grid gd, const uint8_t *__restrict__ nflag,
+__global__ void k_diag0(const double *__restrict__ Ke, const double *__restrict__ E, Grid gd,
+                        const uint8_t *__restrict__ nflag,
                         double omega, double *__restrict__ dinv, float *__restrict__ dinv32,
                         uint16_t *__restrict__ dinv16) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1458,7 +1486,7 @@ __global__ void k_diag0(const double *__restrict__ E, Grid gd, const uint8_t *__
                 if (er < 0 || er >= gd.ny || ec < 0 || ec >= gd.nx || el < 0 || el >= gd.nz) continue;
                 const int j = corner_index(1 - dc, dr, 1 - dl);
                 const double Ee = E[er + (int64_t)gd.ny * ec + (int64_t)gd.ny * gd.nx * el];
-                for (int a = 0; a < 3; ++a) d[a] += cKe64[(3 * j + a) * 25] * Ee;
+                for (int a = 0; a < 3; ++a) d[a] += Ke[(3 * j + a) * 25] * Ee;
             }
     const bool solid = nflag[g] & kSolid;
     for (int a = 0; a < 3; ++a) {
@@ -1648,7 +1676,7 @@ __global__ void k_scatter_elems(const double *__restrict__ src, const int32_t *_
 
 // Unit compliance ce_e = u_e^T Ke u_e per lattice element
 // (TopOpti_CmptUnitCompliance_mex.cpp:61-107), then gathered to compressed order.
-__global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u, Grid gd,
+__global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ Ke, const double *__restrict__ u, Grid gd,
                                                     const int32_t *__restrict__ emap, int64_t ne,
                                                     double *__restrict__ ce) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1672,7 +1700,7 @@ __global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u
     for (int r = 0; r < 24; ++r) {
         double y = 0.0;
 #pragma unroll
-        for (int c = 0; c < 24; ++c) y += cKe64[r + 24 * c] * ue[c];
+        for (int c = 0; c < 24; ++c) y += Ke[r + 24 * c] * ue[c];
         q += ue[r] * y;
     }
     ce[i] = q;
@@ -1682,7 +1710,7 @@ __global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ u
 // TopOpti_DeMaterialInterpolation.m:5): ce = u_e^T Ke u_e,
 // dc_e = -penal (E0 - Emin) x_e^(penal-1) ce * scale, and the partial of the
 // compliance sum_e E_e ce (E_e the modulus on the lattice).
-__global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ u, Grid gd,
+__global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ Ke, const double *__restrict__ u, Grid gd,
                                                      const int32_t *__restrict__ emap, int64_t ne,
                                                      const double *__restrict__ x, const double *__restrict__ E,
                                                      double penal, double dE, double scale,
@@ -1708,7 +1736,7 @@ __global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ 
         for (int r = 0; r < 24; ++r) {
             double y = 0.0;
 #pragma unroll
-            for (int c = 0; c < 24; ++c) y += cKe64[r + 24 * c] * ue[c];
+            for (int c = 0; c < 24; ++c) y += Ke[r + 24 * c] * ue[c];
             q += ue[r] * y;
         }
         dc[i] = -penal * dE * pow(x[i], penal - 1.0) * q * scale;
@@ -1797,6 +1825,12 @@ template <> const uint16_t *fine_dinv<float>(sgld_ctx *C) { return C->lv[0].dinv
 template <class T> const T *ctx_E(sgld_ctx *C);
 template <> const double *ctx_E<double>(sgld_ctx *C) { return C->E.p; }
 template <> const float *ctx_E<float>(sgld_ctx *C) { return C->E32.p; }
+template <class T> const T *ctx_Ke(sgld_ctx *C);
+template <> const double *ctx_Ke<double>(sgld_ctx *C) { return C->KeDev.p; }
+template <> const float *ctx_Ke<float>(sgld_ctx *C) { return C->KeDev32.p; }
+template <class T> ModeK<T> ctx_km(const sgld_ctx *C);
+template <> ModeK<double> ctx_km<double>(const sgld_ctx *C) { return C->km64; }
+template <> ModeK<float> ctx_km<float>(const sgld_ctx *C) { return C->km32; }
 
 double *slot(sgld_ctx *C, int k) { return C->partial.p + (int64_t)k * kRedBlocks; }
 
@@ -1845,6 +1879,14 @@ int kt_harvest(sgld_ctx *C, int worked) {
     return SGLD_OK;
 }
 
+const char *owner_name(bool f64, bool stage, bool skip) {
+    static const char *const n[8] = {"k_kbyu_owner<float, false, false>", "k_kbyu_owner<float, false, true>",
+                                     "k_kbyu_owner<float, true, false>",  "k_kbyu_owner<float, true, true>",
+                                     "k_kbyu_owner<double, false, false>", "k_kbyu_owner<double, false, true>",
+                                     "k_kbyu_owner<double, true, false>",  "k_kbyu_owner<double, true, true>"};
+    return n[4 * f64 + 2 * stage + skip];
+}
+
 // y = K u; with zin != null first pout = zin + beta u (the PCG direction
 // update of the previous iteration) and y = K pout.
 template <class T>
@@ -1862,29 +1904,35 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     constexpr bool f64 = sizeof(T) == 8;
     SGLD_TRY(kt_begin(C, f64 ? KT_KU64 : KT_KU32,
                       C->kbyuVariant != 1 ? (f64 ? "k_kbyu<double>" : "k_kbyu<float>")
-                      : C->kbyuOwner ? (f64 ? "k_kbyu_owner<double>" : "k_kbyu_owner<float>")
+                      : C->kbyuOwner ? owner_name(f64, C->kbyuStageOut[f64 ? 0 : 1], !C->full && C->kbyuSkipVoid)
                       : C->kbyuOneBar ? (f64 ? "k_kbyu_tiled<double, true>" : "k_kbyu_tiled<float, true>")
                                       : (f64 ? "k_kbyu_tiled<double, false>" : "k_kbyu_tiled<float, false>")));
     if (C->kbyuVariant == 1) {
         const int q = f64 ? 0 : 1;
         const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
-        if (C->kbyuOwner)
-            hipLaunchKernelGGL(k_kbyu_owner<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p, l0.g,
-                               ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial,
-                               ctl);
+        if (C->kbyuOwner) {
+            // voids are skipped per wavefront only on irregular domains
+            auto kern = C->kbyuStageOut[q] ? (C->full || !C->kbyuSkipVoid ? k_kbyu_owner<T, true, false>
+                                                                             : k_kbyu_owner<T, true, true>)
+                                           : (C->full || !C->kbyuSkipVoid ? k_kbyu_owner<T, false, false>
+                                                                             : k_kbyu_owner<T, false, true>);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p, l0.g, ts.tm,
+                               C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial, ctl,
+                               ctx_km<T>(C));
+        }
         else if (C->kbyuOneBar)
             hipLaunchKernelGGL((k_kbyu_tiled<T, true>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
                                l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
-                               C->nKwork[q], partial, ctl);
+                               C->nKwork[q], partial, ctl, ctx_km<T>(C));
         else
             hipLaunchKernelGGL((k_kbyu_tiled<T, false>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
                                l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
-                               C->nKwork[q], partial, ctl);
+                               C->nKwork[q], partial, ctl, ctx_km<T>(C));
     } else {
         const Span S = node_span(C);
         nb = grid_blocks(S.count);
-        hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p,
+        hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_Ke<T>(C), ctx_E<T>(C), l0.nflag.p,
                            l0.g, S, partial, ctl);
     }
     SGLD_HIP(hipGetLastError());
@@ -1907,7 +1955,7 @@ bool mode_coefficients(const double *Ke, double *km9) {
                 if ((m >> d) & 1) f *= c[d] ? 1.0 : -1.0;
             H[m][j] = f;
         }
-    static double Kp[24][24];
+    std::vector<std::vector<double>> Kp(24, std::vector<double>(24));
     double maxabs = 0;
     for (int a = 0; a < 3; ++a)
         for (int m = 0; m < 8; ++m)
@@ -1960,11 +2008,13 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int nb = blocks_for(Cc.g.nodes(), 256);
     // by size and precision: the tiled kernel marches 2 czc + 1 fine layers per
-    // block, which pays off only in FP64 and with enough coarse nodes to fill
-    // the device (256x128x128 first level, FP64 / FP32: tiled 31 / 17 us,
-    // node-centric 35 / 13 us)
-    const bool tiled = C->restrictTiled < 0 ? (sizeof(T) == 8 && Cc.g.nodes() >= C->restrictTiledMin)
-                                            : C->restrictTiled != 0;
+    // block, which pays off with enough coarse nodes to fill the device, in
+    // FP32 only beyond the Infinity Cache (256x128x128 first level, FP64 /
+    // FP32: tiled 31 / 17 us, node-centric 35 / 13 us; 512^3 ellipsoids FP32
+    // node-centric 222 us for 0.56 GB)
+    const bool tiled = C->restrictTiled < 0
+                           ? Cc.g.nodes() >= (sizeof(T) == 8 ? C->restrictTiledMin : C->restrictTiledMin32)
+                           : C->restrictTiled != 0;
     const NodeDiv dc = NodeDiv::make(Cc.g);
     if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
@@ -2367,29 +2417,19 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     return SGLD_OK;
 }
 
-// The element matrix (and its mode-space coefficients) live in constant
-// memory, shared by every context of the process.  A context whose Ke differs
-// from the uploaded one (another Poisson ratio or cell size) re-uploads them
-// before its kernels run, after the device has drained work that read them.
-double g_constKe[576];
-bool g_constValid = false;
-
-int use_constants(const sgld_ctx *C) {
-    if (g_constValid && !std::memcmp(g_constKe, C->Ke, sizeof(g_constKe))) return SGLD_OK;
-    if (g_constValid) SGLD_HIP(hipDeviceSynchronize());
-    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe64), C->Ke, sizeof(double) * 576));
+// The context's element matrix on the device (dense FP64 and FP32 copies)
+// and its mode-space coefficients, set once when the context is created.
+int element_data(sgld_ctx *C) {
+    SGLD_TRY(C->KeDev.upload(C->Ke, 576));
     float k32[576];
     for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
-    SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKe32), k32, sizeof(float) * 576));
+    SGLD_TRY(C->KeDev32.upload(k32, 576));
     double km[9];
-    if (mode_coefficients(C->Ke, km)) {
-        float km32[9];
-        for (int i = 0; i < 9; ++i) km32[i] = (float)km[i];
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm64), km, sizeof(double) * 9));
-        SGLD_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cKm32), km32, sizeof(float) * 9));
-    }
-    std::memcpy(g_constKe, C->Ke, sizeof(g_constKe));
-    g_constValid = true;
+    if (mode_coefficients(C->Ke, km))
+        for (int i = 0; i < 9; ++i) {
+            C->km64.v[i] = km[i];
+            C->km32.v[i] = (float)km[i];
+        }
     return SGLD_OK;
 }
 
@@ -2558,7 +2598,6 @@ int check_ready(sgld_ctx *C) {
         set_error("sgld:state: call sgld_ctx_set_fixed, sgld_ctx_set_modulus and sgld_ctx_assemble first");
         return SGLD_ERR_STATE;
     }
-    SGLD_TRY(use_constants(C));
     return SGLD_OK;
 }
 
@@ -2580,6 +2619,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (nonDyadic == 1 && numLevels >= 4) numLevels -= 1; else nonDyadic = 0;
     C->L = numLevels;
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
+    if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN32")) C->restrictTiledMin32 = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_BRESTRICT_TILED")) C->bRestrictTiled = std::atoi(env) != 0;
@@ -2741,7 +2781,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     }
     SGLD_TRY(C->partial.alloc((int64_t)kRedSlots * kRedBlocks));
     SGLD_TRY(C->ctl.alloc(1));
-    SGLD_TRY(use_constants(C.get()));
+    SGLD_TRY(element_data(C.get()));
     double km[9];
     C->kbyuVariant = mode_coefficients(C->Ke, km) ? 1 : 0;
     if (const char *env = std::getenv("SGLD_KBYU")) {
@@ -2759,9 +2799,22 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         // owner-computes node sums (no LDS atomics, run-to-run identical);
         // SGLD_KU_OWNER=0 selects the atomic-accumulation kernel
         if (const char *env = std::getenv("SGLD_KU_OWNER")) C->kbyuOwner = std::atoi(env) != 0;
+        // Outputs staged through LDS once the output vector outgrows the 256 MB
+        // Infinity Cache share (its partial-line stores then reach HBM):
+        // 256x128x128 direct / staged FP64 57.6 / 67.5 us, FP32 36.1 / 38.2 us;
+        // 125 M-DOF ellipsoids 914 / 867 and 560 / 483 us (the LDS atomic kernel:
+        // 63.5, 47.0, 885, 513 us)
+        for (int q = 0; q < 2; ++q) C->kbyuStageOut[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
+        if (const char *env = std::getenv("SGLD_KU_SKIP_VOID")) C->kbyuSkipVoid = std::atoi(env) != 0;
+        if (const char *env = std::getenv("SGLD_KU_STAGE_OUT"))
+            C->kbyuStageOut[0] = C->kbyuStageOut[1] = std::atoi(env) != 0;
         if (C->kbyuOwner) {
-            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_owner<double>, 256, 0));
-            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_owner<float>, 256, 0));
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per[0], C->kbyuStageOut[0] ? k_kbyu_owner<double, true, false> : k_kbyu_owner<double, false, false>,
+                256, 0));
+            SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per[1], C->kbyuStageOut[1] ? k_kbyu_owner<float, true, false> : k_kbyu_owner<float, false, false>,
+                256, 0));
         } else if (C->kbyuOneBar) {
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double, true>, 256, 0));
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_kbyu_tiled<float, true>, 256, 0));
@@ -2880,7 +2933,6 @@ namespace {
 int assemble_impl(sgld_ctx *C, double omega) {
     SGLD_ARG(C, "sgld:arg: null context");
     SGLD_ARG(omega > 0 && omega < 2, "sgld:arg: omega must be in (0, 2)");
-    SGLD_TRY(use_constants(C));  // the finest diagonal reads Ke
     if (!C->haveFixed || !C->haveModulus) {
         set_error("sgld:state: set fixed DOFs and modulus before assembling");
         return SGLD_ERR_STATE;
@@ -2888,7 +2940,7 @@ int assemble_impl(sgld_ctx *C, double omega) {
     const double t0 = now_s();
     C->omega = omega;
     Level &l0 = C->lv[0];
-    hipLaunchKernelGGL(k_diag0, dim3(blocks_for(l0.g.nodes(), 256)), dim3(256), 0, C->s, C->E.p, l0.g,
+    hipLaunchKernelGGL(k_diag0, dim3(blocks_for(l0.g.nodes(), 256)), dim3(256), 0, C->s, C->KeDev.p, C->E.p, l0.g,
                        l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p, l0.dinv16.p);
     SGLD_HIP(hipGetLastError());
     DevBuf<double> KsPrev, KsCur;
@@ -3110,13 +3162,12 @@ extern "C" int sgld_ctx_pcg(sgld_ctx *C, const double *F, double *U, double tol,
 
 extern "C" int sgld_ctx_unit_compliance(sgld_ctx *C, const double *U, double *ce) {
     SGLD_ARG(C && U && ce, "sgld:arg: null argument");
-    SGLD_TRY(use_constants(C));
     const int64_t nc = 3 * C->lv[0].numNodes, ne = C->lv[0].numElements;
     DevBuf<double> dU, dce;
     SGLD_TRY(dU.upload(U, nc));
     SGLD_TRY(dce.alloc(ne));
     SGLD_TRY(to_lattice(C, dU.p, C->z.p));
-    hipLaunchKernelGGL(k_compliance, dim3(blocks_for(ne, 256)), dim3(256), 0, C->s, C->z.p, C->lv[0].g,
+    hipLaunchKernelGGL(k_compliance, dim3(blocks_for(ne, 256)), dim3(256), 0, C->s, C->KeDev.p, C->z.p, C->lv[0].g,
                        C->full ? nullptr : C->eleMap.p, ne, dce.p);
     SGLD_HIP(hipGetLastError());
     SGLD_HIP(hipMemcpyAsync(ce, dce.p, sizeof(double) * ne, hipMemcpyDeviceToHost, C->s));
@@ -3126,7 +3177,6 @@ extern "C" int sgld_ctx_unit_compliance(sgld_ctx *C, const double *U, double *ce
 
 extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, int precision) {
     SGLD_ARG(C && du && dy, "sgld:arg: null argument");
-    SGLD_TRY(use_constants(C));
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
     SGLD_TRY(to_lattice(C, du, C->z.p));
     if (precision == SGLD_PREC_FP64) {
@@ -3155,7 +3205,6 @@ extern "C" int sgld_ctx_kbyu_device(sgld_ctx *C, const double *du, double *dy, i
 extern "C" int sgld_ctx_bench_kbyu(sgld_ctx *C, int precision, int reps, double *msPerLaunch) {
     SGLD_ARG(C && msPerLaunch && reps > 0, "sgld:arg: bad argument");
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: set fixed DOFs and modulus first");
-    SGLD_TRY(use_constants(C));
     SGLD_HIP(hipStreamSynchronize(C->s));
     SGLD_HIP(hipEventRecord(C->e0, C->s));
     for (int i = 0; i < reps; ++i) {
@@ -3205,13 +3254,12 @@ extern "C" int sgld_ctx_kernel_times(const sgld_ctx *C, double *out) {
 extern "C" int sgld_ctx_sensitivity_device(sgld_ctx *C, const double *dU, const double *dx, double penal,
                                            double E0, double Emin, double scale, double *ddc, double *compliance) {
     SGLD_ARG(C && dU && dx && ddc, "sgld:arg: null argument");
-    SGLD_TRY(use_constants(C));
     SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
     SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
     const int64_t ne = C->lv[0].numElements;
     SGLD_TRY(to_lattice(C, dU, C->z.p));
     const int nb = grid_blocks(ne);
-    hipLaunchKernelGGL(k_sensitivity, dim3(nb), dim3(256), 0, C->s, C->z.p, C->lv[0].g,
+    hipLaunchKernelGGL(k_sensitivity, dim3(nb), dim3(256), 0, C->s, C->KeDev.p, C->z.p, C->lv[0].g,
                        C->full ? nullptr : C->eleMap.p, ne, dx, C->E.p, penal, E0 - Emin, scale, ddc, slot(C, 3));
     SGLD_HIP(hipGetLastError());
     std::vector<double> part(nb);
@@ -3239,7 +3287,6 @@ extern "C" int sgld_ctx_sensitivity(sgld_ctx *C, const double *U, const double *
 extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
     SGLD_ARG(C && report && reps > 0, "sgld:arg: bad arguments");
     SGLD_ARG(C->haveModulus && C->haveFixed, "sgld:state: call sgld_ctx_set_fixed and sgld_ctx_set_modulus first");
-    SGLD_TRY(use_constants(C));
     const int variant0 = C->kbyuVariant;
     const int czs[4] = {8, 12, 16, 32};
     double tiled[2] = {1e300, 1e300};
@@ -3308,8 +3355,6 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         SGLD_TRY(check_ready(ctxs[i]));
         SGLD_ARG(dF[i] && dU[i], "sgld:arg: null vector of instance %d", i);
         for (int k = 0; k < i; ++k) SGLD_ARG(ctxs[k] != ctxs[i], "sgld:arg: instance %d repeats a context", i);
-        SGLD_ARG(!std::memcmp(ctxs[i]->Ke, ctxs[0]->Ke, sizeof(ctxs[0]->Ke)),
-                 "sgld:arg: instance %d has another element matrix (Poisson ratio / cell size) than instance 0", i);
     }
     SGLD_HIP(hipDeviceSynchronize());
     const double t0 = now_s();

@@ -84,6 +84,10 @@ struct sgld_ctx {
     int L = 0, nonDyadic = 0;
     double nu = 0.3, h = 1.0, omega = 0.6;
     double Ke[576];
+    sgld::DevBuf<double> KeDev;   // Ke on the device (FP64, column-major)
+    sgld::DevBuf<float> KeDev32;  // and in FP32
+    sgld::ModeK<double> km64{};   // mode-space coefficients of Ke (tiled K.u)
+    sgld::ModeK<float> km32{};
     std::vector<sgld::Level> lv;
     bool full = true;
     std::vector<int32_t> hNodMap, hEleMap;  // compressed -> lattice (0-based)
@@ -109,9 +113,14 @@ struct sgld_ctx {
     bool haveFixed = false, haveModulus = false, assembled = false;
     int kbyuVariant = 0;  // 0 node-centric dense Ke, 1 tiled mode-space
     bool kbyuOneBar = true;  // tiled K.u with one block barrier per element layer (else two)
-    bool kbyuOwner = false;  // tiled K.u with owner-computes node sums (else LDS atomics)
+    bool kbyuOwner = true;   // tiled K.u with owner-computes node sums (else LDS atomics)
+    // owner K.u, [0] FP64 / [1] FP32: outputs through LDS for contiguous stores
+    // (set when the output vector exceeds the Infinity Cache share, see sgld_ctx_create)
+    bool kbyuStageOut[2] = {true, true};
+    bool kbyuSkipVoid = false;  // owner K.u on irregular domains: skip all-void wavefronts
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
-    int64_t restrictTiledMin = (int64_t)1 << 18;  // "by size": tiled from this many coarse nodes
+    int64_t restrictTiledMin = (int64_t)1 << 18;    // "by size": FP64 tiled from this many coarse nodes
+    int64_t restrictTiledMin32 = (int64_t)1 << 40;  // and FP32
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
     bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)

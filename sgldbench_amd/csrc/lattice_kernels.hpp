@@ -64,6 +64,13 @@ struct NodeDiv {
     }
 };
 
+// The nine numbers of the isotropic cube's mode-space element stiffness
+// K' = H Ke H^T / 64 (see mode_forces in lattice.hip), a kernel argument.
+template <class T>
+struct ModeK {
+    T v[9];
+};
+
 // Device-side PCG state (one per solve); all solver kernels read it so that a
 // batch of iterations can be queued (or graph-captured) without host syncs.
 struct Ctl {

@@ -352,3 +352,63 @@ def test_coarsest_solve_is_dense_cholesky_solve(case):
     s = rng.standard_normal(F.size) * (1 - fixed)
     a, b = s @ g.vcycle(r), r @ g.vcycle(s)
     assert abs(a - b) <= 1e-12 * abs(a)
+
+
+@pytest.mark.parametrize("case", ["cuboid", "holes"])
+def test_kbyu_and_pcg_run_to_run_identical(case):
+    """K.u sums every node's element contributions in a fixed order, so FP64
+    K.u results and whole FP64 / mixed solves repeat bit for bit."""
+    vox = problems.cuboid(96, 48, 48) if case == "cuboid" else holes(64, 32, 32, 5)
+    print("case: determinism", case, flush=True)
+    g, m, F, fixed, E = build_pair(vox, 4)
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal(F.size) * (1 - fixed)
+    ys = [g.kbyu(u) for _ in range(4)]
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0])
+    for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+        U1, its1, _ = g.pcg(F, tol=1e-8, maxIT=500, precision=prec)
+        U2, its2, _ = g.pcg(F, tol=1e-8, maxIT=500, precision=prec)
+        assert its1 == its2 and np.array_equal(U1, U2), prec
+
+
+def test_contexts_with_different_element_matrices_interleave():
+    """Each context holds its own element matrix: two contexts with other
+    Poisson ratios and cell sizes, used alternately (and from two threads),
+    each match the oracle built with their own Ke."""
+    import threading
+    vox = holes(24, 16, 8, 4, nholes=2, r=2)
+    F, fixed = problems.cantilever3(vox)
+    E = problems.simp_modulus(np.random.default_rng(9).uniform(0.05, 1.0, int(vox.sum())))
+    pairs = []
+    for nu, h in ((0.3, 1.0), (0.2, 0.5)):
+        g = solver.FEASolver(vox, 3, poissonRatio=nu, cellSize=h)
+        g.set_fixed(fixed)
+        g.set_modulus(E)
+        g.assemble(0.6)
+        m = orc.Mesh(vox, 3, 0, nu, h)
+        m.set_fixed(fixed)
+        m.set_modulus(E)
+        m.assemble()
+        pairs.append((g, m))
+    u = np.random.default_rng(10).standard_normal(F.size) * (1 - fixed)
+    for _ in range(2):
+        for g, m in pairs:
+            yo = m.kbyu(u)
+            assert np.max(np.abs(g.kbyu(u) - yo)) <= 1e-13 * np.max(np.abs(yo))
+    out = {}
+
+    def run(k):
+        g, m = pairs[k]
+        out[k] = [g.pcg(F, tol=1e-8, maxIT=300) for _ in range(3)]
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k, (g, m) in enumerate(pairs):
+        Uo, itso, _, _ = m.pcg(F, 0.6, 1e-8, 300)
+        for U, its, rr in out[k]:
+            assert rr <= 1e-8 and abs(its - itso) <= 1
+            assert abs(F @ U - F @ Uo) <= 1e-9 * abs(F @ Uo)
