@@ -49,14 +49,28 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite f)
 }
 template <class T> struct FineW;
+// Loads of a vector that a pass streams once: nontemporal (NT) when the
+// pass's vectors outgrow the Infinity Cache, so that nothing of them would
+// survive to the next pass anyway (measured on 147 M-entry FP32 vectors: a
+// direction-update-shaped pass over 64-entry chunks 4.90 -> 5.33 TB/s; on the
+// 125 M-DOF ellipsoid solve, mixed 875 -> 840 ms, and 864 ms with p and r,
+// which the next kernel reads, left on cached loads), ordinary loads otherwise.
+template <bool NT, class V>
+__device__ __forceinline__ V ld_stream(const V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 template <> struct FineW<double> {
     using D = double;
-    __device__ __forceinline__ static double get(const double *p, int64_t i) { return p[i]; }
+    template <bool NT = false>
+    __device__ __forceinline__ static double get(const double *p, int64_t i) { return ld_stream<NT>(&p[i]); }
 };
 template <> struct FineW<float> {
     using D = uint16_t;
+    template <bool NT = false>
     __device__ __forceinline__ static float get(const uint16_t *p, int64_t i) {
-        return __uint_as_float((uint32_t)p[i] << 16);
+        return __uint_as_float((uint32_t)ld_stream<NT>(&p[i]) << 16);
     }
 };
 
@@ -510,7 +524,7 @@ template <> __device__ __forceinline__ double row_shr1<double>(double v) {
 // are stored one step later with contiguous full-wavefront stores (value
 // t + 256 q of the layer); otherwise every thread stores its node's three
 // values itself.
-template <class T, bool StageOut, bool SkipVoid>
+template <class T, bool StageOut>
 __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
@@ -629,17 +643,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
                         Flo[a][q] = Fhi[a][q];
                     }
             }
-            // a wavefront whose 64 elements of this layer are all void (E = 0,
-            // or outside the lattice) has zero forces: skip their arithmetic
-            const T Eel = eokCur ? Ecur : T(0);
-            if (!SkipVoid || __builtin_amdgcn_ballot_w64(Eel != T(0)) != 0) {
-                mode_forces<T>(m, Eel, km);
-            } else {
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) m[a][q] = T(0);
-            }
+            mode_forces<T>(m, eokCur ? Ecur : T(0), km);
             T left[3] = {T(0), T(0), T(0)};
             T *xw = &xch[el & 1][0][0];
             if (out) {
@@ -937,7 +941,7 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
 // stored: rho' = z.r is formed without it as x_2.r_2 + sum dinv r^2, because
 // P^T = R, r_2 = R r and r = 0 on fixed DOFs.
 // ----------------------------------------------------------------------------
-template <class T, int S>
+template <class T, int S, bool NT>
 __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, Grid gc, NodeDiv df,
                                                      const T *__restrict__ rf,
                                                      const typename FineW<T>::D *__restrict__ dinv,
@@ -946,15 +950,16 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
     if (stopped(ctl)) return;
     const T lam = (T)ctl->lambda, beta = (T)ctl->beta;
     const FastDiv d3 = FastDiv::make3();
-    FOR_SPAN(i, dofs) {  // one DOF per lane: every vector access is contiguous
+    // one DOF per lane: every vector access of a wavefront is contiguous
+    FOR_SPAN(i, dofs) {
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         const T pv = interp_dof<T, S>(xc, gc, df, n, a);
-        const T sv = FineW<T>::get(dinv, i) * rf[i];
+        const T sv = FineW<T>::template get<NT>(dinv, i) * ld_stream<NT>(&rf[i]);
         const uint8_t f = fflag[n];
+        const T po = ld_stream<NT>(&p[i]);
         const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
-        const T po = p[i];
-        x[i] = x[i] + lam * po;
+        x[i] = ld_stream<NT>(&x[i]) + lam * po;
         p[i] = z + beta * po;
     }
 }
@@ -1058,7 +1063,7 @@ __device__ __forceinline__ void block_sum2(double &a, double &b) {
 // Solving_PreconditionedConjugateGradientSolver.m:16-17: r -= lambda q with
 // partials of r.r (stopping test) and of sum dinv r^2 (the fine smoother's
 // share of the next rho).
-template <class T>
+template <class T, bool NT>
 __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
                                                   const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
@@ -1079,11 +1084,11 @@ __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__
     const T lt = (T)lambda;
     double rr = 0.0, dr = 0.0;
     FOR_SPAN(i, S) {
-        const T rn = r[i] - lt * q[i];
+        const T rn = ld_stream<NT>(&r[i]) - lt * ld_stream<NT>(&q[i]);
         r[i] = rn;
         const double rd = (double)rn;
         rr += rd * rd;
-        dr += (double)FineW<T>::get(dinv, i) * rd * rd;
+        dr += (double)FineW<T>::template get<NT>(dinv, i) * rd * rd;
     }
     block_sum2(rr, dr);
     if (threadIdx.x == 0) {
@@ -1879,12 +1884,19 @@ int kt_harvest(sgld_ctx *C, int worked) {
     return SGLD_OK;
 }
 
-const char *owner_name(bool f64, bool stage, bool skip) {
-    static const char *const n[8] = {"k_kbyu_owner<float, false, false>", "k_kbyu_owner<float, false, true>",
-                                     "k_kbyu_owner<float, true, false>",  "k_kbyu_owner<float, true, true>",
-                                     "k_kbyu_owner<double, false, false>", "k_kbyu_owner<double, false, true>",
-                                     "k_kbyu_owner<double, true, false>",  "k_kbyu_owner<double, true, true>"};
-    return n[4 * f64 + 2 * stage + skip];
+// Kernel names of the streaming passes: kind 0 direction update (span 2),
+// 1 direction update (span 4), 2 residual update; NT: nontemporal loads.
+const char *stream_name(bool f64, bool nt, int kind) {
+    static const char *const n[12] = {
+        "k_prolong_dir<float, 2, false>",  "k_prolong_dir<float, 2, true>",  "k_prolong_dir<float, 4, false>",
+        "k_prolong_dir<float, 4, true>",   "k_update_r<float, false>",       "k_update_r<float, true>",
+        "k_prolong_dir<double, 2, false>", "k_prolong_dir<double, 2, true>", "k_prolong_dir<double, 4, false>",
+        "k_prolong_dir<double, 4, true>",  "k_update_r<double, false>",      "k_update_r<double, true>"};
+    return n[6 * f64 + 2 * kind + nt];
+}const char *owner_name(bool f64, bool stage) {
+    static const char *const n[4] = {"k_kbyu_owner<float, false>", "k_kbyu_owner<float, true>",
+                                     "k_kbyu_owner<double, false>", "k_kbyu_owner<double, true>"};
+    return n[2 * f64 + stage];
 }
 
 // y = K u; with zin != null first pout = zin + beta u (the PCG direction
@@ -1904,7 +1916,7 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     constexpr bool f64 = sizeof(T) == 8;
     SGLD_TRY(kt_begin(C, f64 ? KT_KU64 : KT_KU32,
                       C->kbyuVariant != 1 ? (f64 ? "k_kbyu<double>" : "k_kbyu<float>")
-                      : C->kbyuOwner ? owner_name(f64, C->kbyuStageOut[f64 ? 0 : 1], !C->full && C->kbyuSkipVoid)
+                      : C->kbyuOwner ? owner_name(f64, C->kbyuStageOut[f64 ? 0 : 1])
                       : C->kbyuOneBar ? (f64 ? "k_kbyu_tiled<double, true>" : "k_kbyu_tiled<float, true>")
                                       : (f64 ? "k_kbyu_tiled<double, false>" : "k_kbyu_tiled<float, false>")));
     if (C->kbyuVariant == 1) {
@@ -1912,11 +1924,7 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
         const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
         if (C->kbyuOwner) {
-            // voids are skipped per wavefront only on irregular domains
-            auto kern = C->kbyuStageOut[q] ? (C->full || !C->kbyuSkipVoid ? k_kbyu_owner<T, true, false>
-                                                                             : k_kbyu_owner<T, true, true>)
-                                           : (C->full || !C->kbyuSkipVoid ? k_kbyu_owner<T, false, false>
-                                                                             : k_kbyu_owner<T, false, true>);
+            auto kern = C->kbyuStageOut[q] ? k_kbyu_owner<T, true> : k_kbyu_owner<T, false>;
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p, l0.g, ts.tm,
                                C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial, ctl,
                                ctx_km<T>(C));
@@ -2195,37 +2203,40 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     Ctl *ctl = C->ctl.p;
     Level &l0 = C->lv[0], &l1 = C->lv[1];
     const NodeDiv d0 = NodeDiv::make(l0.g);
-    const int nbv = grid_blocks(S.count);
+    const int nbv = (int)std::max<int64_t>(1, std::min<int64_t>(C->vecBlocks, (S.count + 255) / 256));
+    constexpr bool f64 = sizeof(T) == 8;
+    const bool nt = C->streamNT[f64 ? 0 : 1];
     // LDS-staged coarse columns where the gathers bound the pass: FP64 on a
     // full cuboid (measured 146 -> 105 us on 256x128x128).  FP32 is HBM-bound
     // either way, and on irregular domains whole column pairs cost more than
     // the 64-entry chunks of the DOF pass (cfg 2: 1.86 -> 2.51 s).
     const size_t colBytes = sizeof(T) * 4 * 3 * (size_t)l1.g.NY();  // staged coarse runs
-    constexpr bool f64 = sizeof(T) == 8;
     const bool cols = l1.span == 2 && C->dirCols && f64 && C->full && colBytes <= 48 * 1024;
     SGLD_TRY(kt_begin(C, f64 ? KT_DIR64 : KT_DIR32,
                       cols ? "k_prolong_dir_cols<double>"
-                      : l1.span == 2 ? (f64 ? "k_prolong_dir<double, 2>" : "k_prolong_dir<float, 2>")
-                                     : (f64 ? "k_prolong_dir<double, 4>" : "k_prolong_dir<float, 4>")));
+                      : stream_name(f64, nt, l1.span == 2 ? 0 : 1)));
     if (cols) {
         const int64_t items = ((l0.g.NX() + 1) / 2) * l0.g.NZ();
         const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(items, 65536));
         hipLaunchKernelGGL(k_prolong_dir_cols<T>, dim3(nb), dim3(256), colBytes, C->s, (const T *)lvl_x<T>(l1),
                            l1.g, l0.g, (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, (const int32_t *)nullptr,
                            items, (const Ctl *)ctl);
-    } else if (l1.span == 2)
-        hipLaunchKernelGGL((k_prolong_dir<T, 2>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
-                           (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
-    else
-        hipLaunchKernelGGL((k_prolong_dir<T, 4>), dim3(nbv), dim3(256), 0, C->s, (const T *)lvl_x<T>(l1), l1.g, d0,
-                           (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S, (const Ctl *)ctl);
+    } else {
+        const T *xc = lvl_x<T>(l1);
+        const auto *dv = fine_dinv<T>(C);
+        auto kern = l1.span == 2 ? (nt ? k_prolong_dir<T, 2, true> : k_prolong_dir<T, 2, false>)
+                                 : (nt ? k_prolong_dir<T, 4, true> : k_prolong_dir<T, 4, false>);
+        hipLaunchKernelGGL(kern, dim3(nbv), dim3(256), 0, C->s, xc, l1.g, d0, (const T *)r, dv, l0.nflag.p, x, p, S,
+                           (const Ctl *)ctl);
+    }
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(kt_end(C));
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
-    SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, f64 ? "k_update_r<double>" : "k_update_r<float>"));
-    hipLaunchKernelGGL(k_update_r<T>, dim3(nbv), dim3(256), 0, C->s, r, (const T *)q, fine_dinv<T>(C), S,
-                       slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl);
+    SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, stream_name(f64, nt, 2)));
+    auto upd = nt ? k_update_r<T, true> : k_update_r<T, false>;
+    hipLaunchKernelGGL(upd, dim3(nbv), dim3(256), 0, C->s, r,
+                       (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl);
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(kt_end(C));
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
@@ -2782,6 +2793,13 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     SGLD_TRY(C->partial.alloc((int64_t)kRedSlots * kRedBlocks));
     SGLD_TRY(C->ctl.alloc(1));
     SGLD_TRY(element_data(C.get()));
+    // Streaming passes with nontemporal loads once one DOF vector outgrows
+    // half the 256 MB Infinity Cache (a pass streams 3-4 of them, so nothing
+    // it reads survives to the next pass); SGLD_STREAM_NT=0/1 forces either.
+    for (int q = 0; q < 2; ++q) C->streamNT[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
+    if (const char *env = std::getenv("SGLD_STREAM_NT")) C->streamNT[0] = C->streamNT[1] = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_VEC_BLOCKS"))
+        C->vecBlocks = std::max(64, std::min(kRedBlocks, std::atoi(env)));
     double km[9];
     C->kbyuVariant = mode_coefficients(C->Ke, km) ? 1 : 0;
     if (const char *env = std::getenv("SGLD_KBYU")) {
@@ -2805,15 +2823,14 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
         // 125 M-DOF ellipsoids 914 / 867 and 560 / 483 us (the LDS atomic kernel:
         // 63.5, 47.0, 885, 513 us)
         for (int q = 0; q < 2; ++q) C->kbyuStageOut[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
-        if (const char *env = std::getenv("SGLD_KU_SKIP_VOID")) C->kbyuSkipVoid = std::atoi(env) != 0;
         if (const char *env = std::getenv("SGLD_KU_STAGE_OUT"))
             C->kbyuStageOut[0] = C->kbyuStageOut[1] = std::atoi(env) != 0;
         if (C->kbyuOwner) {
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per[0], C->kbyuStageOut[0] ? k_kbyu_owner<double, true, false> : k_kbyu_owner<double, false, false>,
+                &per[0], C->kbyuStageOut[0] ? k_kbyu_owner<double, true> : k_kbyu_owner<double, false>,
                 256, 0));
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per[1], C->kbyuStageOut[1] ? k_kbyu_owner<float, true, false> : k_kbyu_owner<float, false, false>,
+                &per[1], C->kbyuStageOut[1] ? k_kbyu_owner<float, true> : k_kbyu_owner<float, false>,
                 256, 0));
         } else if (C->kbyuOneBar) {
             SGLD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_kbyu_tiled<double, true>, 256, 0));

@@ -117,12 +117,14 @@ struct sgld_ctx {
     // owner K.u, [0] FP64 / [1] FP32: outputs through LDS for contiguous stores
     // (set when the output vector exceeds the Infinity Cache share, see sgld_ctx_create)
     bool kbyuStageOut[2] = {true, true};
-    bool kbyuSkipVoid = false;  // owner K.u on irregular domains: skip all-void wavefronts
     int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
     int64_t restrictTiledMin = (int64_t)1 << 18;    // "by size": FP64 tiled from this many coarse nodes
-    int64_t restrictTiledMin32 = (int64_t)1 << 40;  // and FP32
+    // and FP32 (125 M-DOF ellipsoids, 17 M coarse nodes: tiled 218 us, node-centric 222 us)
+    int64_t restrictTiledMin32 = (int64_t)1 << 22;
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
+    int vecBlocks = sgld::kVecBlocks;  // grid of the finest-level direction and residual updates
+    bool streamNT[2] = {false, false};  // [0] FP64 / [1] FP32 streaming passes with nontemporal loads
     bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)
     bool bRestrictTiled = true;  // batched span-2 restriction: LDS-tiled layer march (else node-centric)
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
