@@ -1595,54 +1595,195 @@ __global__ void __launch_bounds__(576) k_galerkin_from_Ks(const double *__restri
     }
 }
 
-// Coarsest-level SPD inverse by Gauss-Jordan elimination without pivoting
-// (the matrix is SPD, so every pivot is positive), one elimination step per
-// launch, ping-pong between two n x n buffers so no entry is overwritten while
-// other threads still read it.  Step k: B = A - a_col a_row / a_kk off the
-// pivot row/col, B_kj = a_kj / a_kk, B_ik = -a_ik / a_kk, B_kk = 1 / a_kk.
-// A non-positive pivot sets *bad (to 1 + k; first one wins).
-// Dense Cholesky of the coarsest free-DOF matrix (column-major n x n, lower
-// triangle used), right-looking, one launch per pivot k: the trailing lower
-// triangle takes the Schur update A(i,j) -= A(i,k) A(j,k) / A(k,k) for
-// i >= j > k, in place (column k is only read in step k).  A 16 x 16 tile of
-// (i, j) per block, i fastest; tiles above the diagonal exit.  A pivot that
-// is not positive is reported (1-based) in *bad.
-__global__ void __launch_bounds__(256) k_chol_step(double *__restrict__ A, int64_t n, int64_t k,
-                                                   unsigned long long *bad) {
-    const double piv = A[k + n * k];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && !(piv > 0.0))
-        atomicCAS(bad, 0ull, (unsigned long long)(k + 1));
-    const int64_t i0 = k + 1 + 16 * (int64_t)blockIdx.y, j0 = k + 1 + 16 * (int64_t)blockIdx.x;
-    if (i0 + 15 < j0) return;
-    const int64_t i = i0 + (threadIdx.x & 15), j = j0 + (threadIdx.x >> 4);
-    if (i >= n || j >= n || i < j) return;
-    A[i + n * j] -= A[i + n * k] * (A[j + n * k] / piv);
-}
-// L (row-major, lower) from the Schur-updated columns: L(i,k) = A(i,k) / sqrt(A(k,k)).
-__global__ void k_chol_finish(const double *__restrict__ A, int64_t n, double *__restrict__ L) {
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx / n, k = idx % n;
-        L[idx] = i < k ? 0.0 : (i == k ? sqrt(A[k + n * k]) : A[i + n * k] / sqrt(A[k + n * k]));
+// Dense FP64 Cholesky K = L L^T of the coarsest free-DOF matrix and its
+// inverse factor W = L^-1, blocked in 64 x 64 tiles (column-major n x n, lower
+// triangle; L overwrites A).  Per 64-column panel k0: factor the diagonal tile
+// in LDS (k_potrf_tile), solve the rows below against it (k_trsm_panel), then
+// subtract the panel's outer product from the trailing lower triangle
+// (k_syrk_tiles).  W is formed block row by block row: W_II = L_II^-1 for every
+// diagonal tile (k_trinv_tiles), then W_IJ = -W_II sum_{J<=K<I} L_IK W_KJ for
+// all J < I of block row I in one launch (k_trinv_blockrow).  A pivot that is
+// not positive is reported (1-based) in *bad.
+constexpr int kCB = 64;  // tile edge
+
+// 4 x 4 outputs per thread of a 64 x 64 tile product over 64 inner indices:
+// acc[a][b] += X[k][tr + 16 a] * Y[k][tc + 16 b] (k-major LDS tiles).
+__device__ __forceinline__ void tile_fma(const double (*X)[kCB + 1], const double (*Y)[kCB + 1], int tr, int tc,
+                                         double acc[4][4]) {
+#pragma unroll 4
+    for (int k = 0; k < kCB; ++k) {
+        double x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            x[q] = X[k][tr + 16 * q];
+            y[q] = Y[k][tc + 16 * q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) acc[q][w] = fma(x[q], y[w], acc[q][w]);
     }
 }
-// Row k of W = L^-1 (row-major) by forward substitution of e_k, from the
-// rows 0..k-1 already formed: W(k,c) = (delta_kc - sum_{c<=j<k} L(k,j) W(j,c)) / L(k,k).
-__global__ void __launch_bounds__(256) k_trinv_row(const double *__restrict__ L, double *__restrict__ W,
-                                                   int64_t n, int64_t k) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > k) return;
-    const double *lk = L + k * n;
-    double acc = c == k ? 1.0 : 0.0;
-    for (int64_t j = c; j < k; ++j) acc -= lk[j] * W[j * n + c];
-    W[k * n + c] = acc / lk[k];
+
+// Diagonal tile k0 (kb = min(64, n - k0) columns), one block of 256 threads.
+__global__ void __launch_bounds__(256) k_potrf_tile(double *__restrict__ A, int64_t n, int64_t k0,
+                                                    unsigned long long *bad) {
+    __shared__ double t[kCB][kCB + 1];  // t[col][row]
+    const int kb = (int)min<int64_t>(kCB, n - k0);
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int i = e % kb, j = e / kb;
+        t[j][i] = i >= j ? A[(k0 + i) + n * (k0 + j)] : 0.0;
+    }
+    for (int k = 0; k < kb; ++k) {
+        __syncthreads();
+        const double piv = t[k][k];
+        if (threadIdx.x == 0 && !(piv > 0.0)) atomicCAS(bad, 0ull, (unsigned long long)(k0 + k + 1));
+        const double d = sqrt(piv);
+        __syncthreads();
+        for (int i = k + threadIdx.x; i < kb; i += blockDim.x) t[k][i] = i == k ? d : t[k][i] / d;
+        __syncthreads();
+        for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+            const int i = e % kb, j = e / kb;
+            if (j > k && i >= j) t[j][i] -= t[k][i] * t[k][j];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int i = e % kb, j = e / kb;
+        if (i >= j) A[(k0 + i) + n * (k0 + j)] = t[j][i];
+    }
 }
-// Packed rows and columns of the lower-triangular W (tri_row_off / tri_col_off).
+
+// Rows i >= k0 + 64 of panel k0: x L_kk^T = a (forward substitution), one
+// row per thread; the panel tile L_kk is read from LDS by every lane at once.
+__global__ void __launch_bounds__(64) k_trsm_panel(double *__restrict__ A, int64_t n, int64_t k0) {
+    __shared__ double l[kCB][kCB + 1];  // l[row][col] of L_kk
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int i = e % kCB, j = e / kCB;
+        l[i][j] = i >= j ? A[(k0 + i) + n * (k0 + j)] : 0.0;
+    }
+    __syncthreads();
+    const int64_t i = k0 + kCB + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x[kCB];
+#pragma unroll
+    for (int j = 0; j < kCB; ++j) x[j] = A[i + n * (k0 + j)];
+#pragma unroll
+    for (int j = 0; j < kCB; ++j) {
+        double v = x[j];
+#pragma unroll
+        for (int m = 0; m < j; ++m) v = fma(-x[m], l[j][m], v);
+        x[j] = v / l[j][j];
+    }
+#pragma unroll
+    for (int j = 0; j < kCB; ++j) A[i + n * (k0 + j)] = x[j];
+}
+
+// Linear id -> (ti, tj), ti >= tj, of the lower-triangular tile pairs.
+__device__ __forceinline__ void lower_pair(int64_t id, int64_t &ti, int64_t &tj) {
+    int64_t r = (int64_t)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > id) --r;
+    while ((r + 1) * (r + 2) / 2 <= id) ++r;
+    ti = r;
+    tj = id - r * (r + 1) / 2;
+}
+
+// Trailing update of panel k0: A(R_i, R_j) -= P(R_i) P(R_j)^T for the lower
+// tile pairs (i >= j) of rows/columns >= k0 + 64, P = A(:, k0 .. k0 + 63).
+__global__ void __launch_bounds__(256) k_syrk_tiles(double *__restrict__ A, int64_t n, int64_t k0) {
+    __shared__ double Pi[kCB][kCB + 1], Pj[kCB][kCB + 1];  // [k][row]
+    int64_t ti, tj;
+    lower_pair(blockIdx.x, ti, tj);
+    const int64_t base = k0 + kCB, r0 = base + kCB * ti, c0 = base + kCB * tj;
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int r = e % kCB, k = e / kCB;
+        Pi[k][r] = r0 + r < n ? A[(r0 + r) + n * (k0 + k)] : 0.0;
+        Pj[k][r] = c0 + r < n ? A[(c0 + r) + n * (k0 + k)] : 0.0;
+    }
+    __syncthreads();
+    const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    double acc[4][4] = {};
+    tile_fma(Pi, Pj, tr, tc, acc);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int64_t r = r0 + tr + 16 * q, c = c0 + tc + 16 * w;
+            if (r < n && c < n && r >= c) A[r + n * c] -= acc[q][w];
+        }
+}
+
+// W_II = L_II^-1 for diagonal tile I = blockIdx.x: column c of the tile per
+// thread, by forward substitution of e_c in LDS (x[r][c], each thread its own
+// column; the upper part of the tile comes out 0).
+__global__ void __launch_bounds__(64) k_trinv_tiles(const double *__restrict__ L, double *__restrict__ W,
+                                                    int64_t n) {
+    __shared__ double l[kCB][kCB + 1], x[kCB][kCB + 1];  // l[row][col], x[row][col]
+    const int64_t i0 = kCB * (int64_t)blockIdx.x;
+    const int kb = (int)min<int64_t>(kCB, n - i0);
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int i = e % kCB, j = e / kCB;
+        l[i][j] = (i < kb && j < kb && i >= j) ? L[(i0 + i) + n * (i0 + j)] : (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    for (int r = 0; r < kCB; ++r) {
+        double v = r == c ? 1.0 : 0.0;
+        for (int m = c; m < r; ++m) v = fma(-l[r][m], x[m][c], v);  // x[m][c] = 0 for m < c
+        x[r][c] = r < c ? 0.0 : v / l[r][r];
+    }
+    if (c >= kb) return;
+    for (int r = 0; r < kb; ++r) W[(i0 + r) + n * (i0 + c)] = x[r][c];
+}
+
+// Block row I of W: W_IJ = -W_II S, S = sum_{K=J}^{I-1} L_IK W_KJ, for J =
+// blockIdx.x < I (rows of the last block row beyond n read as 0).
+__global__ void __launch_bounds__(256) k_trinv_blockrow(const double *__restrict__ L, double *__restrict__ W,
+                                                        int64_t n, int64_t I) {
+    __shared__ double X[kCB][kCB + 1], Y[kCB][kCB + 1];  // [k][row], [k][col]
+    const int64_t J = blockIdx.x, i0 = kCB * I, j0 = kCB * J;
+    const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    double acc[4][4] = {};
+    for (int64_t K = J; K < I; ++K) {
+        const int64_t k0 = kCB * K;
+        __syncthreads();
+        for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+            const int a = e % kCB, b = e / kCB;
+            X[b][a] = i0 + a < n ? L[(i0 + a) + n * (k0 + b)] : 0.0;  // L_IK(a, b)
+            Y[a][b] = W[(k0 + a) + n * (j0 + b)];                      // W_KJ(a, b)
+        }
+        __syncthreads();
+        tile_fma(X, Y, tr, tc, acc);
+    }
+    // S to LDS (Y[k][col] = S(k, col)), W_II to X ([k][row] = W_II(row, k))
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) Y[tr + 16 * q][tc + 16 * w] = acc[q][w];
+    for (int e = threadIdx.x; e < kCB * kCB; e += blockDim.x) {
+        const int a = e % kCB, b = e / kCB;
+        X[b][a] = (i0 + a < n && i0 + b < n) ? W[(i0 + a) + n * (i0 + b)] : 0.0;
+    }
+    __syncthreads();
+    double out[4][4] = {};
+    tile_fma(X, Y, tr, tc, out);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int64_t r = i0 + tr + 16 * q;
+            if (r < n) W[r + n * (j0 + tc + 16 * w)] = -out[q][w];
+        }
+}
+
+// Packed rows and columns of the lower-triangular W (column-major n x n;
+// tri_row_off / tri_col_off).
 __global__ void k_tri_pack(const double *__restrict__ W, int64_t n, double *__restrict__ Wrow,
                            double *__restrict__ Wcol) {
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx / n, j = idx % n;
+        const int64_t i = idx % n, j = idx / n;
         if (j > i) continue;
         Wrow[tri_row_off(i) + j] = W[idx];
         Wcol[tri_col_off(j, n) + (i - j)] = W[idx];
@@ -3041,15 +3182,20 @@ int assemble_impl(sgld_ctx *C, double omega) {
                 if (dof[r] >= 0) A[dof[r] + nf * dof[c]] += Ks[576 * e + r + 24 * c];
         }
     }
-    DevBuf<double> dA, dL, dW;
+    DevBuf<double> dA, dW;
     DevBuf<unsigned long long> dBad;
     SGLD_TRY(dA.upload(A.data(), A.size()));
     SGLD_TRY(dBad.alloc(1));
     SGLD_TRY(dBad.zero(C->s));
-    for (int64_t k = 0; k < nf; ++k) {
-        const unsigned t = (unsigned)((nf - k - 1 + 15) / 16);
-        hipLaunchKernelGGL(k_chol_step, dim3(std::max(1u, t), std::max(1u, t)), dim3(256), 0, C->s, dA.p, nf, k,
-                           dBad.p);
+    const int64_t T = (nf + kCB - 1) / kCB;  // tiles per edge
+    for (int64_t K = 0; K < T; ++K) {
+        const int64_t k0 = kCB * K;
+        hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(256), 0, C->s, dA.p, nf, k0, dBad.p);
+        const int64_t rows = nf - k0 - kCB;  // > 0 only below a full tile
+        if (rows <= 0) continue;
+        hipLaunchKernelGGL(k_trsm_panel, dim3(blocks_for(rows, 64)), dim3(64), 0, C->s, dA.p, nf, k0);
+        const int64_t tt = (rows + kCB - 1) / kCB;
+        hipLaunchKernelGGL(k_syrk_tiles, dim3((unsigned)(tt * (tt + 1) / 2)), dim3(256), 0, C->s, dA.p, nf, k0);
     }
     SGLD_HIP(hipGetLastError());
     unsigned long long bad = 0;
@@ -3063,13 +3209,14 @@ int assemble_impl(sgld_ctx *C, double omega) {
                   (long long)(g / (NY * NX)), (long long)(dof % 3), A[(bad - 1) * (nf + 1)]);
         return SGLD_ERR_NOT_SPD;
     }
-    SGLD_TRY(dL.alloc(A.size()));
-    hipLaunchKernelGGL(k_chol_finish, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s, dA.p, nf, dL.p);
-    dA.release();
-    SGLD_TRY(dW.alloc(A.size()));
-    for (int64_t k = 0; k < nf; ++k)
-        hipLaunchKernelGGL(k_trinv_row, dim3(blocks_for(k + 1, 256)), dim3(256), 0, C->s, dL.p, dW.p, nf, k);
+    std::vector<double>().swap(A);
+    SGLD_TRY(dW.alloc((size_t)nf * nf));
+    hipLaunchKernelGGL(k_trinv_tiles, dim3((unsigned)T), dim3(64), 0, C->s, (const double *)dA.p, dW.p, nf);
+    for (int64_t I = 1; I < T; ++I)
+        hipLaunchKernelGGL(k_trinv_blockrow, dim3((unsigned)I), dim3(256), 0, C->s, (const double *)dA.p, dW.p,
+                           nf, I);
     SGLD_HIP(hipGetLastError());
+    dA.release();
     SGLD_TRY(C->cWrow.alloc(tri_row_off(nf)));
     SGLD_TRY(C->cWcol.alloc(tri_row_off(nf)));
     hipLaunchKernelGGL(k_tri_pack, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s, dW.p, nf, C->cWrow.p,

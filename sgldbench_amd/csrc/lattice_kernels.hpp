@@ -101,9 +101,11 @@ struct Span {
 // (entries j..n-1, a row of W^T) at tri_col_off(j, n).
 __host__ __device__ inline int64_t tri_row_off(int64_t i) { return i * (i + 1) / 2; }
 __host__ __device__ inline int64_t tri_col_off(int64_t j, int64_t n) { return j * n - j * (j - 1) / 2; }
-// Largest coarsest free-DOF count of the dense factor (8 n^2 bytes packed;
-// factorisation about n^3 / 3 flops at set-up).
-constexpr int64_t kCoarseMaxFree = 8192;
+// Largest coarsest free-DOF count of the dense factor: W is stored packed
+// twice (rows and columns, 8 n^2 bytes: 8.6 GB at the limit) and both copies
+// are read once per V-cycle; set-up holds the dense n x n matrix on the host
+// and two on the device and takes about 2 n^3 / 3 flops.
+constexpr int64_t kCoarseMaxFree = 32768;
 
 constexpr int kRedBlocks = 8192;  // partial-sum slots of one reduction (max grid of a reducing kernel)
 constexpr int kVecBlocks = 2048;  // grid of the grid-stride vector kernels

@@ -317,17 +317,21 @@ def test_coarsest_size_guard():
     g.set_modulus(np.ones(int(vox.sum())))
     with pytest.raises(solver.SGLDError) as e:
         g.assemble(0.6)
-    assert e.value.code == 1 and "at most 8192" in str(e.value), str(e.value)
+    assert e.value.code == 1 and "at most 32768" in str(e.value), str(e.value)
     g.close()
 
 
-@pytest.mark.parametrize("case", ["cuboid", "holes"])
+@pytest.mark.parametrize("case", ["cuboid", "holes", "cuboid_4k"])
 def test_coarsest_solve_is_dense_cholesky_solve(case):
     """Two-level V-cycle minus its smoothing term isolates P K_L^-1 P^T r:
     with the coarsest level's free-DOF matrix assembled densely by the oracle
     (Galerkin Ks) and solved by numpy, the GPU's triangular products agree to
-    FP64 rounding; the operator is symmetric."""
-    vox = problems.cuboid(16, 8, 8) if case == "cuboid" else holes(16, 8, 8, 3, nholes=2, r=2)
+    FP64 rounding; the operator is symmetric.  The factor is blocked in 64 x 64
+    tiles: 9-10 tiles per edge for the small cases, 64 (4,050 free DOFs, a
+    ragged last tile) for cuboid_4k."""
+    vox = {"cuboid": lambda: problems.cuboid(16, 8, 8), "holes": lambda: holes(16, 8, 8, 3, nholes=2, r=2),
+           "cuboid_4k": lambda: problems.cuboid(32, 16, 16)}[case]()
+    print("case: coarsest dense solve", case, flush=True)
     g, m, F, fixed, E = build_pair(vox, 2)
     rng = np.random.default_rng(2)
     r = rng.standard_normal(F.size) * (1 - fixed)

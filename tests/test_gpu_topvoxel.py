@@ -77,3 +77,42 @@ def test_part_r256_mixed_matches_oracle(part):
     assert true_relres(m, p["F"], U) <= 1.01e-8
     c = p["F"] @ U
     assert abs(c - float(gold["compliance"])) <= 1e-8 * abs(float(gold["compliance"]))
+
+
+@pytest.mark.parametrize("non_dyadic", [1, 0])
+def test_part_r256_reference_default_hierarchy(non_dyadic):
+    """The reference's default hierarchy (FEA_VoxelBasedDiscretization.m:31-40
+    with coarsestResolutionControl_ = 50000, Data_GlobalVariables.m:13,15):
+    4 levels, or with nonDyadic_ = 1 three levels whose first coarsening spans
+    4 (Solving_BuildingMeshHierarchy.m:12-23).  Both end on the same 20 x 32 x
+    14-element coarsest level, about 16.5 k free DOFs, factored densely.  The
+    solution does not depend on the hierarchy: it matches the oracle's FP64
+    solve of the same model (committed with the model), and the true residual
+    from the oracle's K.u meets the tolerance."""
+    model, gold = tv.load_packed(GOLDEN), np.load(GOLDEN)
+    p = tv.setup(model)
+    assert p["num_levels"] == 4 and p["voxels"].shape == (160, 256, 112)
+    print("case: part_R256 default hierarchy nonDyadic", non_dyadic, flush=True)
+    g = solver.FEASolver(p["voxels"], p["num_levels"], nonDyadic=non_dyadic)
+    try:
+        assert g.numLevels == (3 if non_dyadic else 4)
+        assert [lv["spanWidth"] for lv in g.levels[1:]] == ([4, 2] if non_dyadic else [2, 2, 2])
+        g.set_fixed(p["fixed"])
+        g.set_modulus(p["E"])
+        g.assemble(p["omega"])
+        print("  set-up %.2f s (coarsest level: dense factor of all its free DOFs)" % g.stats()["setup_s"],
+              flush=True)
+        m = orc.Mesh(p["voxels"], 2)  # the finest level's K.u only: nothing assembled
+        m.set_fixed(p["fixed"])
+        m.set_modulus(p["E"])
+        c0 = float(gold["compliance"])
+        for prec, tol in ((solver.PREC_FP64, 1e-8), (solver.PREC_MIXED, 1.01e-8)):
+            U, its, rr = g.pcg(p["F"], tol=1e-8, maxIT=2000, precision=prec)
+            print("  precision", prec, "its", its, flush=True)
+            assert g.converged
+            assert true_relres(m, p["F"], U) <= tol
+            assert abs(p["F"] @ U - c0) <= 1e-8 * abs(c0)
+            ul = U[gold["u_loaded_dofs"]]
+            assert np.max(np.abs(ul - gold["u_loaded"])) <= 1e-6 * np.max(np.abs(gold["u_loaded"]))
+    finally:
+        g.close()
