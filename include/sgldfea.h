@@ -215,8 +215,8 @@ int sgld_ctx_sensitivity_device(sgld_ctx *ctx, const double *dU, const double *d
                                 double E0, double Emin, double scale, double *ddc,
                                 double *compliance);
 /* Autotuner (no counterpart in the reference): times the K.u variants of this
- * mesh (patch-tiled mode-space with z-chunks of 8, 16 or 32 node layers, and
- * the node-centric dense kernel; `reps` launches each, FP64 and FP32) and the
+ * mesh (patch-tiled mode-space with z-chunks of 8, 12, 16 or 32 node layers,
+ * where a new context starts at 12, and the node-centric dense kernel; `reps` launches each, FP64 and FP32) and the
  * two span-2 restrictions (on FP32 V-cycles, once assembled), and keeps the
  * fastest: the z-chunk per precision, then tiled against dense for the
  * mixed solver (cost = FP32 K.u + FP64 K.u / 8).  report[8]: [0] K.u variant
