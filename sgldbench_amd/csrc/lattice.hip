@@ -957,7 +957,7 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
                                                      const T *__restrict__ rf,
                                                      const typename FineW<T>::D *__restrict__ dinv,
                                                      const uint8_t *__restrict__ fflag, T *__restrict__ x,
-                                                     T *__restrict__ p, Span dofs, const Ctl *ctl) {
+                                                     T *__restrict__ p, Span dofs, int xsplit, const Ctl *ctl) {
     if (stopped(ctl)) return;
     const T lam = (T)ctl->lambda, beta = (T)ctl->beta;
     const FastDiv d3 = FastDiv::make3();
@@ -965,7 +965,7 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
     // one contiguous range per XCD keeps the coarse gathers in its L2
     // (125 M-DOF ellipsoids, FP32: 4.12 GB fetched + written per launch with
     // blocks interleaved over the XCDs, against 2.81 GB algorithmic)
-    FOR_SPAN_XCD(i, dofs) {
+    auto body = [&](int64_t i) {
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         const T pv = interp_dof<T, S>(xc, gc, df, n, a);
@@ -975,6 +975,11 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
         const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
         x[i] = ld_stream<NT>(&x[i]) + lam * po;
         p[i] = z + beta * po;
+    };
+    if (xsplit) {
+        FOR_SPAN_XCD(i, dofs) body(i);
+    } else {
+        FOR_SPAN(i, dofs) body(i);
     }
 }
 
@@ -2382,8 +2387,8 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
         const auto *dv = fine_dinv<T>(C);
         auto kern = l1.span == 2 ? (nt ? k_prolong_dir<T, 2, true> : k_prolong_dir<T, 2, false>)
                                  : (nt ? k_prolong_dir<T, 4, true> : k_prolong_dir<T, 4, false>);
-        hipLaunchKernelGGL(kern, dim3(nbx), dim3(256), 0, C->s, xc, l1.g, d0, (const T *)r, dv, l0.nflag.p, x, p, S,
-                           (const Ctl *)ctl);
+        hipLaunchKernelGGL(kern, dim3(C->dirXcd ? nbx : nbv), dim3(256), 0, C->s, xc, l1.g, d0, (const T *)r, dv,
+                           l0.nflag.p, x, p, S, C->dirXcd ? 1 : 0, (const Ctl *)ctl);
     }
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(kt_end(C));
@@ -2957,6 +2962,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     // it reads survives to the next pass); SGLD_STREAM_NT=0/1 forces either.
     for (int q = 0; q < 2; ++q) C->streamNT[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
     if (const char *env = std::getenv("SGLD_STREAM_NT")) C->streamNT[0] = C->streamNT[1] = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_DIR_XCD")) C->dirXcd = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_VEC_BLOCKS"))
         C->vecBlocks = std::max(64, std::min(kRedBlocks, std::atoi(env)));
     double km[9];

@@ -124,6 +124,7 @@ struct sgld_ctx {
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
     int vecBlocks = sgld::kVecBlocks;  // grid of the finest-level direction and residual updates
+    bool dirXcd = true;  // direction update with one contiguous DOF range per XCD
     bool streamNT[2] = {false, false};  // [0] FP64 / [1] FP32 streaming passes with nontemporal loads
     bool dirCols = true;  // FP64 direction update with LDS-staged coarse columns (span 2, full cuboid)
     bool bRestrictTiled = true;  // batched span-2 restriction: LDS-tiled layer march (else node-centric)
