@@ -29,12 +29,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "part_r256"],
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg3", "cfg4", "part_r256",
+                                                           "part_r256_default"],
                     help="cfg2: ~125 M-DOF ellipsoid union in 512^3 (default); cfg0: 64x32x32 solid cuboid; "
                          "cfg1: 256x128x128 GRF cuboid; "
                          "cfg3: 16 load cases on a 256^3 GRF cuboid, batched mixed vs 16 sequential FP64 "
                          "solves; cfg4: 32 uneven cuboids in one replayed graph plus sensitivities; "
-                         "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels)")
+                         "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels); "
+                         "part_r256_default: the same on the reference's default hierarchy (4 levels, "
+                         "nonDyadic: spans 4, 2; 20x32x14-element coarsest level, 16.5 k free DOFs)")
     ap.add_argument("--n3", type=int, default=256, help="cube edge of the batched workload")
     ap.add_argument("--hw-queues", type=int, default=None,
                     help="hardware queues for the multi-instance workload (sets GPU_MAX_HW_QUEUES, <= 32); "
@@ -332,7 +335,7 @@ def run_batched(a):
     k = p["F"].shape[0]
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
-    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s = solver.FEASolver(p["voxels"], p["num_levels"], nonDyadic=p.get("non_dyadic", 0))
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
     t0 = time.time()
@@ -484,13 +487,17 @@ def load_workload(a):
                 "%d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, float(p["voxels"].mean()), p["F"].size,
                                                     p["num_levels"]))
         return p, desc, "synthetic (seeded ellipsoid union)"
-    if a.workload == "part_r256":
+    if a.workload in ("part_r256", "part_r256_default"):
         from sgldbench_amd import topvoxel
+        default = a.workload == "part_r256_default"
         p = topvoxel.setup(topvoxel.load_packed(os.path.join(ROOT, "tests", "golden", "part_R256.npz")),
-                           num_levels=6)
+                           num_levels=None if default else 6)
+        p["non_dyadic"] = 1 if default else 0
         ny, nx, nz = p["voxels"].shape
         desc = ("voxel model part_R256 (256x160x112, padded %dx%dx%d), %d DOF, E=1, its 13042 fixations "
-                "and 7788 loads, %d levels, omega 0.6, tol 1e-8" % (nx, ny, nz, p["F"].size, p["num_levels"]))
+                "and 7788 loads, %d levels%s, omega 0.6, tol 1e-8"
+                % (nx, ny, nz, p["F"].size, p["num_levels"],
+                   " (the reference's rule; nonDyadic: 3 levels, spans 4, 2)" if default else ""))
         return p, desc, "the reference's voxel model data/part_R256.TopVoxel (packed copy)"
     if a.workload == "cfg0":  # the reference's own CPU-sized case: E = 1, 4 levels
         p = problems.cfg0()
@@ -512,7 +519,7 @@ def reference_case(a):
     import torch
     from sgldbench_amd import problems, solver
     p = problems.cfg0()
-    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s = solver.FEASolver(p["voxels"], p["num_levels"], nonDyadic=p.get("non_dyadic", 0))
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
     s.assemble(p["omega"])
@@ -550,7 +557,7 @@ def run_single(a):
     progress("problem %s: %d DOF, %d elements (generated in %.1f s)" % (a.workload, n, ne, time.time() - t_gen))
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
-    s = solver.FEASolver(p["voxels"], p["num_levels"])
+    s = solver.FEASolver(p["voxels"], p["num_levels"], nonDyadic=p.get("non_dyadic", 0))
     s.set_fixed(p["fixed"])
     s.set_modulus(p["E"])
     if a.kbyu_only:
@@ -682,7 +689,11 @@ def run_single(a):
     s.close()
     del dF, dU
     torch.cuda.empty_cache()
-    if not a.no_cpu:
+    if a.workload == "part_r256_default" and not a.no_cpu:
+        # the oracle factors its coarsest level with an unblocked dense
+        # Cholesky: hours at 16.5 k free DOFs
+        out["cpu_baseline"] = {"value": None, "note": "not sampled: the oracle's coarsest factor at 16.5 k DOFs"}
+    elif not a.no_cpu:
         big = n > 50_000_000
         out["cpu_baseline"] = cpu_baseline(p, res["fp64"]["its"], sample_its=1 if big else a.cpu_its,
                                            one_thread=not big, lean=big, full_solves=a.cpu_full)

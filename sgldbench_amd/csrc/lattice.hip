@@ -75,7 +75,7 @@ template <> struct FineW<float> {
 };
 
 // Grid-stride passes over the entries of a Span (all entries, or the listed
-// 64-entry chunks of an irregular domain).
+// chunks of an irregular domain).
 #define FOR_SPAN(i, S)                                                                           \
     for (int64_t t_ = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t_ < (S).count;            \
          t_ += (int64_t)gridDim.x * blockDim.x)                                                    \
@@ -935,7 +935,8 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
 // runs on XCD b mod 8) owning one contiguous eighth of the span, in 64-entry
 // units, so that values gathered from a shared coarser vector are fetched
 // into one L2 rather than into all eight.  Positions of one wavefront still
-// share t >> 6.  Needs gridDim.x to be a multiple of 8.
+// share t >> 6 (what Span::at needs for 64-entry chunks).  Needs gridDim.x to
+// be a multiple of 8.
 #define FOR_SPAN_XCD(i, S)                                                                              \
     for (int64_t u_ = ((S).count + 63) >> 6, e_ = min<int64_t>((S).count, (u_ * (blockIdx.x % 8 + 1) / 8) << 6), \
                  t_ = ((u_ * (blockIdx.x % 8) / 8) << 6) + (int64_t)(blockIdx.x / 8) * blockDim.x + threadIdx.x; \
@@ -1360,10 +1361,13 @@ __global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__r
 // K_L(free, free) = L L^T, applied as two triangular products with the
 // inverse factor W = L^-1 (the device-parallel form of the two triangular
 // solves; the applied operator W^T W stays symmetric positive definite in
-// floating point).  FP64 in both precisions; one wavefront per row.
+// floating point).  Accumulated in FP64; W in FP64, or for the FP32 inner
+// solve an FP32 copy once W outgrows the Infinity Cache (the preconditioner
+// W32^T W32 is as symmetric positive definite as W^T W, and the product reads
+// half the bytes).  One wavefront per row.
 // y = W r(free): row i of W holds entries 0..i.
-template <class T>
-__global__ void __launch_bounds__(256) k_coarse_lower(const double *__restrict__ Wrow,
+template <class T, class TW>
+__global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wrow,
                                                       const int32_t *__restrict__ freeIdx, int64_t n,
                                                       const T *__restrict__ r, double *__restrict__ y,
                                                       const Ctl *ctl) {
@@ -1371,16 +1375,16 @@ __global__ void __launch_bounds__(256) k_coarse_lower(const double *__restrict__
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    const double *w = Wrow + tri_row_off(row);
+    const TW *w = Wrow + tri_row_off(row);
     double acc = 0.0;
-    for (int64_t j = lane; j <= row; j += 64) acc += w[j] * (double)r[freeIdx[j]];
+    for (int64_t j = lane; j <= row; j += 64) acc += (double)w[j] * (double)r[freeIdx[j]];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) y[row] = acc;
 }
 // x(free) = W^T y: row j of W^T (column j of W) holds entries j..n-1.  Fixed
 // and void DOFs of x are never written (zero since the assembly).
-template <class T>
-__global__ void __launch_bounds__(256) k_coarse_upper(const double *__restrict__ Wcol,
+template <class T, class TW>
+__global__ void __launch_bounds__(256) k_coarse_upper(const TW *__restrict__ Wcol,
                                                       const int32_t *__restrict__ freeIdx, int64_t n,
                                                       const double *__restrict__ y, T *__restrict__ x,
                                                       const Ctl *ctl) {
@@ -1388,9 +1392,9 @@ __global__ void __launch_bounds__(256) k_coarse_upper(const double *__restrict__
     const int64_t j = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (j >= n) return;
-    const double *w = Wcol + tri_col_off(j, n);
+    const TW *w = Wcol + tri_col_off(j, n);
     double acc = 0.0;
-    for (int64_t t = lane; t < n - j; t += 64) acc += w[t] * y[j + t];
+    for (int64_t t = lane; t < n - j; t += 64) acc += (double)w[t] * y[j + t];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) x[freeIdx[j]] = (T)acc;
 }
@@ -2209,11 +2213,12 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
                     const Ctl *ctl, int *nbOut) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
     const int finest = (k == 1);
-    // irregular domains: only the level's 64-entry DOF chunks holding a solid
+    // irregular domains: only the level's DOF chunks holding a solid
     // node (the rest stays zero from set-up)
     const Span S = finest ? dof_span(C)
                           : (C->full || !F.dofChunks.p ? Span::all(3 * F.g.nodes())
-                                                       : Span{F.dofChunks.p, 3 * F.g.nodes(), 64 * F.nDofChunks});
+                                                       : Span::listed(F.dofChunks.p, 3 * F.g.nodes(), F.nDofChunks,
+                                                                      C->chunkLog));
     const int nb = grid_blocks(S.count);
     const NodeDiv df = NodeDiv::make(F.g);
     if (Cc.span == 2)
@@ -2308,10 +2313,17 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
     }
     Level &Lc = C->lv[L - 1];
     const int64_t n = C->nCoarseFree;
-    hipLaunchKernelGGL(k_coarse_lower<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->cWrow.p, C->cFree.p, n,
-                       (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
-    hipLaunchKernelGGL(k_coarse_upper<T>, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, C->cWcol.p, C->cFree.p, n,
-                       (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
+    if (sizeof(T) == 4 && C->cWrow32.p) {
+        hipLaunchKernelGGL((k_coarse_lower<T, float>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
+                           (const float *)C->cWrow32.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
+        hipLaunchKernelGGL((k_coarse_upper<T, float>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
+                           (const float *)C->cWcol32.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
+    } else {
+        hipLaunchKernelGGL((k_coarse_lower<T, double>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
+                           (const double *)C->cWrow.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
+        hipLaunchKernelGGL((k_coarse_upper<T, double>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
+                           (const double *)C->cWcol.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
+    }
     SGLD_HIP(hipGetLastError());
     if (t0 > 0) {
         const int nb = grid_blocks(3 * A.g0.nodes());
@@ -2799,6 +2811,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_BRESTRICT_TILED")) C->bRestrictTiled = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
+    if (const char *env = std::getenv("SGLD_CHUNK_LOG")) C->chunkLog = std::max(4, std::min(6, std::atoi(env)));
     C->nonDyadic = nonDyadic;
     C->nu = nu;
     C->h = h;
@@ -2856,23 +2869,26 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             if (f0.esolid[v]) C->hEleMap.push_back((int32_t)v);
         SGLD_TRY(C->nodMap.upload(C->hNodMap.data(), C->hNodMap.size()));
         SGLD_TRY(C->eleMap.upload(C->hEleMap.data(), C->hEleMap.size()));
-        // 64-entry chunks of the finest node and DOF vectors that hold a solid node
+        // chunks (2^chunkLog entries) of the finest node and DOF vectors that
+        // hold a solid node
         const Grid &g0 = f0.g;
-        const int64_t nNodes = g0.nodes();
+        const int64_t nNodes = g0.nodes(), cs = (int64_t)1 << C->chunkLog;
         std::vector<int32_t> nodeCh, dofCh;
-        for (int64_t c = 0; 64 * c < nNodes; ++c) {
+        for (int64_t c = 0; cs * c < nNodes; ++c) {
             bool any = false;
-            for (int64_t g = 64 * c; g < std::min(64 * c + 64, nNodes) && !any; ++g) any = f0.hflag[g] & kSolid;
+            for (int64_t g = cs * c; g < std::min(cs * c + cs, nNodes) && !any; ++g) any = f0.hflag[g] & kSolid;
             if (any) nodeCh.push_back((int32_t)c);
         }
-        for (int64_t c = 0; 64 * c < 3 * nNodes; ++c) {
+        for (int64_t c = 0; cs * c < 3 * nNodes; ++c) {
             bool any = false;
-            for (int64_t g = 64 * c / 3; g <= std::min(64 * c + 63, 3 * nNodes - 1) / 3 && !any; ++g)
+            for (int64_t g = cs * c / 3; g <= std::min(cs * c + cs - 1, 3 * nNodes - 1) / 3 && !any; ++g)
                 any = f0.hflag[g] & kSolid;
             if (any) dofCh.push_back((int32_t)c);
         }
         C->nNodeChunks = (int64_t)nodeCh.size();
         C->nDofChunks = (int64_t)dofCh.size();
+        nodeCh.insert(nodeCh.end(), 3, 0);  // padding entries (see Span::at)
+        dofCh.insert(dofCh.end(), 3, 0);
         SGLD_TRY(C->nodeChunks.upload(nodeCh.data(), nodeCh.size()));
         SGLD_TRY(C->dofChunks.upload(dofCh.data(), dofCh.size()));
         // element layers of each tiled K.u patch window (element rows R0-1 .. R0+kTO-1
@@ -2929,15 +2945,17 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
                 l.rtilesCz = czc;
             }
             if (!C->full) {
-                // 64-entry chunks of this level's DOF vectors with a solid node
+                // chunks of this level's DOF vectors with a solid node
                 std::vector<int32_t> ch;
-                for (int64_t c = 0; 64 * c < n; ++c) {
+                const int64_t cs = (int64_t)1 << C->chunkLog;
+                for (int64_t c = 0; cs * c < n; ++c) {
                     bool any = false;
-                    for (int64_t g = 64 * c / 3; g <= std::min(64 * c + 63, n - 1) / 3 && !any; ++g)
+                    for (int64_t g = cs * c / 3; g <= std::min(cs * c + cs - 1, n - 1) / 3 && !any; ++g)
                         any = l.hflag[g] & kSolid;
                     if (any) ch.push_back((int32_t)c);
                 }
                 l.nDofChunks = (int64_t)ch.size();
+                ch.insert(ch.end(), 3, 0);  // padding entries (see Span::at)
                 SGLD_TRY(l.dofChunks.upload(ch.data(), ch.size()));
             }
         }
@@ -3245,6 +3263,20 @@ int assemble_impl(sgld_ctx *C, double omega) {
     SGLD_TRY(C->cWcol.alloc(tri_row_off(nf)));
     hipLaunchKernelGGL(k_tri_pack, dim3(bounded_blocks(nf * nf, 256)), dim3(256), 0, C->s, dW.p, nf, C->cWrow.p,
                        C->cWcol.p);
+    // FP32 copies for the FP32 inner solve once the FP64 factor outgrows the
+    // Infinity Cache share (8 n^2 bytes > 128 MB, n > 4096)
+    C->cWrow32.release();
+    C->cWcol32.release();
+    if (8.0 * (double)nf * (double)nf > 128.0 * (1 << 20) && !std::getenv("SGLD_COARSE_W64")) {
+        const int64_t nw = tri_row_off(nf);
+        SGLD_TRY(C->cWrow32.alloc(nw));
+        SGLD_TRY(C->cWcol32.alloc(nw));
+        const Span all = Span::all(nw);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(nw)), dim3(256), 0, C->s, C->cWrow32.p,
+                           (const double *)C->cWrow.p, 1.0, all);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(nw)), dim3(256), 0, C->s, C->cWcol32.p,
+                           (const double *)C->cWcol.p, 1.0, all);
+    }
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(C->cFree.upload(freeIdx.data(), freeIdx.size()));
     SGLD_TRY(C->cY.alloc(16 * nf));

@@ -50,7 +50,7 @@ struct Level {
     DevBuf<int32_t> rtiles;   // irregular domains: tiles of the tiled restriction into this level
     int64_t nRtiles = 0;
     int rtilesCz = 0;         // z-chunk the tile list was built for (0: none)
-    DevBuf<int32_t> dofChunks;  // irregular domains, levels >= 2: 64-entry DOF chunks with a solid node
+    DevBuf<int32_t> dofChunks;  // irregular domains, levels >= 2: DOF chunks with a solid node
     int64_t nDofChunks = 0;
 };
 
@@ -99,6 +99,7 @@ struct sgld_ctx {
     // coarsest level: packed rows / columns of W = L^-1 (K_L(free, free) = L L^T),
     // its free lattice DOFs (ascending) and the intermediate W r (16 cases wide)
     sgld::DevBuf<double> cWrow, cWcol, cY;
+    sgld::DevBuf<float> cWrow32, cWcol32;  // FP32 copies for the FP32 inner solve (n > 4096 only)
     sgld::DevBuf<int32_t> cFree;
     int64_t nCoarseFree = 0;
     sgld::DevBuf<double> b, x, r, z, p, p2, q;
@@ -130,13 +131,14 @@ struct sgld_ctx {
     bool bRestrictTiled = true;  // batched span-2 restriction: LDS-tiled layer march (else node-centric)
     sgld::TiledSchedule ksched[2];  // tiled K.u schedule: [0] FP64, [1] FP32
     // Irregular domains (full == false): the finest-level passes visit only
-    // what holds solid material.  dofChunks / nodeChunks list the 64-entry
-    // chunks of the DOF and node vectors with a solid node; kwork[q] lists the
+    // what holds solid material.  dofChunks / nodeChunks list the chunks of
+    // 2^chunkLog entries of the DOF and node vectors with a solid node; kwork[q] lists the
     // tiled K.u work items of schedule q whose output nodes touch a solid
     // element; hPatchLayer[l * patches + patch] says whether element layer l of
     // a patch's 16 x 16 element window holds a solid element.
     sgld::DevBuf<int32_t> dofChunks, nodeChunks, kwork[2];
     int64_t nDofChunks = 0, nNodeChunks = 0, nKwork[2] = {0, 0};
+    int chunkLog = 5;  // chunks of 2^chunkLog entries: 16, 32 or 64
     std::vector<uint8_t> hPatchLayer;
     int kOcc[2] = {1, 1};           // resident tiled K.u blocks per compute unit
     int numCU = 0;
@@ -154,11 +156,11 @@ inline int grid_blocks(int64_t n) {
 // Finest-level DOF / node passes: every entry of a full cuboid, the listed
 // chunks of an irregular domain.
 inline Span dof_span(const sgld_ctx *C) {
-    return C->full ? Span::all(C->nDof()) : Span{C->dofChunks.p, C->nDof(), 64 * C->nDofChunks};
+    return C->full ? Span::all(C->nDof()) : Span::listed(C->dofChunks.p, C->nDof(), C->nDofChunks, C->chunkLog);
 }
 inline Span node_span(const sgld_ctx *C) {
     const int64_t n = C->lv[0].g.nodes();
-    return C->full ? Span::all(n) : Span{C->nodeChunks.p, n, 64 * C->nNodeChunks};
+    return C->full ? Span::all(n) : Span::listed(C->nodeChunks.p, n, C->nNodeChunks, C->chunkLog);
 }
 // Sets the z-chunk of tiled K.u schedule q (0 FP64, 1 FP32) and, on an
 // irregular domain, rebuilds its list of work items.

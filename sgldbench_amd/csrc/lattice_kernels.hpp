@@ -79,20 +79,37 @@ struct Ctl {
 };
 
 // Entries visited by a grid-stride pass over a vector of length n: all of
-// them when `chunks` is null, else the listed 64-entry chunks (chunk c is
-// [64c, 64c + 64) clipped to n).  Irregular domains list the chunks that hold
-// a solid node; entries outside them are void, stay zero and are skipped.
-// Position t of the pass maps to entry at(t); positions t of one wavefront
-// share t >> 6 (grid strides are multiples of 64), so the chunk id is a
-// scalar load.
+// them when `chunks` is null, else the listed chunks of 2^log entries (chunk
+// c is [c 2^log, (c + 1) 2^log) clipped to n).  Irregular domains list the
+// chunks that hold a solid node; entries outside them are void, stay zero and
+// are skipped.  Position t of the pass maps to entry at(t).  The positions
+// of one wavefront share t >> 6 (grid strides are multiples of 64), so its
+// chunk ids are scalar loads: one for 64-entry chunks, two for 32-entry and
+// four for 16-entry ones (which visit fewer void entries; a list carries
+// three padding entries so that these loads stay inside it).
 struct Span {
     const int32_t *chunks;
-    int64_t n, count;  // count = n, or 64 x listed chunks
-    __host__ __device__ static Span all(int64_t len) { return Span{nullptr, len, len}; }
+    int64_t n, count;  // count = n, or 2^log x listed chunks
+    int log;
+    __host__ __device__ static Span all(int64_t len) { return Span{nullptr, len, len, 6}; }
+    __host__ __device__ static Span listed(const int32_t *ch, int64_t len, int64_t nChunks, int lg) {
+        return Span{ch, len, nChunks << lg, lg};
+    }
     __device__ __forceinline__ int64_t at(int64_t t) const {
         if (!chunks) return t;
-        const int ci = __builtin_amdgcn_readfirstlane((int)(t >> 6));
-        return ((int64_t)chunks[ci] << 6) + (t & 63);
+        if (log == 6) {
+            const int ci = __builtin_amdgcn_readfirstlane((int)(t >> 6));
+            return ((int64_t)chunks[ci] << 6) + (t & 63);
+        }
+        const int cw = __builtin_amdgcn_readfirstlane((int)(t >> 6));
+        if (log == 5) {
+            const int32_t c0 = chunks[2 * cw], c1 = chunks[2 * cw + 1];
+            return ((int64_t)((t & 32) ? c1 : c0) << 5) + (t & 31);
+        }
+        const int32_t c0 = chunks[4 * cw], c1 = chunks[4 * cw + 1], c2 = chunks[4 * cw + 2],
+                      c3 = chunks[4 * cw + 3];
+        const int q = (int)(t >> 4) & 3;
+        return ((int64_t)(q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3) << 4) + (t & 15);
     }
 };
 
