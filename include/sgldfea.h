@@ -31,7 +31,7 @@ enum {
     SGLD_ERR_NOT_SPD = 3,   /* coarsest matrix not positive definite */
     SGLD_ERR_MAXIT = 4,     /* PCG hit maxIT without reaching tol */
     SGLD_ERR_STATE = 5,     /* call out of order (e.g. solve before assemble) */
-    SGLD_ERR_NOMEM = 6      /* device allocation failed */
+    SGLD_ERR_NOMEM = 6      /* device or host allocation failed */
 };
 
 enum { SGLD_PREC_FP64 = 0, SGLD_PREC_MIXED = 1,

@@ -2057,7 +2057,11 @@ const char *stream_name(bool f64, bool nt, int kind) {
         "k_prolong_dir<double, 2, false>", "k_prolong_dir<double, 2, true>", "k_prolong_dir<double, 4, false>",
         "k_prolong_dir<double, 4, true>",  "k_update_r<double, false>",      "k_update_r<double, true>"};
     return n[6 * f64 + 2 * kind + nt];
-}const char *owner_name(bool f64, bool stage) {
+}
+
+// (the FP32 owner K.u compiled for 5 waves per SIMD, 96 VGPRs with 16-25
+// spilled, measured slower than at 4: 125 M-DOF ellipsoids 504 against 483 us)
+const char *owner_name(bool f64, bool stage) {
     static const char *const n[4] = {"k_kbyu_owner<float, false>", "k_kbyu_owner<float, true>",
                                      "k_kbyu_owner<double, false>", "k_kbyu_owner<double, true>"};
     return n[2 * f64 + stage];
