@@ -553,6 +553,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
     (void)f00; (void)f10; (void)f11;
     uint32_t lOff[kStageSlots];
     double dotacc = 0.0;
+    // every blocksOnXcd-th item of the XCD's range from slotInXcd: the blocks
+    // of one XCD work on neighbouring items at any time (tickets from a
+    // per-XCD atomic counter instead balanced the irregular domain's uneven
+    // items better, 125 M-DOF ellipsoids FP32 481 -> 466 us, but made the
+    // p.q partials depend on timing and cost 36 -> 52 us on 256x128x128)
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
         int64_t rem = wlist ? wlist[w] : w;
         const int tz = (int)(rem / ((int64_t)tm.tilesR * tm.tilesC));
@@ -1366,7 +1371,11 @@ __global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__r
 // W32^T W32 is as symmetric positive definite as W^T W, and the product reads
 // half the bytes).  One wavefront per row.
 // y = W r(free): row i of W holds entries 0..i.
-template <class T, class TW>
+// With Lds (n <= kCoarseLds), a block first gathers the entries r(free)
+// its rows need into LDS, so the dot products read W and LDS only (the
+// free-DOF gather inside the loop made each step two dependent loads).
+constexpr int64_t kCoarseLds = 4096;
+template <class T, class TW, bool Lds>
 __global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wrow,
                                                       const int32_t *__restrict__ freeIdx, int64_t n,
                                                       const T *__restrict__ r, double *__restrict__ y,
@@ -1374,10 +1383,17 @@ __global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wro
     if (stopped(ctl)) return;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    __shared__ double rs[Lds ? kCoarseLds : 1];
+    if constexpr (Lds) {
+        const int64_t jEnd = min<int64_t>(n, (int64_t)(blockIdx.x + 1) * (blockDim.x >> 6));
+        for (int64_t j = threadIdx.x; j < jEnd; j += blockDim.x) rs[j] = (double)r[freeIdx[j]];
+        __syncthreads();
+    }
     if (row >= n) return;
     const TW *w = Wrow + tri_row_off(row);
     double acc = 0.0;
-    for (int64_t j = lane; j <= row; j += 64) acc += (double)w[j] * (double)r[freeIdx[j]];
+    for (int64_t j = lane; j <= row; j += 64)
+        acc += (double)w[j] * (Lds ? rs[j] : (double)r[freeIdx[j]]);
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) y[row] = acc;
 }
@@ -2318,13 +2334,14 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
     Level &Lc = C->lv[L - 1];
     const int64_t n = C->nCoarseFree;
     if (sizeof(T) == 4 && C->cWrow32.p) {
-        hipLaunchKernelGGL((k_coarse_lower<T, float>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
+        hipLaunchKernelGGL((k_coarse_lower<T, float, false>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
                            (const float *)C->cWrow32.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
         hipLaunchKernelGGL((k_coarse_upper<T, float>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
                            (const float *)C->cWcol32.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
     } else {
-        hipLaunchKernelGGL((k_coarse_lower<T, double>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
-                           (const double *)C->cWrow.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
+        auto lower = n <= kCoarseLds ? k_coarse_lower<T, double, true> : k_coarse_lower<T, double, false>;
+        hipLaunchKernelGGL(lower, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, (const double *)C->cWrow.p,
+                           C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
         hipLaunchKernelGGL((k_coarse_upper<T, double>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
                            (const double *)C->cWcol.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
     }
@@ -3043,7 +3060,9 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
             // z-chunks of 12 node layers, measured (FP64 / FP32 us, chunk 8 / 12 / 16):
             // 256x128x128 72 / 65 / 73 and 50 / 47 / 52; 256^3 320 / 301 / 295 and
             // 196 / 188 / 184; 125 M-DOF ellipsoids 918 / 887 / 894 and 538 / 517 / 521
-            SGLD_TRY(set_kchunk(C.get(), q, 12));
+            int cz = 12;
+            if (const char *env = std::getenv("SGLD_KU_CZ")) cz = std::max(2, std::min(256, std::atoi(env)));
+            SGLD_TRY(set_kchunk(C.get(), q, cz));
         }
     }
     SGLD_HIP(hipStreamSynchronize(C->s));
