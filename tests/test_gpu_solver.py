@@ -416,3 +416,39 @@ def test_contexts_with_different_element_matrices_interleave():
         for U, its, rr in out[k]:
             assert rr <= 1e-8 and abs(its - itso) <= 1
             assert abs(F @ U - F @ Uo) <= 1e-9 * abs(F @ Uo)
+
+
+def test_chunk_sizes_change_only_rounding(monkeypatch):
+    """Irregular domains visit the 16-, 32- or 64-entry chunks of the DOF
+    vectors that hold a solid node (chosen when a context is created).  The
+    V-cycle and K.u are entry-wise identical whatever the chunking; the PCG
+    reductions sum in another order, so solves agree to rounding."""
+    vox = holes(64, 32, 32, 9, nholes=8, r=5)
+    F, fixed = problems.cantilever3(vox)
+    E = np.ones(int(vox.sum()))
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(F.size) * (1 - fixed)
+    out = {}
+    for log in (6, 5, 4):
+        monkeypatch.setenv("SGLD_CHUNK_LOG", str(log))
+        print("case: chunk log", log, flush=True)
+        g = solver.FEASolver(vox, 4)
+        g.set_fixed(fixed)
+        g.set_modulus(E)
+        g.assemble(0.6)
+        res = {"kbyu": g.kbyu(u), "vcycle": g.vcycle(u)}
+        for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+            U, its, rr = g.pcg(F, tol=1e-8, maxIT=500, precision=prec)
+            assert g.converged
+            res[prec] = (U, its)
+        g.close()
+        out[log] = res
+    for log in (5, 4):
+        assert np.array_equal(out[log]["kbyu"], out[6]["kbyu"])
+        assert np.array_equal(out[log]["vcycle"], out[6]["vcycle"])
+        for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+            U, its = out[log][prec]
+            U6, its6 = out[6][prec]
+            assert abs(its - its6) <= 2, (log, prec, its, its6)
+            # both meet tol 1e-8 along nearly the same iterates
+            assert np.max(np.abs(U - U6)) <= 1e-6 * np.max(np.abs(U6))
