@@ -8,7 +8,7 @@
 // g, component a, case j.  K.u runs on v_mfma_f32_32x32x16_bf16 with BF16
 // hi/lo split operands (x = hi + lo, hi = bf16(x), lo = bf16(x - hi): 16
 // significant bits) and FP32 accumulation:
-//   q_e = Ke_hi d_hi + Ke_hi d_lo + Ke_lo d_hi,  d = E_e (u_e - u_e,node0)
+//   q_e = E_e (Ke_hi d_hi + Ke_hi d_lo + Ke_lo d_hi),  d = u_e - u_e,node0
 // Subtracting the element's node-0 displacement (a rigid translation, in the
 // null space of Ke) before the split leaves only strain-sized differences to
 // round, so smooth directions - the usual case in a multigrid-preconditioned
@@ -44,14 +44,25 @@ struct BCtl {
     int active[NR], restart[NR], its[NR], pad[NR];
 };
 
-// BF16 hi/lo split of 8 values into MFMA operand vectors.
-__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
+// BF16 hi/lo split of 8 values into MFMA operand vectors, pairwise: one
+// packed conversion gives both hi parts (round to nearest even), their FP32
+// values come back by a shift and a mask, one more packed conversion gives
+// the lo parts (3 instructions per value).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split8p(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
+    uint32_t h[4], l[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const __bf16 h = (__bf16)v[i];
-        hi[i] = h;
-        lo[i] = (__bf16)(v[i] - (float)h);
+    for (int i = 0; i < 4; ++i) {
+        const bf16x2 hp = {(__bf16)v[2 * i], (__bf16)v[2 * i + 1]};
+        h[i] = __builtin_bit_cast(uint32_t, hp);
+        const float h0 = __uint_as_float(h[i] << 16), h1 = __uint_as_float(h[i] & 0xffff0000u);
+        const bf16x2 lp = {(__bf16)(v[2 * i] - h0), (__bf16)(v[2 * i + 1] - h1)};
+        l[i] = __builtin_bit_cast(uint32_t, lp);
     }
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 hv = {h[0], h[1], h[2], h[3]}, lv = {l[0], l[1], l[2], l[3]};
+    hi = __builtin_bit_cast(bf16x8, hv);
+    lo = __builtin_bit_cast(bf16x8, lv);
 }
 
 // Sum v over the threads of a block that share threadIdx.x % 16 (the case
@@ -81,21 +92,29 @@ __device__ __forceinline__ void block_sum16(double v, double *__restrict__ out) 
 // layers of p are staged in LDS (coalesced float4 loads, prefetched one layer
 // ahead in registers) for the 17 x 9 patch nodes; the 16 x 8 element columns
 // touching the patch are MFMA column pairs (2 elements x 16 cases = 32
-// columns), one pair per wavefront per colour pass:
+// columns):
 //   B[k][col]: k 0-7 x of nodes 0-7, 8-15 y of nodes 0-7, 16-19 z of nodes
-//              0-3, 24-27 z of nodes 4-7 (rest 0), as E_e (p - p_node0)
-//              split hi/lo;
+//              0-3, 24-27 z of nodes 4-7 (rest 0), as p - p_node0 split
+//              hi/lo;
 //   A = Ke with rows i = 8 a + n (component a, element-local node n) and the
 //       same column order;
 //   D = A B, lane l holds rows 8 (r/4) + 4 (l/32) + r%4: lanes 0-31 the
 //       forces of the lower face (nodes 0-3), lanes 32-63 the upper face.
-// The 4 colour passes (columns with (ei % 2, ej % 2) fixed) touch disjoint
-// nodes, so face forces are added into a 2-layer LDS accumulator ring with
-// plain read-add-write (FP32 LDS atomics are several times slower than FP64
-// ones on this part).  A node layer is complete after the element layers
-// below and above it; its owned nodes are then masked (fixed / void DOFs 0,
-// as Solving_KbyU_MatrixFree.m:39), stored, dotted with p for the PCG step
-// length, and the slot cleared.
+// Schedule (owner-accumulated):
+// * each wavefront w owns a fixed block of 2 x 4 element columns of the patch
+//   (element rows 2 (w % 8) + {0,1}, columns 4 (w / 8) + {0..3}) and adds its
+//   elements' face forces into a private LDS accumulator of the block's 3 x 5
+//   nodes (two node layers, the lower and the upper face), so no colour passes
+//   or block barriers separate the element pairs; a wavefront's four pairs per
+//   element layer are (r, c) and (r, c + 2) of its block, which share no node;
+// * the modulus scales D after the product (B = p_e - p_e,node0 split hi/lo);
+// * once an element layer is done (one block barrier), node layer el is
+//   complete: every owned value sums the 1, 2 or 4 private copies of its node
+//   in a fixed order (deterministic), clears them, is masked and stored; the
+//   stage slot of that layer is refilled in the same phase, and a second
+//   barrier ends the layer;
+// * LDS strides are padded so that the two elements of a pair (and the two
+//   faces) fall in different banks.
 // ----------------------------------------------------------------------------
 constexpr int kBT = 1024;                 // threads per block (16 wavefronts)
 constexpr int kPE_R = 16, kPE_C = 8;      // element columns per patch (rows, cols)
@@ -106,7 +125,6 @@ constexpr int kNV = 3 * NR;               // floats per node (3 components x 16 
 constexpr int kLay = kPN * kNV;           // floats per staged / accumulated layer
 constexpr int kLd4 = kLay / 4;            // float4 per layer
 constexpr int kLdPer = (kLd4 + kBT - 1) / kBT;
-constexpr int kStPer = (kPO_R * kPO_C * kNV + kBT - 1) / kBT;  // stored floats per thread and layer
 static_assert(kPE_R * kPE_C <= kBT && kPN <= kBT, "staging threads");
 
 __device__ __forceinline__ int bcol_k(int k) {  // B/A column k -> element DOF (3 node + comp), -1 = zero
@@ -117,23 +135,58 @@ __device__ __forceinline__ int bcol_k(int k) {  // B/A column k -> element DOF (
     return -1;
 }
 
+constexpr int kACS = 3 * kNV + 8;              // private accumulator: floats per node column (3 nodes + pad)
+constexpr int kAW = 5 * kACS;                  // floats per wavefront block (5 node columns)
+constexpr int kAZero = (kBT / 64) * kAW;       // 16 floats of zeros after the blocks (never accumulated into)
+constexpr int kASlot = kAZero + 32;            // floats per node layer
+constexpr int kOItems = kPO_R * kPO_C * 3 * 4;  // owned (node, component, 4 cases) items per node layer (1260)
+constexpr int kOPer = (kOItems + kBT - 1) / kBT;  // per thread (2)
+
+// Sum over the threads of a block of four per-thread values that belong to
+// cases 4 (t % 4) + i; thread j < 16 writes out[j].
+__device__ __forceinline__ void block_sum16q(const double (&v)[4], double *__restrict__ out) {
+    __shared__ double sh[kBT / 64][NR];
+    double d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        d[i] = v[i];
+        d[i] += __shfl_xor(d[i], 32, 64);
+        d[i] += __shfl_xor(d[i], 16, 64);
+        d[i] += __shfl_xor(d[i], 8, 64);
+        d[i] += __shfl_xor(d[i], 4, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < 4)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sh[w][4 * lane + i] = d[i];
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        double s = 0.0;
+        for (int i = 0; i < kBT / 64; ++i) s += sh[i][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+template <int CS>  // staged floats per patch node column (17 nodes x 48 + pad)
 __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, float *__restrict__ q,
-                                                  const float *__restrict__ E,
-                                                  const uint8_t *__restrict__ nflag, Grid gd, TileMap tm,
-                                                  const float *__restrict__ Ke, double *__restrict__ partial) {
-    __shared__ float stage[2][kLay];
-    __shared__ float acc[2][kLay];
-    __shared__ float Es[2][kPE_R * kPE_C];  // moduli of an element layer (ring)
-    __shared__ uint8_t Fs[2][kPN];          // node flags of a staged layer (ring, as stage)
+                                                      const float *__restrict__ E,
+                                                      const uint8_t *__restrict__ nflag, Grid gd, TileMap tm,
+                                                      const float *__restrict__ Ke, double *__restrict__ partial) {
+    constexpr int kSL = kPN_C * CS;  // floats per staged node layer
+    static_assert(CS % 4 == 0 && 2 * kASlot < (1 << 16) && kSL < (1 << 13), "packed LDS indices");
+    __shared__ __attribute__((aligned(16))) float stage[2 * kSL];
+    __shared__ __attribute__((aligned(16))) float acc[2 * kASlot];
+    __shared__ float Es[2][kPE_R * kPE_C];
+    __shared__ uint8_t Fs[2][kPN + 3];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int g = lane >> 5;   // 0: lower-face output rows; 1: upper face
+    const int g = lane >> 5;  // 0: lower-face rows of D (nodes 0-3); 1: upper face (4-7)
     const int col = lane & 31;
     const int j = col & 15, e = col >> 4;
+    const int wr = wave & 7, wc = wave >> 3;
     const int NY = gd.ny + 1, NX = gd.nx + 1, NZ = gd.nz + 1;
     const int64_t pl = (int64_t)NY * NX;
     const int64_t ple = (int64_t)gd.ny * gd.nx;
 
-    // A operands: rows i = lane % 32 (8 comp + node), columns 16 s + 8 g + 0..7
     bf16x8 Ahi[2], Alo[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -145,7 +198,41 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
             Ahi[s][i] = hi;
             Alo[s][i] = (__bf16)(a - (float)hi);
         }
-    for (int i = t; i < 2 * kLay; i += kBT) (&acc[0][0])[i] = 0.0f;
+    // lane bases: stage (node (2 wr, 4 wc + 2 e) of the patch, case j), private
+    // accumulator (local node (0, 2 e)), modulus of element (2 wr, 4 wc + 2 e)
+    const int sLane = (4 * wc + 2 * e) * CS + (2 * wr) * kNV + j;
+    const int aLane = wave * kAW + (2 * e) * kACS + j;
+    const int eLane = 2 * wr + kPE_R * (4 * wc + 2 * e);
+    // Owned items of this thread: item it = t + kBT i is node-component
+    // it / 4 = 3 node + comp of the 15 x 7 owned nodes (oi, oj) (node =
+    // oi + 15 oj, staged patch node (oi + 1, oj + 1)) and cases 4 (t % 4) +
+    // 0..3.  The node lies in the block of wavefront (rp / 2, cp / 4) and, for
+    // even rp, also in the one below (local row 2), for cp = 4 also in the one
+    // to its left (local column 4).  Packed per item: oA = primary | row
+    // partner << 16, oC = column partner | both << 16 (absent partners point
+    // at zeros), oS = stage index | staged node << 13 | comp << 21 | owned << 23
+    // | oi << 24 | oj << 28, oG = global float offset from the patch's first
+    // owned node.
+    uint32_t oA[kOPer], oC[kOPer], oS[kOPer], oG[kOPer];
+    constexpr int kRowP = -kAW + 2 * kNV, kColP = -8 * kAW + 4 * kACS;
+#pragma unroll
+    for (int i = 0; i < kOPer; ++i) {
+        const int it = t + kBT * i;
+        const int nc = it >> 2, quad = it & 3;
+        const int node = nc / 3, comp = nc - 3 * node;
+        const int oi = node % kPO_R, oj = node / kPO_R;
+        const int rp = oi + 1, cp = oj + 1;
+        const int w = (rp >> 1) + 8 * (cp >> 2), lr = rp & 1, lc = cp & 3;
+        const int within = comp * NR + 4 * quad;
+        const int o0 = w * kAW + lc * kACS + lr * kNV + within;
+        const int o1 = lr == 0 ? o0 + kRowP : kAZero, o2 = lc == 0 ? o0 + kColP : kAZero;
+        const int o3 = (lr == 0 && lc == 0) ? o0 + kRowP + kColP : kAZero;
+        oA[i] = (uint32_t)o0 | (uint32_t)o1 << 16;
+        oC[i] = (uint32_t)o2 | (uint32_t)o3 << 16;
+        oS[i] = (uint32_t)(cp * CS + rp * kNV + within) | (uint32_t)(rp + kPN_R * cp) << 13 | (uint32_t)comp << 21 |
+                (uint32_t)(it < kOItems) << 23 | (uint32_t)oi << 24 | (uint32_t)oj << 28;
+        oG[i] = (uint32_t)((oi + NY * oj) * kNV + within);
+    }
 
     const int64_t W = tm.work();
     const int G = gridDim.x;
@@ -153,7 +240,7 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
     const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
     const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
     const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
-    double dotacc = 0.0;  // case t % 16 in the store phase
+    double dacc[4] = {0.0, 0.0, 0.0, 0.0};  // cases 4 (t % 4) + i
     const float4 *p4 = reinterpret_cast<const float4 *>(p);
 
     for (int64_t w = wBeg + slotInXcd; w < wEnd; w += blocksOnXcd) {
@@ -163,8 +250,7 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
         const int tc = (int)(rem / tm.tilesR), tr = (int)(rem % tm.tilesR);
         const int R0 = tr * kPO_R, C0 = tc * kPO_C, Z0 = tz * tm.cz;
         const int Zend = min(Z0 + tm.cz, NZ);
-        // this thread's staged float4s: patch node, float4 within the node, lattice node column
-        int64_t src[kLdPer];
+        int src[kLdPer];  // float4 index within a node layer (a layer has < 2^31 / 12 nodes)
         int dst[kLdPer];
 #pragma unroll
         for (int i = 0; i < kLdPer; ++i) {
@@ -175,141 +261,154 @@ __global__ void __launch_bounds__(kBT, 1) k_bkbyu(const float *__restrict__ p, f
                 const int node = f / (kNV / 4), w4 = f % (kNV / 4);
                 const int rp = node % kPN_R, cp = node / kPN_R;
                 const int r = R0 - 1 + rp, c = C0 - 1 + cp;
-                dst[i] = node * (kNV / 4) + w4;
-                if (r >= 0 && r < NY && c >= 0 && c < NX) src[i] = ((int64_t)r + (int64_t)NY * c) * (kNV / 4) + w4;
+                dst[i] = cp * CS + rp * kNV + 4 * w4;
+                if (r >= 0 && r < NY && c >= 0 && c < NX) src[i] = (r + NY * c) * (kNV / 4) + w4;
             }
         }
         auto load_layer = [&](int lay, float4 (&v)[kLdPer]) {
+            const float4 *pl4 = p4 + pl * (int64_t)lay * (kNV / 4);
 #pragma unroll
             for (int i = 0; i < kLdPer; ++i)
-                v[i] = (src[i] >= 0 && lay >= 0 && lay < NZ) ? p4[src[i] + pl * lay * (kNV / 4)]
-                                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+                v[i] = (src[i] >= 0 && lay >= 0 && lay < NZ) ? pl4[src[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
         };
         auto put_layer = [&](float *sl, const float4 (&v)[kLdPer]) {
 #pragma unroll
             for (int i = 0; i < kLdPer; ++i)
-                if (dst[i] >= 0) reinterpret_cast<float4 *>(sl)[dst[i]] = v[i];
+                if (dst[i] >= 0) *reinterpret_cast<float4 *>(sl + dst[i]) = v[i];
         };
+        uint32_t inRange = 0;  // owned item i inside the lattice
+#pragma unroll
+        for (int i = 0; i < kOPer; ++i) {
+            const int oi = (oS[i] >> 24) & 15, oj = (int)(oS[i] >> 28);
+            if (((oS[i] >> 23) & 1) && R0 + oi < NY && C0 + oj < NX) inRange |= 1u << i;
+        }
+        float *const qPatch = q + ((int64_t)R0 + (int64_t)NY * C0) * kNV;
         float4 pf[kLdPer];
-        // thread t < 128 stages the modulus of patch element column t, t < 153
-        // the flags of patch node t, one layer ahead of use (as the node layers)
-        int64_t eSrc = -1, fSrc = -1;
+        int eSrc = -1, fSrc = -1;
         if (t < kPE_R * kPE_C) {
             const int er = R0 - 1 + t % kPE_R, ec = C0 - 1 + t / kPE_R;
-            if (er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx) eSrc = er + (int64_t)gd.ny * ec;
+            if (er >= 0 && er < gd.ny && ec >= 0 && ec < gd.nx) eSrc = er + gd.ny * ec;
         }
         if (t < kPN) {
             const int r = R0 - 1 + t % kPN_R, c = C0 - 1 + t / kPN_R;
-            if (r >= 0 && r < NY && c >= 0 && c < NX) fSrc = r + (int64_t)NY * c;
+            if (r >= 0 && r < NY && c >= 0 && c < NX) fSrc = r + NY * c;
         }
         auto load_E = [&](int el) {
-            return (eSrc >= 0 && el >= 0 && el < gd.nz) ? E[eSrc + ple * el] : 0.0f;
+            return (eSrc >= 0 && el >= 0 && el < gd.nz) ? E[ple * el + eSrc] : 0.0f;
         };
         auto load_F = [&](int lay) {
-            return (fSrc >= 0 && lay >= 0 && lay < NZ) ? nflag[fSrc + pl * lay] : (uint8_t)0;
+            return (fSrc >= 0 && lay >= 0 && lay < NZ) ? nflag[pl * lay + fSrc] : (uint8_t)0;
         };
-        __syncthreads();  // previous work item is done with the stage
+        __syncthreads();  // the previous work item is done with LDS
+        for (int i = t; i < 2 * kASlot / 4; i += kBT) reinterpret_cast<float4 *>(acc)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         load_layer(Z0 - 1, pf);
-        put_layer(stage[(Z0 - 1) & 1], pf);
+        put_layer(stage + ((Z0 - 1) & 1) * kSL, pf);
         load_layer(Z0, pf);
-        put_layer(stage[Z0 & 1], pf);
+        put_layer(stage + (Z0 & 1) * kSL, pf);
         load_layer(Z0 + 1, pf);  // in flight
         if (t < kPE_R * kPE_C) Es[(Z0 - 1) & 1][t] = load_E(Z0 - 1);
-        if (t < kPN) {
-            Fs[(Z0 - 1) & 1][t] = load_F(Z0 - 1);
-            Fs[Z0 & 1][t] = load_F(Z0);
-        }
         float eNext = load_E(Z0);
-        uint8_t fNext = load_F(Z0 + 1);
+        uint8_t fNext = load_F(Z0);  // flags of node layer el + 1, staged during layer el
         __syncthreads();
         for (int el = Z0 - 1; el < Zend; ++el) {
-            const bool layerOk = el >= 0 && el < gd.nz;
-            const float *sb = stage[el & 1], *st = stage[(el + 1) & 1];
+            const int s0 = el & 1;
+            const float *sb = stage + s0 * kSL, *st = stage + (s0 ^ 1) * kSL;
+            const float *sz = stage + (s0 ^ g) * kSL + 2 * NR;  // z of this lane's face
+            float *af = acc + (s0 ^ g) * kASlot;                // this lane's face layer
+            // the lower face of element layer Z0 - 1 is node layer Z0 - 1, not an output
+            const float fo = (g == 1 || el >= Z0) ? 1.0f : 0.0f;
+            // the wavefront's 4 element pairs: rows tr2 of its block, columns (tc2, tc2 + 2)
 #pragma unroll 1
-            for (int colour = 0; colour < 4; ++colour) {
-                if (!layerOk) break;
-                const int ci = colour & 1, cj = colour >> 1;
-                // 32 columns of the colour = 16 pairs, one per wavefront
-                const int ei = 2 * ((wave & 3) + 4 * e) + ci, ej = 2 * (wave >> 2) + cj;
-                const float Ee = Es[el & 1][ei + kPE_R * ej];
-                // element-local node n at patch node (ei + 1 - y_n) + 17 (ej + x_n)
-                const int nb = (ei + kPN_R * ej) * kNV + j;
-                float va[8], vb[4];
-                float ref_a, ref_b;
-                {
-                    const int ca = g;  // component of va: x (g = 0) or y (g = 1)
-                    // nodes 0-3 lower face (stage sb), 4-7 upper face (stage st)
-                    const int offn[4] = {1, 1 + kPN_R, kPN_R, 0};
+            for (int tr2 = 0; tr2 < 2; ++tr2)
+#pragma unroll
+                for (int tc2 = 0; tc2 < 2; ++tc2) {
+                    const int b = sLane + tr2 * kNV + tc2 * CS;
+                    // local node n: (dr, dc) = (1,0), (1,1), (0,1), (0,0)
+                    const int so[4] = {kNV, kNV + CS, CS, 0};
+                    float va[8], vb[4];
 #pragma unroll
                     for (int n = 0; n < 4; ++n) {
-                        va[n] = sb[nb + offn[n] * kNV + ca * NR];
-                        va[n + 4] = st[nb + offn[n] * kNV + ca * NR];
-                        vb[n] = (g ? st : sb)[nb + offn[n] * kNV + 2 * NR];
+                        va[n] = sb[b + so[n] + NR * g];
+                        va[n + 4] = st[b + so[n] + NR * g];
+                        vb[n] = sz[b + so[n]];
                     }
-                    ref_a = va[0];
-                    ref_b = g ? sb[nb + offn[0] * kNV + 2 * NR] : vb[0];
-                }
-                float da[8], db[8];
+                    const float ref_a = va[0], ref_b = sb[b + so[0] + 2 * NR];
+                    float da[8], db[8];
+                    da[0] = 0.0f;
 #pragma unroll
-                for (int n = 0; n < 8; ++n) da[n] = Ee * (va[n] - ref_a);
+                    for (int n = 1; n < 8; ++n) da[n] = va[n] - ref_a;
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    db[n] = Ee * (vb[n] - ref_b);
-                    db[n + 4] = 0.0f;
-                }
-                bf16x8 Bh0, Bl0, Bh1, Bl1;
-                split8(da, Bh0, Bl0);
-                split8(db, Bh1, Bl1);
-                f32x16 D = {};
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bh0, D, 0, 0, 0);
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bl0, D, 0, 0, 0);
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[0], Bh0, D, 0, 0, 0);
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bh1, D, 0, 0, 0);
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bl1, D, 0, 0, 0);
-                D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[1], Bh1, D, 0, 0, 0);
-                const int tl = el + g;  // node layer of this lane's face
-                if (Ee != 0.0f && tl >= Z0 && tl < Zend) {
-                    float *a = &acc[tl & 1][nb];
-                    const int offn[4] = {1, 1 + kPN_R, kPN_R, 0};
+                    for (int n = 0; n < 4; ++n) {
+                        db[n] = vb[n] - ref_b;
+                        db[n + 4] = 0.0f;
+                    }
+                    bf16x8 Bh0, Bl0, Bh1, Bl1;
+                    split8p(da, Bh0, Bl0);
+                    split8p(db, Bh1, Bl1);
+                    f32x16 D = {};
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bh0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[0], Bl0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[0], Bh0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bh1, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi[1], Bl1, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo[1], Bh1, D, 0, 0, 0);
+                    const float Ee = fo * Es[s0][eLane + tr2 + kPE_R * tc2];
+                    float *a = af + aLane + tr2 * kNV + tc2 * kACS;
+                    const int ao[4] = {kNV, kNV + kACS, kACS, 0};
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
 #pragma unroll
-                        for (int m = 0; m < 4; ++m) a[offn[m] * kNV + c * NR] += D[4 * c + m];
+                        for (int m = 0; m < 4; ++m) a[ao[m] + c * NR] = __builtin_fmaf(Ee, D[4 * c + m], a[ao[m] + c * NR]);
                 }
-                __syncthreads();
-            }
-            if (el >= Z0) {  // node layer el is complete
-                const float *a = acc[el & 1];
-                const int64_t lay = pl * el;
-#pragma unroll 1
-                for (int i = 0; i < kStPer; ++i) {
-                    const int f = t + i * kBT;
-                    if (f >= kPO_R * kPO_C * kNV) break;
-                    const int node = f / kNV, within = f % kNV;
-                    const int oi = node % kPO_R, oj = node / kPO_R;
-                    const int r = R0 + oi, c = C0 + oj;
-                    if (r < NY && c < NX) {
-                        const int comp = within / NR;
-                        const int pn = (oi + 1) + kPN_R * (oj + 1), sidx = pn * kNV + within;
-                        const uint8_t fl = Fs[el & 1][pn];
-                        const float v = ((fl & kSolid) && !(fl & (1 << comp))) ? a[sidx] : 0.0f;
-                        q[(r + (int64_t)NY * c + lay) * kNV + within] = v;
-                        dotacc += (double)v * (double)sb[sidx];
-                    }
+            // the p values this thread's owned items are dotted with (node layer el)
+            float4 pv[kOPer];
+#pragma unroll
+            for (int i = 0; i < kOPer; ++i)
+                pv[i] = (inRange >> i) & 1 ? *reinterpret_cast<const float4 *>(sb + (oS[i] & 0x1fff))
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();  // element layer el done: node layer el complete, stage slot s0 free
+            if (el >= Z0) {
+                float *a0 = acc + s0 * kASlot;
+                const uint8_t *fl = Fs[s0];
+                float *const qL = qPatch + pl * (int64_t)el * kNV;
+#pragma unroll
+                for (int i = 0; i < kOPer; ++i) {
+                    if (!((inRange >> i) & 1)) continue;
+                    float4 *x0 = reinterpret_cast<float4 *>(a0 + (oA[i] & 0xffff));
+                    float4 *x1 = reinterpret_cast<float4 *>(a0 + (oA[i] >> 16));
+                    float4 *x2 = reinterpret_cast<float4 *>(a0 + (oC[i] & 0xffff));
+                    float4 *x3 = reinterpret_cast<float4 *>(a0 + (oC[i] >> 16));
+                    const float4 v0 = *x0, v1 = *x1, v2 = *x2, v3 = *x3;
+                    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                    *x0 = z;
+                    *x1 = z;
+                    *x2 = z;
+                    *x3 = z;
+                    const uint8_t f8 = fl[(oS[i] >> 13) & 0xff];
+                    const int comp = (oS[i] >> 21) & 3;
+                    const float ok = ((f8 & kSolid) && !(f8 & (1 << comp))) ? 1.0f : 0.0f;
+                    float4 v;
+                    v.x = ok * ((v0.x + v1.x) + (v2.x + v3.x));
+                    v.y = ok * ((v0.y + v1.y) + (v2.y + v3.y));
+                    v.z = ok * ((v0.z + v1.z) + (v2.z + v3.z));
+                    v.w = ok * ((v0.w + v1.w) + (v2.w + v3.w));
+                    *reinterpret_cast<float4 *>(qL + oG[i]) = v;
+                    dacc[0] += (double)v.x * (double)pv[i].x;
+                    dacc[1] += (double)v.y * (double)pv[i].y;
+                    dacc[2] += (double)v.z * (double)pv[i].z;
+                    dacc[3] += (double)v.w * (double)pv[i].w;
                 }
             }
-            __syncthreads();  // stores read the slot; colour passes done with stage[el & 1]
-            for (int i = t; i < kLay; i += kBT) acc[el & 1][i] = 0.0f;
-            put_layer(stage[el & 1], pf);  // node layer el + 2
+            put_layer(stage + s0 * kSL, pf);  // node layer el + 2
             load_layer(el + 3, pf);
-            if (t < kPN) Fs[el & 1][t] = fNext;
-            fNext = load_F(el + 3);
-            if (t < kPE_R * kPE_C) Es[(el + 1) & 1][t] = eNext;  // element layer el + 1
+            if (t < kPN) Fs[s0 ^ 1][t] = fNext;  // node layer el + 1
+            fNext = load_F(el + 2);
+            if (t < kPE_R * kPE_C) Es[s0 ^ 1][t] = eNext;  // element layer el + 1
             eNext = load_E(el + 2);
             __syncthreads();
         }
     }
-    block_sum16(dotacc, partial + (int64_t)blockIdx.x * NR);
+    block_sum16q(dacc, partial + (int64_t)blockIdx.x * NR);
 }
 
 // ------------------------------ vector kernels ------------------------------
@@ -882,8 +981,8 @@ int bfin(sgld_ctx *C, int nb, int op, const double *part = nullptr, const double
 int enqueue_bkbyu(sgld_ctx *C, const float *p, float *q, int *nbOut) {
     BatchState *B = C->batch;
     const Level &l0 = C->lv[0];
-    hipLaunchKernelGGL(k_bkbyu, dim3(B->kGrid), dim3(kBT), 0, C->s, p, q, C->E32.p, l0.nflag.p, l0.g, B->tiles,
-                       B->Kc.p, B->part.p);
+    hipLaunchKernelGGL(k_bkbyu<kPN_R * kNV + 8>, dim3(B->kGrid), dim3(kBT), 0, C->s, p, q, C->E32.p,
+                       l0.nflag.p, l0.g, B->tiles, B->Kc.p, B->part.p);
     SGLD_HIP(hipGetLastError());
     *nbOut = B->kGrid;
     return SGLD_OK;
