@@ -34,14 +34,16 @@ def parse():
                     help="cfg2: ~125 M-DOF ellipsoid union in 512^3 (default); cfg0: 64x32x32 solid cuboid; "
                          "cfg1: 256x128x128 GRF cuboid; "
                          "cfg3: 16 load cases on a 256^3 GRF cuboid, batched mixed vs 16 sequential FP64 "
-                         "solves; cfg4: 32 uneven cuboids in one replayed graph plus sensitivities; "
+                         "solves; cfg4: 32 uneven cuboids solved together (one launch per kernel over all instances, "
+                         "replayed as a graph) plus sensitivities; "
                          "part_r256: the 256x160x112 voxel model data/part_R256.TopVoxel (E=1, 6 levels); "
                          "part_r256_default: the same on the reference's default hierarchy (4 levels, "
                          "nonDyadic: spans 4, 2; 20x32x14-element coarsest level, 16.5 k free DOFs)")
     ap.add_argument("--n3", type=int, default=256, help="cube edge of the batched workload")
     ap.add_argument("--hw-queues", type=int, default=None,
-                    help="hardware queues for the multi-instance workload (sets GPU_MAX_HW_QUEUES, <= 32); "
-                         "default: the runtime's (4, measured fastest: 0.126 s vs 0.187 s with 8, 0.175 s with 16)")
+                    help="hardware queues for the multi-instance workload (sets GPU_MAX_HW_QUEUES, <= 32; "
+                         "matters for the per-instance graphs of SGLD_MULTI_GRAPHS=1, where 4 measured fastest: "
+                         "0.126 s vs 0.187 s with 8, 0.175 s with 16)")
     ap.add_argument("--size", default="256x128x128")
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--cpu-its", type=int, default=3, help="PCG iterations in the CPU sample (1 above 50 M DOF)")
@@ -211,7 +213,9 @@ BF16_PEAK_TFLOPS = 2500.0  # dense
 def run_multi(a):
     """32 uneven cuboid instances (sizes from {32,48,64,96,128} per axis), each
     with its own SIMP density: one step = all instances solved to FP64 true
-    rel-res 1e-8 (mixed precision) in one replayed graph, then the fused
+    rel-res 1e-8 (mixed precision) together - every kernel of the PCG
+    iteration launched once over all instances, reliable updates on the
+    device, batches of iterations replayed as one graph - then the fused
     compliance sensitivities -p x^(p-1) u_e^T Ke u_e of every instance.
     Compared with the same work done instance by instance."""
     # instances replay on their own streams, which the runtime maps onto its
@@ -230,7 +234,15 @@ def run_multi(a):
         s.set_fixed(d["fixed"])
         s.set_modulus(d["E"])
         s.assemble(d["omega"])
-        tune.append(s.autotune(reps=3))
+        t = s.autotune(reps=3)
+        if t["kbyu_variant"] != "tiled":
+            # the autotuner times one launch, where the dense node-centric K.u
+            # wins on small meshes by latency; in the merged multi-instance
+            # launches throughput counts, where the mode-space K.u costs less
+            # per element (32 uneven cuboids, mixed: 85.7 -> 77.0 ms of graphs)
+            s.set_schedule("tiled", 12, t["restriction"])
+            t = dict(t, kbyu_variant="tiled (pinned for the merged launches)", z_chunk=12)
+        tune.append(t)
         S.append(s)
         dF.append(torch.from_numpy(d["F"]).to(dev))
         dU.append(torch.zeros_like(dF[-1]))
@@ -308,8 +320,9 @@ def run_multi(a):
         "data": "synthetic (seeded sizes and Gaussian random density fields)",
         "config": {"workload": ("%d cuboids, edges from {32,48,64,96,128} (seeded), %d DOF in total, "
                                 "cantilever BCs, SIMP p=3 densities, 4-5 levels, mixed precision, tol 1e-8, "
-                                "one replayed graph, then sensitivities" % (len(ins), n_tot)),
-                   "path": "multi-instance graph, mixed", "graph_batch": batch, "instances": len(ins)},
+                                "merged launches over all instances replayed as a graph, then sensitivities"
+                                % (len(ins), n_tot)),
+                   "path": "multi-instance merged launches, mixed", "graph_batch": batch, "instances": len(ins)},
         "multi": {"s_per_step": t_multi, "graph_batch_trials_s": trial, "fp64_s_per_step": t_multi64,
                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default"),
                   "speedup_vs_sequential_mixed": seq["sequential_mixed"]["s_per_step"] / t_multi,

@@ -81,6 +81,19 @@ template <> struct FineW<float> {
          t_ += (int64_t)gridDim.x * blockDim.x)                                                    \
         if (const int64_t i = (S).at(t_); i < (S).n)
 
+// Block index and grid size a kernel body works with: the hardware ones for
+// a launch of one solve; for a launch over many solves (one block range per
+// instance, see the multi-instance iteration) the block's index within its
+// instance's range and that range's length.
+struct VB {
+    unsigned b, g;  // unsigned, as blockIdx.x / gridDim.x
+    __device__ static VB self() { return VB{blockIdx.x, gridDim.x}; }
+};
+#define FOR_SPAN_V(i, S, vb)                                                                      \
+    for (int64_t t_ = (int64_t)(vb).b * blockDim.x + threadIdx.x; t_ < (S).count;               \
+         t_ += (int64_t)(vb).g * blockDim.x)                                                        \
+        if (const int64_t i = (S).at(t_); i < (S).n)
+
 // ----------------------------------------------------------------------------
 // K.u: y = sum_e E_e Ke u_e at every lattice node, gathered node-centrically
 // over the node's 8 incident voxels (no atomics, no colours).  Fixed and void
@@ -88,7 +101,7 @@ template <> struct FineW<float> {
 // u.y for the PCG step length.  Grid-stride over nodes.
 // ----------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__restrict__ y,
+__device__ __forceinline__ void kbyu_body(VB vb, const T *__restrict__ u, T *__restrict__ y,
                                               const T *__restrict__ Ke, const T *__restrict__ E,
                                               const uint8_t *__restrict__ nflag, Grid gd, Span S,
                                               double *__restrict__ partial, const Ctl *ctl) {
@@ -96,7 +109,7 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
     const int64_t NY = gd.NY(), NX = gd.NX();
     const int64_t pl = NY * NX;
     double dotacc = 0.0;
-    FOR_SPAN(g, S) {
+    FOR_SPAN_V(g, S, vb) {
         const int r = (int)(g % NY), c = (int)((g / NY) % NX), l = (int)(g / pl);
         const uint8_t f = nflag[g];
         T acc[3] = {0, 0, 0};
@@ -145,8 +158,15 @@ __global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__rest
     }
     if (partial) {
         const double s = block_sum(dotacc);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) partial[vb.b] = s;
     }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_kbyu(const T *__restrict__ u, T *__restrict__ y,
+                                              const T *__restrict__ Ke, const T *__restrict__ E,
+                                              const uint8_t *__restrict__ nflag, Grid gd, Span S,
+                                              double *__restrict__ partial, const Ctl *ctl) {
+    kbyu_body<T>(VB::self(), u, y, Ke, E, nflag, gd, S, partial, ctl);
 }
 
 
@@ -525,7 +545,7 @@ template <> __device__ __forceinline__ double row_shr1<double>(double v) {
 // t + 256 q of the layer); otherwise every thread stores its node's three
 // values itself.
 template <class T, bool StageOut>
-__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(const T *__restrict__ u, T *__restrict__ y,
+__device__ __forceinline__ void kbyu_owner_body(VB vb, const T *__restrict__ u, T *__restrict__ y,
                                                        const T *__restrict__ E,
                                                        const uint8_t *__restrict__ nflag, Grid gd,
                                                        TileMap tm, const int32_t *__restrict__ wlist,
@@ -543,9 +563,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
     const int64_t ple = (int64_t)gd.ny * gd.nx;  // elements per layer
     const int ei = t % kTE, ej = t / kTE;
     const int64_t W = wlist ? wn : tm.work();
-    const int G = gridDim.x;
+    const int G = vb.g;
     const int nxcd = G < kXCD ? G : kXCD;
-    const int xcd = blockIdx.x % nxcd, slotInXcd = blockIdx.x / nxcd;
+    const int xcd = vb.b % nxcd, slotInXcd = vb.b / nxcd;
     const int blocksOnXcd = G / nxcd + (xcd < G % nxcd ? 1 : 0);
     const int64_t wBeg = W * xcd / nxcd, wEnd = W * (xcd + 1) / nxcd;
     const int f00 = 3 * ((ei + 1) + kTN * ej), f10 = 3 * ((ei + 1) + kTN * (ej + 1));
@@ -704,8 +724,17 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
     }
     if (partial) {
         const double s = block_sum(dotacc);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) partial[vb.b] = s;
     }
+}
+template <class T, bool StageOut>
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(const T *__restrict__ u, T *__restrict__ y,
+                                                       const T *__restrict__ E,
+                                                       const uint8_t *__restrict__ nflag, Grid gd,
+                                                       TileMap tm, const int32_t *__restrict__ wlist,
+                                                       int64_t wn, double *__restrict__ partial,
+                                                       const Ctl *ctl, ModeK<T> km) {
+    kbyu_owner_body<T, StageOut>(VB::self(), u, y, E, nflag, gd, tm, wlist, wn, partial, ctl, km);
 }
 
 // ----------------------------------------------------------------------------
@@ -713,12 +742,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) k_kbyu_owner(cons
 // fine nodes S N + d, |d_i| < S, weight prod (1 - |d_i|/S).
 // ----------------------------------------------------------------------------
 template <class T, int S>
-__global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid gf,
+__device__ __forceinline__ void restrict_body(VB vb, const T *__restrict__ rf, Grid gf,
                                                   T *__restrict__ rc, Grid gc, NodeDiv dc,
                                                   const uint8_t *__restrict__ cflag, int skipVoid,
                                                   const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t G = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t G = (int64_t)vb.b * blockDim.x + threadIdx.x;
     if (G >= gc.nodes()) return;
     if (skipVoid && !(cflag[G] & kSolid)) {  // irregular domains: void coarse nodes read nothing
         rc[3 * G] = rc[3 * G + 1] = rc[3 * G + 2] = T(0);
@@ -757,6 +786,13 @@ __global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid
     rc[3 * G + 1] = solid ? a1 : T(0);
     rc[3 * G + 2] = solid ? a2 : T(0);
 }
+template <class T, int S>
+__global__ void __launch_bounds__(256) k_restrict(const T *__restrict__ rf, Grid gf,
+                                                  T *__restrict__ rc, Grid gc, NodeDiv dc,
+                                                  const uint8_t *__restrict__ cflag, int skipVoid,
+                                                  const Ctl *ctl) {
+    restrict_body<T, S>(VB::self(), rf, gf, rc, gc, dc, cflag, skipVoid, ctl);
+}
 
 // ----------------------------------------------------------------------------
 // Span-2 restriction, LDS-tiled.  A block owns a 32 x 8 patch of coarse node
@@ -773,7 +809,7 @@ constexpr int kRFN = kRFR * kRFC;                    // staged fine nodes per la
 constexpr int kRPer = (kRFN + 255) / 256;            // staged nodes per thread
 
 template <class T>
-__global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ rf, Grid gf,
+__device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict__ rf, Grid gf,
                                                          T *__restrict__ rc, Grid gc,
                                                          const uint8_t *__restrict__ cflag,
                                                          int czc, int tilesR, int tilesC,
@@ -782,7 +818,7 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
     if (stopped(ctl)) return;
     __shared__ T sh[2][kRFN * 3];
     const int t = threadIdx.x;
-    int bid = tiles ? tiles[blockIdx.x] : (int)blockIdx.x;  // listed tiles: those with a solid coarse node
+    int bid = tiles ? tiles[vb.b] : (int)vb.b;  // listed tiles: those with a solid coarse node
     const int tr = bid % tilesR;
     bid /= tilesR;
     const int tc = bid % tilesC;
@@ -881,6 +917,15 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
         }
     }
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ rf, Grid gf,
+                                                         T *__restrict__ rc, Grid gc,
+                                                         const uint8_t *__restrict__ cflag,
+                                                         int czc, int tilesR, int tilesC,
+                                                         const int32_t *__restrict__ tiles,
+                                                         const Ctl *ctl) {
+    restrict2_tiled_body<T>(VB::self(), rf, gf, rc, gc, cflag, czc, tilesR, tilesC, tiles, ctl);
+}
 
 // Component a of (P x_c)(n), span S, at any fine node n (the caller zeroes
 // void nodes).  Branch-free: per axis the two coarse neighbours are q and
@@ -911,7 +956,7 @@ __device__ __forceinline__ T interp_dof(const T *__restrict__ xc, Grid gc, const
 // produced for the next PCG rho.
 // ----------------------------------------------------------------------------
 template <class T, int S>
-__global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid gc,
+__device__ __forceinline__ void prolong_body(VB vb, const T *__restrict__ xc, Grid gc,
                                                  T *__restrict__ xf, NodeDiv df,
                                                  const T *__restrict__ rf,
                                                  const T *__restrict__ dinv,
@@ -920,7 +965,7 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
     if (stopped(ctl)) return;
     const FastDiv d3 = FastDiv::make3();
     double dotacc = 0.0;
-    FOR_SPAN(i, dofs) {  // one DOF per lane: every vector access is contiguous
+    FOR_SPAN_V(i, dofs, vb) {  // one DOF per lane: every vector access is contiguous
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         const T pv = interp_dof<T, S>(xc, gc, df, n, a);
@@ -932,20 +977,29 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
     }
     if (partial) {
         const double s = block_sum(dotacc);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) partial[vb.b] = s;
     }
+}
+template <class T, int S>
+__global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid gc,
+                                                 T *__restrict__ xf, NodeDiv df,
+                                                 const T *__restrict__ rf,
+                                                 const T *__restrict__ dinv,
+                                                 const uint8_t *__restrict__ fflag, int finest, Span dofs,
+                                                 double *__restrict__ partial, const Ctl *ctl) {
+    prolong_body<T, S>(VB::self(), xc, gc, xf, df, rf, dinv, fflag, finest, dofs, partial, ctl);
 }
 
 // The same pass with each accelerator complex die (XCD, one L2 each; block b
 // runs on XCD b mod 8) owning one contiguous eighth of the span, in 64-entry
 // units, so that values gathered from a shared coarser vector are fetched
 // into one L2 rather than into all eight.  Positions of one wavefront still
-// share t >> 6 (what Span::at needs for 64-entry chunks).  Needs gridDim.x to
+// share t >> 6 (what Span::at needs for 64-entry chunks).  Needs the grid to
 // be a multiple of 8.
-#define FOR_SPAN_XCD(i, S)                                                                              \
-    for (int64_t u_ = ((S).count + 63) >> 6, e_ = min<int64_t>((S).count, (u_ * (blockIdx.x % 8 + 1) / 8) << 6), \
-                 t_ = ((u_ * (blockIdx.x % 8) / 8) << 6) + (int64_t)(blockIdx.x / 8) * blockDim.x + threadIdx.x; \
-         t_ < e_; t_ += (int64_t)(gridDim.x / 8) * blockDim.x)                                           \
+#define FOR_SPAN_XCD_V(i, S, vb)                                                                        \
+    for (int64_t u_ = ((S).count + 63) >> 6, e_ = min<int64_t>((S).count, (u_ * ((vb).b % 8 + 1) / 8) << 6), \
+                 t_ = ((u_ * ((vb).b % 8) / 8) << 6) + (int64_t)((vb).b / 8) * blockDim.x + threadIdx.x; \
+         t_ < e_; t_ += (int64_t)((vb).g / 8) * blockDim.x)                                           \
         if (const int64_t i = (S).at(t_); i < (S).n)
 
 // ----------------------------------------------------------------------------
@@ -959,7 +1013,7 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
 // P^T = R, r_2 = R r and r = 0 on fixed DOFs.
 // ----------------------------------------------------------------------------
 template <class T, int S, bool NT>
-__global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, Grid gc, NodeDiv df,
+__device__ __forceinline__ void prolong_dir_body(VB vb, const T *__restrict__ xc, Grid gc, NodeDiv df,
                                                      const T *__restrict__ rf,
                                                      const typename FineW<T>::D *__restrict__ dinv,
                                                      const uint8_t *__restrict__ fflag, T *__restrict__ x,
@@ -983,10 +1037,18 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
         p[i] = z + beta * po;
     };
     if (xsplit) {
-        FOR_SPAN_XCD(i, dofs) body(i);
+        FOR_SPAN_XCD_V(i, dofs, vb) body(i);
     } else {
-        FOR_SPAN(i, dofs) body(i);
+        FOR_SPAN_V(i, dofs, vb) body(i);
     }
+}
+template <class T, int S, bool NT>
+__global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, Grid gc, NodeDiv df,
+                                                     const T *__restrict__ rf,
+                                                     const typename FineW<T>::D *__restrict__ dinv,
+                                                     const uint8_t *__restrict__ fflag, T *__restrict__ x,
+                                                     T *__restrict__ p, Span dofs, int xsplit, const Ctl *ctl) {
+    prolong_dir_body<T, S, NT>(VB::self(), xc, gc, df, rf, dinv, fflag, x, p, dofs, xsplit, ctl);
 }
 
 // The same pass with the coarse values staged in LDS (span 2, used when the
@@ -996,7 +1058,7 @@ __global__ void __launch_bounds__(256) k_prolong_dir(const T *__restrict__ xc, G
 // once (4 contiguous runs of 3 NYc values) instead of being gathered from the
 // vector cache eight times per DOF.  Clamped runs carry weight 0.
 template <class T>
-__global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ xc, Grid gc, Grid gf,
+__device__ __forceinline__ void prolong_dir_cols_body(VB vb, const T *__restrict__ xc, Grid gc, Grid gf,
                                                           const T *__restrict__ rf,
                                                           const typename FineW<T>::D *__restrict__ dinv,
                                                           const uint8_t *__restrict__ fflag, T *__restrict__ x,
@@ -1009,7 +1071,7 @@ __global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ 
     const int NYf = gf.ny + 1, NXf = gf.nx + 1;
     const int NYc = gc.ny + 1, NXc = gc.nx + 1, NZc = gc.nz + 1;
     const int ppl = (NXf + 1) / 2, runLen = 3 * NYc, colLen = 3 * NYf;
-    for (int64_t w = blockIdx.x; w < nItems; w += gridDim.x) {
+    for (int64_t w = vb.b; w < nItems; w += vb.g) {
         const int item = items ? items[w] : (int)w;
         const int l = item / ppl, k = item - l * ppl;
         const int c0 = 2 * k, ncol = (2 * k + 1 < NXf) ? 2 : 1;
@@ -1042,6 +1104,15 @@ __global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ 
             p[i] = z + beta * po;
         }
     }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong_dir_cols(const T *__restrict__ xc, Grid gc, Grid gf,
+                                                          const T *__restrict__ rf,
+                                                          const typename FineW<T>::D *__restrict__ dinv,
+                                                          const uint8_t *__restrict__ fflag, T *__restrict__ x,
+                                                          T *__restrict__ p, const int32_t *__restrict__ items,
+                                                          int64_t nItems, const Ctl *ctl) {
+    prolong_dir_cols_body<T>(VB::self(), xc, gc, gf, rf, dinv, fflag, x, p, items, nItems, ctl);
 }
 
 // Sum over the block returned to every thread: the lane-0 wave sums are
@@ -1085,11 +1156,33 @@ __device__ __forceinline__ void block_sum2(double &a, double &b) {
         }
 }
 
+// ||r||^2 and ||f||^2 of a reliable update from their per-block partials, in
+// one fixed order (a 1024-thread block): out[0], out[1] in thread 0.
+__device__ __forceinline__ void sum2_partials(const double *__restrict__ partRR, const double *__restrict__ partBB,
+                                              int nb, double &rr, double &bb) {
+    rr = 0.0;
+    bb = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        rr += partRR[i];
+        bb += partBB[i];
+    }
+    block_sum2(rr, bb);
+}
+__global__ void __launch_bounds__(1024) k_sum2(const double *__restrict__ partRR, const double *__restrict__ partBB,
+                                               int nb, double *__restrict__ out) {
+    double rr, bb;
+    sum2_partials(partRR, partBB, nb, rr, bb);
+    if (threadIdx.x == 0) {
+        out[0] = rr;
+        out[1] = bb;
+    }
+}
+
 // Solving_PreconditionedConjugateGradientSolver.m:16-17: r -= lambda q with
 // partials of r.r (stopping test) and of sum dinv r^2 (the fine smoother's
 // share of the next rho).
 template <class T, bool NT>
-__global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
+__device__ __forceinline__ void update_r_body(VB vb, T *__restrict__ r, const T *__restrict__ q,
                                                   const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
                                                   const double *__restrict__ partPQ, int nPQ, Ctl *ctl) {
@@ -1102,13 +1195,13 @@ __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__
     for (int i = threadIdx.x; i < nPQ; i += blockDim.x) pq += partPQ[i];
     pq = block_allsum(pq);
     const double lambda = ctl->rho / pq;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (vb.b == 0 && threadIdx.x == 0) {
         ctl->pq = pq;
         ctl->lambda = lambda;
     }
     const T lt = (T)lambda;
     double rr = 0.0, dr = 0.0;
-    FOR_SPAN(i, S) {
+    FOR_SPAN_V(i, S, vb) {
         const T rn = ld_stream<NT>(&r[i]) - lt * ld_stream<NT>(&q[i]);
         r[i] = rn;
         const double rd = (double)rn;
@@ -1117,9 +1210,16 @@ __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__
     }
     block_sum2(rr, dr);
     if (threadIdx.x == 0) {
-        partRR[blockIdx.x] = rr;
-        partDR[blockIdx.x] = dr;
+        partRR[vb.b] = rr;
+        partDR[vb.b] = dr;
     }
+}
+template <class T, bool NT>
+__global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
+                                                  const typename FineW<T>::D *__restrict__ dinv, Span S,
+                                                  double *__restrict__ partRR, double *__restrict__ partDR,
+                                                  const double *__restrict__ partPQ, int nPQ, Ctl *ctl) {
+    update_r_body<T, NT>(VB::self(), r, q, dinv, S, partRR, partDR, partPQ, nPQ, ctl);
 }
 
 // partial of sum dinv r^2 (rho of a residual that k_update_r did not produce)
@@ -1250,10 +1350,10 @@ __host__ __device__ constexpr int tail_lanes(int m) { return m == 0 ? 1 : (m == 
 // r of one node of level t0+1+m (span S = 2^(m+1) from level t0), gathered by
 // P lanes; spans and lane counts are compile-time so the tap decode is cheap.
 template <class T, int S, int P>
-__device__ __forceinline__ void tail_restrict_node(const TailArgs<T> &A, int m, double (*sh)[4]) {
+__device__ __forceinline__ void tail_restrict_node(const TailArgs<T> &A, int m, double (*sh)[4], unsigned vbb) {
     constexpr int Wd = 2 * S - 1, taps = Wd * Wd * Wd;
     const int lane = threadIdx.x % P;
-    const int64_t G = (int64_t)(blockIdx.x - A.blk[m]) * (256 / P) + threadIdx.x / P;
+    const int64_t G = (int64_t)(vbb - A.blk[m]) * (256 / P) + threadIdx.x / P;
     const Grid gc = A.g[m];
     const bool live = G < gc.nodes();
     int R = 0, Cc = 0, Lc = 0;
@@ -1312,14 +1412,18 @@ __device__ __forceinline__ void tail_restrict_node(const TailArgs<T> &A, int m, 
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_tail_restrict(TailArgs<T> A, const Ctl *ctl) {
+__device__ __forceinline__ void tail_restrict_body(VB vb, TailArgs<T> A, const Ctl *ctl) {
     __shared__ double sh[3][4];
     if (stopped(ctl)) return;
     int m = 0;
-    while (m + 1 < A.M && (int)blockIdx.x >= A.blk[m + 1]) ++m;  // uniform per block
-    if (m == 0) tail_restrict_node<T, 2, tail_lanes(0)>(A, 0, sh);
-    else if (m == 1) tail_restrict_node<T, 4, tail_lanes(1)>(A, 1, sh);
-    else tail_restrict_node<T, 8, tail_lanes(2)>(A, 2, sh);
+    while (m + 1 < A.M && (int)vb.b >= A.blk[m + 1]) ++m;  // uniform per block
+    if (m == 0) tail_restrict_node<T, 2, tail_lanes(0)>(A, 0, sh, vb.b);
+    else if (m == 1) tail_restrict_node<T, 4, tail_lanes(1)>(A, 1, sh, vb.b);
+    else tail_restrict_node<T, 8, tail_lanes(2)>(A, 2, sh, vb.b);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_tail_restrict(TailArgs<T> A, const Ctl *ctl) {
+    tail_restrict_body<T>(VB::self(), A, ctl);
 }
 
 // component a of sum_corners w(n) v[G] for the span-S interpolation from
@@ -1339,11 +1443,11 @@ __device__ __forceinline__ T tail_interp(const T *__restrict__ v, const T *__res
 }
 
 template <class T, int M>
-__global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__restrict__ partial, const Ctl *ctl) {
+__device__ __forceinline__ void tail_prolong_body(VB vb, TailArgs<T> A, double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
     const FastDiv d3 = FastDiv::make3();
     double dotacc = 0.0;
-    FOR_SPAN(i, Span::all(3 * A.g0.nodes())) {
+    FOR_SPAN_V(i, Span::all(3 * A.g0.nodes()), vb) {
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         int r, c, l;
@@ -1358,8 +1462,12 @@ __global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__r
     }
     if (partial) {
         const double s = block_sum(dotacc);
-        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) partial[vb.b] = s;
     }
+}
+template <class T, int M>
+__global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__restrict__ partial, const Ctl *ctl) {
+    tail_prolong_body<T, M>(VB::self(), A, partial, ctl);
 }
 
 // Coarsest level (Solving_Vcycle.m:29-31): x(free) = L^-T L^-1 r(free) with
@@ -1376,16 +1484,16 @@ __global__ void __launch_bounds__(256) k_tail_prolong(TailArgs<T> A, double *__r
 // free-DOF gather inside the loop made each step two dependent loads).
 constexpr int64_t kCoarseLds = 4096;
 template <class T, class TW, bool Lds>
-__global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wrow,
+__device__ __forceinline__ void coarse_lower_body(VB vb, const TW *__restrict__ Wrow,
                                                       const int32_t *__restrict__ freeIdx, int64_t n,
                                                       const T *__restrict__ r, double *__restrict__ y,
                                                       const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t row = (int64_t)vb.b * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     __shared__ double rs[Lds ? kCoarseLds : 1];
     if constexpr (Lds) {
-        const int64_t jEnd = min<int64_t>(n, (int64_t)(blockIdx.x + 1) * (blockDim.x >> 6));
+        const int64_t jEnd = min<int64_t>(n, (int64_t)(vb.b + 1) * (blockDim.x >> 6));
         for (int64_t j = threadIdx.x; j < jEnd; j += blockDim.x) rs[j] = (double)r[freeIdx[j]];
         __syncthreads();
     }
@@ -1397,15 +1505,22 @@ __global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wro
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) y[row] = acc;
 }
+template <class T, class TW, bool Lds>
+__global__ void __launch_bounds__(256) k_coarse_lower(const TW *__restrict__ Wrow,
+                                                      const int32_t *__restrict__ freeIdx, int64_t n,
+                                                      const T *__restrict__ r, double *__restrict__ y,
+                                                      const Ctl *ctl) {
+    coarse_lower_body<T, TW, Lds>(VB::self(), Wrow, freeIdx, n, r, y, ctl);
+}
 // x(free) = W^T y: row j of W^T (column j of W) holds entries j..n-1.  Fixed
 // and void DOFs of x are never written (zero since the assembly).
 template <class T, class TW>
-__global__ void __launch_bounds__(256) k_coarse_upper(const TW *__restrict__ Wcol,
+__device__ __forceinline__ void coarse_upper_body(VB vb, const TW *__restrict__ Wcol,
                                                       const int32_t *__restrict__ freeIdx, int64_t n,
                                                       const double *__restrict__ y, T *__restrict__ x,
                                                       const Ctl *ctl) {
     if (stopped(ctl)) return;
-    const int64_t j = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t j = (int64_t)vb.b * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (j >= n) return;
     const TW *w = Wcol + tri_col_off(j, n);
@@ -1414,16 +1529,28 @@ __global__ void __launch_bounds__(256) k_coarse_upper(const TW *__restrict__ Wco
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) x[freeIdx[j]] = (T)acc;
 }
+template <class T, class TW>
+__global__ void __launch_bounds__(256) k_coarse_upper(const TW *__restrict__ Wcol,
+                                                      const int32_t *__restrict__ freeIdx, int64_t n,
+                                                      const double *__restrict__ y, T *__restrict__ x,
+                                                      const Ctl *ctl) {
+    coarse_upper_body<T, TW>(VB::self(), Wcol, freeIdx, n, y, x, ctl);
+}
 
 // ----------------------------- PCG vector kernels ---------------------------
 template <class T>
-__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b, Span S,
+__device__ __forceinline__ void dot_body(VB vb, const T *__restrict__ a, const T *__restrict__ b, Span S,
                                              double *__restrict__ partial, const Ctl *ctl) {
     if (stopped(ctl)) return;
     double acc = 0.0;
-    FOR_SPAN(i, S) acc += (double)a[i] * (double)b[i];
+    FOR_SPAN_V(i, S, vb) acc += (double)a[i] * (double)b[i];
     const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[vb.b] = s;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b, Span S,
+                                             double *__restrict__ partial, const Ctl *ctl) {
+    dot_body<T>(VB::self(), a, b, S, partial, ctl);
 }
 
 // r = b - q (q may be null: r = b).
@@ -1482,9 +1609,8 @@ enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_RR, FIN_RHO };  // (lambda = rho / p.q
 
 // Second reduction stage (fixed order, deterministic) and the scalar updates
 // of the PCG recurrence.
-__global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial, int nb,
-                                              const double *__restrict__ partial2, int nb2, int op,
-                                              Ctl *ctl) {
+__device__ __forceinline__ void fin_body(const double *__restrict__ partial, int nb,
+                                         const double *__restrict__ partial2, int nb2, int op, Ctl *ctl) {
     if (op != FIN_NORMB && op != FIN_RHO0 && ctl->done) return;
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
@@ -1509,6 +1635,393 @@ __global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial
             ctl->rho = s;
             break;
     }
+}
+__global__ void __launch_bounds__(1024) k_fin(const double *__restrict__ partial, int nb,
+                                              const double *__restrict__ partial2, int nb2, int op,
+                                              Ctl *ctl) {
+    fin_body(partial, nb, partial2, nb2, op, ctl);
+}
+
+// ------------------------- multi-instance launches --------------------------
+// One launch of an iteration kernel over many independent solves: instance i
+// runs virtual blocks [start[i], start[i + 1]) of the grid with its own
+// arguments, as its own launch of start[i + 1] - start[i] blocks would (same
+// block indices, so per-block partial sums and their order are the same).  An
+// instance whose solve has stopped returns at once, as in its own launch.
+template <class A>
+struct MTab {
+    const A *args;
+    const int *start;  // n + 1 entries
+    int n;
+};
+__device__ __forceinline__ int mt_inst(const int *start, int n) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;  // the last i with start[i] <= b (uniform per block)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Kernel descriptors: the argument block A, the body run(vb, a), the launch
+// of the single-solve kernel with the same arguments, and the launch bounds
+// (kOcc 0: none beyond the block size).
+template <class T> struct DirA {
+    const T *xc; Grid gc; NodeDiv df; const T *rf; const typename FineW<T>::D *dinv; const uint8_t *fflag;
+    T *x; T *p; Span dofs; int xsplit; const Ctl *ctl;
+};
+template <class T, int S, bool NT> struct OpDir {
+    using A = DirA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        prolong_dir_body<T, S, NT>(vb, a.xc, a.gc, a.df, a.rf, a.dinv, a.fflag, a.x, a.p, a.dofs, a.xsplit, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_prolong_dir<T, S, NT>), dim3(grid), dim3(256), shm, s, a.xc, a.gc, a.df, a.rf, a.dinv,
+                           a.fflag, a.x, a.p, a.dofs, a.xsplit, a.ctl);
+    }
+};
+template <class T> struct DirColsA {
+    const T *xc; Grid gc, gf; const T *rf; const typename FineW<T>::D *dinv; const uint8_t *fflag; T *x; T *p;
+    const int32_t *items; int64_t nItems; const Ctl *ctl;
+};
+template <class T> struct OpDirCols {
+    using A = DirColsA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        prolong_dir_cols_body<T>(vb, a.xc, a.gc, a.gf, a.rf, a.dinv, a.fflag, a.x, a.p, a.items, a.nItems, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_prolong_dir_cols<T>, dim3(grid), dim3(256), shm, s, a.xc, a.gc, a.gf, a.rf, a.dinv,
+                           a.fflag, a.x, a.p, a.items, a.nItems, a.ctl);
+    }
+};
+template <class T> struct KuOwnerA {
+    const T *u; T *y; const T *E; const uint8_t *nflag; Grid gd; TileMap tm; const int32_t *wlist; int64_t wn;
+    double *partial; const Ctl *ctl; ModeK<T> km;
+};
+template <class T, bool StageOut> struct OpKuOwner {
+    using A = KuOwnerA<T>;
+    static constexpr int kThreads = 256, kOcc = sizeof(T) == 8 ? 3 : 4;
+    __device__ static void run(VB vb, const A &a) {
+        kbyu_owner_body<T, StageOut>(vb, a.u, a.y, a.E, a.nflag, a.gd, a.tm, a.wlist, a.wn, a.partial, a.ctl, a.km);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_kbyu_owner<T, StageOut>), dim3(grid), dim3(256), shm, s, a.u, a.y, a.E, a.nflag, a.gd,
+                           a.tm, a.wlist, a.wn, a.partial, a.ctl, a.km);
+    }
+};
+template <class T> struct KuDenseA {
+    const T *u; T *y; const T *Ke; const T *E; const uint8_t *nflag; Grid gd; Span S; double *partial;
+    const Ctl *ctl;
+};
+template <class T> struct OpKuDense {
+    using A = KuDenseA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        kbyu_body<T>(vb, a.u, a.y, a.Ke, a.E, a.nflag, a.gd, a.S, a.partial, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_kbyu<T>, dim3(grid), dim3(256), shm, s, a.u, a.y, a.Ke, a.E, a.nflag, a.gd, a.S,
+                           a.partial, a.ctl);
+    }
+};
+template <class T> struct UpdA {
+    T *r; const T *q; const typename FineW<T>::D *dinv; Span S; double *partRR; double *partDR;
+    const double *partPQ; int nPQ; Ctl *ctl;
+};
+template <class T, bool NT> struct OpUpd {
+    using A = UpdA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        update_r_body<T, NT>(vb, a.r, a.q, a.dinv, a.S, a.partRR, a.partDR, a.partPQ, a.nPQ, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_update_r<T, NT>), dim3(grid), dim3(256), shm, s, a.r, a.q, a.dinv, a.S, a.partRR,
+                           a.partDR, a.partPQ, a.nPQ, a.ctl);
+    }
+};
+struct FinA {
+    const double *partial; int nb; const double *partial2; int nb2; int op; Ctl *ctl;
+};
+struct OpFin {
+    using A = FinA;
+    static constexpr int kThreads = 1024, kOcc = 0;
+    __device__ static void run(VB, const A &a) { fin_body(a.partial, a.nb, a.partial2, a.nb2, a.op, a.ctl); }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_fin, dim3(grid), dim3(1024), shm, s, a.partial, a.nb, a.partial2, a.nb2, a.op, a.ctl);
+    }
+};
+template <class T> struct ResA {
+    const T *rf; Grid gf; T *rc; Grid gc; NodeDiv dc; const uint8_t *cflag; int skipVoid; const Ctl *ctl;
+};
+template <class T, int S> struct OpRes {
+    using A = ResA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        restrict_body<T, S>(vb, a.rf, a.gf, a.rc, a.gc, a.dc, a.cflag, a.skipVoid, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_restrict<T, S>), dim3(grid), dim3(256), shm, s, a.rf, a.gf, a.rc, a.gc, a.dc, a.cflag,
+                           a.skipVoid, a.ctl);
+    }
+};
+template <class T> struct Res2A {
+    const T *rf; Grid gf; T *rc; Grid gc; const uint8_t *cflag; int czc, tilesR, tilesC; const int32_t *tiles;
+    const Ctl *ctl;
+};
+template <class T> struct OpRes2 {
+    using A = Res2A<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        restrict2_tiled_body<T>(vb, a.rf, a.gf, a.rc, a.gc, a.cflag, a.czc, a.tilesR, a.tilesC, a.tiles, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(grid), dim3(256), shm, s, a.rf, a.gf, a.rc, a.gc, a.cflag,
+                           a.czc, a.tilesR, a.tilesC, a.tiles, a.ctl);
+    }
+};
+template <class T> struct ProA {
+    const T *xc; Grid gc; T *xf; NodeDiv df; const T *rf; const T *dinv; const uint8_t *fflag; int finest;
+    Span dofs; double *partial; const Ctl *ctl;
+};
+template <class T, int S> struct OpPro {
+    using A = ProA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        prolong_body<T, S>(vb, a.xc, a.gc, a.xf, a.df, a.rf, a.dinv, a.fflag, a.finest, a.dofs, a.partial, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_prolong<T, S>), dim3(grid), dim3(256), shm, s, a.xc, a.gc, a.xf, a.df, a.rf, a.dinv,
+                           a.fflag, a.finest, a.dofs, a.partial, a.ctl);
+    }
+};
+template <class T> struct TailRA {
+    TailArgs<T> A; const Ctl *ctl;
+};
+template <class T> struct OpTailR {
+    using A = TailRA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) { tail_restrict_body<T>(vb, a.A, a.ctl); }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_tail_restrict<T>, dim3(grid), dim3(256), shm, s, a.A, a.ctl);
+    }
+};
+template <class T> struct TailPA {
+    TailArgs<T> A; double *partial; const Ctl *ctl;
+};
+template <class T, int M> struct OpTailP {
+    using A = TailPA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) { tail_prolong_body<T, M>(vb, a.A, a.partial, a.ctl); }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_tail_prolong<T, M>), dim3(grid), dim3(256), shm, s, a.A, a.partial, a.ctl);
+    }
+};
+template <class T, class TW> struct CLoA {
+    const TW *Wrow; const int32_t *freeIdx; int64_t n; const T *r; double *y; const Ctl *ctl;
+};
+template <class T, class TW, bool Lds> struct OpCLo {
+    using A = CLoA<T, TW>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        coarse_lower_body<T, TW, Lds>(vb, a.Wrow, a.freeIdx, a.n, a.r, a.y, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_coarse_lower<T, TW, Lds>), dim3(grid), dim3(256), shm, s, a.Wrow, a.freeIdx, a.n, a.r,
+                           a.y, a.ctl);
+    }
+};
+template <class T, class TW> struct CUpA {
+    const TW *Wcol; const int32_t *freeIdx; int64_t n; const double *y; T *x; const Ctl *ctl;
+};
+template <class T, class TW> struct OpCUp {
+    using A = CUpA<T, TW>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        coarse_upper_body<T, TW>(vb, a.Wcol, a.freeIdx, a.n, a.y, a.x, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL((k_coarse_upper<T, TW>), dim3(grid), dim3(256), shm, s, a.Wcol, a.freeIdx, a.n, a.y, a.x,
+                           a.ctl);
+    }
+};
+template <class T> struct DotA {
+    const T *a; const T *b; Span S; double *partial; const Ctl *ctl;
+};
+template <class T> struct OpDot {
+    using A = DotA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) { dot_body<T>(vb, a.a, a.b, a.S, a.partial, a.ctl); }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_dot<T>, dim3(grid), dim3(256), shm, s, a.a, a.b, a.S, a.partial, a.ctl);
+    }
+};
+
+template <class Op>
+__global__ void __launch_bounds__(Op::kThreads) k_multi(MTab<typename Op::A> t) {
+    const int i = mt_inst(t.start, t.n);
+    const VB vb{(unsigned)((int)blockIdx.x - t.start[i]), (unsigned)(t.start[i + 1] - t.start[i])};
+    Op::run(vb, t.args[i]);
+}
+template <class Op>
+__global__ void __launch_bounds__(Op::kThreads, Op::kOcc > 0 ? Op::kOcc : 1) k_multi_occ(MTab<typename Op::A> t) {
+    const int i = mt_inst(t.start, t.n);
+    const VB vb{(unsigned)((int)blockIdx.x - t.start[i]), (unsigned)(t.start[i + 1] - t.start[i])};
+    Op::run(vb, t.args[i]);
+}
+template <class Op>
+void multi_launch(const void *args, const int *start, int n, int grid, size_t shm, hipStream_t s) {
+    const MTab<typename Op::A> t{(const typename Op::A *)args, start, n};
+    if constexpr (Op::kOcc > 0) hipLaunchKernelGGL(k_multi_occ<Op>, dim3(grid), dim3(Op::kThreads), shm, s, t);
+    else hipLaunchKernelGGL(k_multi<Op>, dim3(grid), dim3(Op::kThreads), shm, s, t);
+}
+
+// ---- reliable updates of the mixed solve on the device (merged solves) ----
+// Per instance: the solve's fixed numbers and update state (MixCtl), and a
+// gate block whose `done` is 0 only while an update runs, so that the
+// update's kernels (the same V-cycle kernels, gated by it) do nothing for
+// instances that are iterating.  The steps and decisions are those of the
+// host path (mixed_update_begin / mixed_update_end).
+struct MixCtl {
+    double scale, normbS, delta, tolS, res;
+    int itsAtUpdate, maxStretch, updates, finished, keep, doneCode;
+};
+struct MixCheckA {
+    Ctl *ctl; Ctl *gate; MixCtl *mix;
+};
+struct OpMixCheck {
+    using A = MixCheckA;
+    static constexpr int kThreads = 64, kOcc = 0;
+    __device__ static void run(VB, const A &a) {
+        if (threadIdx.x != 0) return;
+        const bool need = !a.mix->finished &&
+                          (a.ctl->done != 0 || a.ctl->its - a.mix->itsAtUpdate >= a.mix->maxStretch);
+        a.gate->done = need ? 0 : 1;
+        if (need) a.mix->itsAtUpdate = a.ctl->its;
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}  // merged solves only
+};
+struct MixFoldA {
+    double *x; float *x32; const float *p32; Span S; const Ctl *ctl; const Ctl *gate; const MixCtl *mix;
+};
+struct OpMixFold {
+    using A = MixFoldA;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        if (a.gate->done) return;
+        const float lam = (float)a.ctl->lambda;
+        const double scale = a.mix->scale;
+        FOR_SPAN_V(i, a.S, vb) {
+            a.x[i] += scale * (double)(a.x32[i] + lam * a.p32[i]);
+            a.x32[i] = 0.0f;
+        }
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}
+};
+struct MixNormsA {
+    const double *b; const double *q; float *r32; const uint8_t *nflag; Span S; double *partRR; double *partBB;
+    const Ctl *gate; const MixCtl *mix;
+};
+struct OpMixNorms {
+    using A = MixNormsA;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        if (a.gate->done) return;
+        const double invScale = 1.0 / a.mix->scale;
+        double rr = 0.0, bb = 0.0;
+        FOR_SPAN_V(i, a.S, vb) {
+            const uint32_t node = (uint32_t)(i / 3), c = (uint32_t)(i - 3 * (int64_t)node);
+            const double bi = a.b[i];
+            const double ri = (a.nflag[node] & (1u << c)) ? 0.0 : bi - a.q[i];
+            a.r32[i] = (float)(ri * invScale);
+            rr += ri * ri;
+            bb += bi * bi;
+        }
+        block_sum2(rr, bb);
+        if (threadIdx.x == 0) {
+            a.partRR[vb.b] = rr;
+            a.partBB[vb.b] = bb;
+        }
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}
+};
+struct MixDecideA {
+    const double *partRR; const double *partBB; int nb; Ctl *ctl; Ctl *gate; MixCtl *mix;
+};
+struct OpMixDecide {
+    using A = MixDecideA;
+    static constexpr int kThreads = 1024, kOcc = 0;
+    __device__ static void run(VB, const A &a) {
+        if (a.gate->done) return;
+        double rr, bb;
+        sum2_partials(a.partRR, a.partBB, a.nb, rr, bb);  // as k_sum2 on the single-solve path
+        if (threadIdx.x != 0) return;
+        Ctl &c = *a.ctl;
+        MixCtl &m = *a.mix;
+        const double normr = sqrt(rr), res = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+        m.updates += 1;
+        m.res = res;
+        if (res < m.tolS || c.done == 2 || !(normr > 0.0)) {
+            m.finished = 1;
+            m.doneCode = res < m.tolS ? 1 : 2;
+            c.done = 1;
+            a.gate->done = 1;  // no rho: the solve has stopped
+            return;
+        }
+        // the direction's rho, as mixed_update_end
+        double rhoDir = c.rho;
+        int keep = 1;
+        if (!c.done) {
+            keep = c.beta > 0.0;
+            if (keep) rhoDir = c.rho / c.beta;
+        }
+        m.keep = keep;
+        c.done = 0;
+        c.rho = rhoDir;
+        c.lambda = 0.0;
+        c.normb = normr / m.scale;
+        c.tol = fmax(m.delta, 0.5 * m.tolS * m.normbS / normr);
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}
+};
+struct MixDinvRRA {
+    const float *r; const uint16_t *dinv; Span S; double *partial; const Ctl *gate;
+};
+struct OpMixDinvRR {
+    using A = MixDinvRRA;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        if (a.gate->done) return;
+        double acc = 0.0;
+        FOR_SPAN_V(i, a.S, vb) {
+            const double rd = (double)a.r[i];
+            acc += (double)FineW<float>::get(a.dinv, i) * rd * rd;
+        }
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) a.partial[vb.b] = s;
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}
+};
+struct MixFinRhoA {
+    const double *partial; int nb; const double *partial2; int nb2; Ctl *ctl; const Ctl *gate; const MixCtl *mix;
+};
+struct OpMixFinRho {
+    using A = MixFinRhoA;
+    static constexpr int kThreads = 1024, kOcc = 0;
+    __device__ static void run(VB, const A &a) {
+        if (a.gate->done) return;
+        fin_body(a.partial, a.nb, a.partial2, a.nb2, a.mix->keep ? FIN_RHO : FIN_RHO0, a.ctl);
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}
+};
+
+// Copies each instance's control block into one array (one host copy per batch).
+__global__ void k_gather_ctl(Ctl *const *ctls, int n, Ctl *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *ctls[i];
 }
 
 // ------------------------------- setup kernels ------------------------------
@@ -1994,6 +2507,32 @@ using namespace sgld;
 
 namespace {
 
+// Positions of the launches in one PCG iteration (enqueue_iteration): every
+// instance's launches appear in increasing order, so the multi-instance
+// iteration can merge the instances' launches step by step.
+enum Step {
+    ST_DIR = 100, ST_KU = 200, ST_UPD = 300, ST_FIN_RR = 400,
+    ST_RES = 500,     // + level
+    ST_TAILR = 600, ST_CLO = 700, ST_CUP = 710, ST_TAILP = 800,
+    ST_PRO = 1000,    // - level
+    ST_DOT = 1100, ST_FIN_RHO = 1200, ST_OTHER = 1300
+};
+
+// Launch kernel Op with argument block a on the context's stream, or, while
+// the context records its iteration, append it to the recording.
+template <class Op>
+int emit(sgld_ctx *C, int step, int grid, size_t shm, const typename Op::A &a) {
+    if (!C->rec) {
+        Op::single(a, grid, shm, C->s);
+        SGLD_HIP(hipGetLastError());
+        return SGLD_OK;
+    }
+    RecOp o{C->rec->base + step, &multi_launch<Op>, grid, shm, std::vector<char>(sizeof(a))};
+    std::memcpy(o.args.data(), &a, sizeof(a));
+    C->rec->ops.push_back(std::move(o));
+    return SGLD_OK;
+}
+
 template <class T> T *lvl_r(Level &l);
 template <> double *lvl_r<double>(Level &l) { return l.r.p; }
 template <> float *lvl_r<float>(Level &l) { return l.r32.p; }
@@ -2108,12 +2647,16 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
         const TiledSchedule &ts = C->ksched[q];
         nb = ts.grid;
         if (C->kbyuOwner) {
-            auto kern = C->kbyuStageOut[q] ? k_kbyu_owner<T, true> : k_kbyu_owner<T, false>;
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C), l0.nflag.p, l0.g, ts.tm,
-                               C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial, ctl,
-                               ctx_km<T>(C));
+            const KuOwnerA<T> a{u, y, ctx_E<T>(C), l0.nflag.p, l0.g, ts.tm,
+                                C->full ? nullptr : (const int32_t *)C->kwork[q].p, C->nKwork[q], partial, ctl,
+                                ctx_km<T>(C)};
+            SGLD_TRY(C->kbyuStageOut[q] ? (emit<OpKuOwner<T, true>>(C, ST_KU, nb, 0, a))
+                                        : (emit<OpKuOwner<T, false>>(C, ST_KU, nb, 0, a)));
         }
-        else if (C->kbyuOneBar)
+        else if (C->rec) {
+            set_error("sgld:arg: the multi-instance iteration needs the owner-computes or the dense K.u");
+            return SGLD_ERR_ARG;
+        } else if (C->kbyuOneBar)
             hipLaunchKernelGGL((k_kbyu_tiled<T, true>), dim3(nb), dim3(256), 0, C->s, u, y, ctx_E<T>(C),
                                l0.nflag.p, l0.g, ts.tm, C->full ? nullptr : (const int32_t *)C->kwork[q].p,
                                C->nKwork[q], partial, ctl, ctx_km<T>(C));
@@ -2124,8 +2667,8 @@ int enqueue_kbyu(sgld_ctx *C, const T *u, T *y, double *partial, const Ctl *ctl,
     } else {
         const Span S = node_span(C);
         nb = grid_blocks(S.count);
-        hipLaunchKernelGGL(k_kbyu<T>, dim3(nb), dim3(256), 0, C->s, u, y, ctx_Ke<T>(C), ctx_E<T>(C), l0.nflag.p,
-                           l0.g, S, partial, ctl);
+        SGLD_TRY(emit<OpKuDense<T>>(C, ST_KU, nb, 0,
+                                    KuDenseA<T>{u, y, ctx_Ke<T>(C), ctx_E<T>(C), l0.nflag.p, l0.g, S, partial, ctl}));
     }
     SGLD_HIP(hipGetLastError());
     SGLD_TRY(kt_end(C));
@@ -2216,16 +2759,13 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
         const bool list = !C->full && Cc.rtilesCz == czc;
         const int nbt = list ? (int)Cc.nRtiles : tilesR * tilesC * tilesZ;
         if (nbt > 0)
-            hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(nbt), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, Cc.nflag.p,
-                               czc, tilesR, tilesC, list ? (const int32_t *)Cc.rtiles.p : nullptr, ctl);
-    } else if (Cc.span == 2)
-        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
-                           Cc.nflag.p, (int)!C->full, ctl);
-    else
-        hipLaunchKernelGGL((k_restrict<T, 4>), dim3(nb), dim3(256), 0, C->s, rf, F.g, rc, Cc.g, dc,
-                           Cc.nflag.p, (int)!C->full, ctl);
-    SGLD_HIP(hipGetLastError());
-    return SGLD_OK;
+            SGLD_TRY(emit<OpRes2<T>>(C, ST_RES + k, nbt, 0,
+                                     Res2A<T>{rf, F.g, rc, Cc.g, Cc.nflag.p, czc, tilesR, tilesC,
+                                              list ? (const int32_t *)Cc.rtiles.p : nullptr, ctl}));
+        return SGLD_OK;
+    }
+    const ResA<T> a{rf, F.g, rc, Cc.g, dc, Cc.nflag.p, (int)!C->full, ctl};
+    return Cc.span == 2 ? emit<OpRes<T, 2>>(C, ST_RES + k, nb, 0, a) : emit<OpRes<T, 4>>(C, ST_RES + k, nb, 0, a);
 }
 
 template <class T>
@@ -2241,13 +2781,8 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
                                                                       C->chunkLog));
     const int nb = grid_blocks(S.count);
     const NodeDiv df = NodeDiv::make(F.g);
-    if (Cc.span == 2)
-        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, df, rf,
-                           lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
-    else
-        hipLaunchKernelGGL((k_prolong<T, 4>), dim3(nb), dim3(256), 0, C->s, xc, Cc.g, xf, df, rf,
-                           lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl);
-    SGLD_HIP(hipGetLastError());
+    const ProA<T> a{xc, Cc.g, xf, df, rf, lvl_dinv<T>(F), F.nflag.p, finest, S, partial, ctl};
+    SGLD_TRY(Cc.span == 2 ? (emit<OpPro<T, 2>>(C, ST_PRO - k, nb, 0, a)) : (emit<OpPro<T, 4>>(C, ST_PRO - k, nb, 0, a)));
     if (nbOut) *nbOut = nb;
     return SGLD_OK;
 }
@@ -2264,9 +2799,8 @@ int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ct
 }
 
 int fin(sgld_ctx *C, const double *partial, int nb, int op, const double *partial2 = nullptr, int nb2 = 0) {
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, C->s, partial, nb, partial2, nb2, op, C->ctl.p);
-    SGLD_HIP(hipGetLastError());
-    return SGLD_OK;
+    return emit<OpFin>(C, op == FIN_RR ? ST_FIN_RR : op == FIN_RHO ? ST_FIN_RHO : ST_OTHER, 1, 0,
+                       FinA{partial, nb, partial2, nb2, op, C->ctl.p});
 }
 
 // Top t0 of the fused small-level tail (k_tail_restrict / k_tail_prolong),
@@ -2328,31 +2862,33 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
     TailArgs<T> A{};
     if (t0 > 0) {
         A = tail_args<T>(C, t0);
-        hipLaunchKernelGGL(k_tail_restrict<T>, dim3(A.blk[A.M]), dim3(256), 0, C->s, A, ctl);
-        SGLD_HIP(hipGetLastError());
+        SGLD_TRY(emit<OpTailR<T>>(C, ST_TAILR, A.blk[A.M], 0, TailRA<T>{A, ctl}));
     }
     Level &Lc = C->lv[L - 1];
     const int64_t n = C->nCoarseFree;
+    const int nbc = blocks_for(n, 4);
     if (sizeof(T) == 4 && C->cWrow32.p) {
-        hipLaunchKernelGGL((k_coarse_lower<T, float, false>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
-                           (const float *)C->cWrow32.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
-        hipLaunchKernelGGL((k_coarse_upper<T, float>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
-                           (const float *)C->cWcol32.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
+        SGLD_TRY((emit<OpCLo<T, float, false>>(C, ST_CLO, nbc, 0,
+                                               CLoA<T, float>{(const float *)C->cWrow32.p, C->cFree.p, n,
+                                                              (const T *)lvl_r<T>(Lc), C->cY.p, ctl})));
+        SGLD_TRY((emit<OpCUp<T, float>>(C, ST_CUP, nbc, 0,
+                                        CUpA<T, float>{(const float *)C->cWcol32.p, C->cFree.p, n,
+                                                       (const double *)C->cY.p, lvl_x<T>(Lc), ctl})));
     } else {
-        auto lower = n <= kCoarseLds ? k_coarse_lower<T, double, true> : k_coarse_lower<T, double, false>;
-        hipLaunchKernelGGL(lower, dim3(blocks_for(n, 4)), dim3(256), 0, C->s, (const double *)C->cWrow.p,
-                           C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl);
-        hipLaunchKernelGGL((k_coarse_upper<T, double>), dim3(blocks_for(n, 4)), dim3(256), 0, C->s,
-                           (const double *)C->cWcol.p, C->cFree.p, n, (const double *)C->cY.p, lvl_x<T>(Lc), ctl);
+        const CLoA<T, double> lo{(const double *)C->cWrow.p, C->cFree.p, n, (const T *)lvl_r<T>(Lc), C->cY.p, ctl};
+        SGLD_TRY(n <= kCoarseLds ? (emit<OpCLo<T, double, true>>(C, ST_CLO, nbc, 0, lo))
+                                 : (emit<OpCLo<T, double, false>>(C, ST_CLO, nbc, 0, lo)));
+        SGLD_TRY((emit<OpCUp<T, double>>(C, ST_CUP, nbc, 0,
+                                         CUpA<T, double>{(const double *)C->cWcol.p, C->cFree.p, n,
+                                                         (const double *)C->cY.p, lvl_x<T>(Lc), ctl})));
     }
-    SGLD_HIP(hipGetLastError());
     if (t0 > 0) {
         const int nb = grid_blocks(3 * A.g0.nodes());
         double *part = t0 == 1 ? partialX2 : nullptr;
-        if (A.M == 1) hipLaunchKernelGGL((k_tail_prolong<T, 1>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
-        else if (A.M == 2) hipLaunchKernelGGL((k_tail_prolong<T, 2>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
-        else hipLaunchKernelGGL((k_tail_prolong<T, 3>), dim3(nb), dim3(256), 0, C->s, A, part, ctl);
-        SGLD_HIP(hipGetLastError());
+        const TailPA<T> a{A, part, ctl};
+        if (A.M == 1) SGLD_TRY((emit<OpTailP<T, 1>>(C, ST_TAILP, nb, 0, a)));
+        else if (A.M == 2) SGLD_TRY((emit<OpTailP<T, 2>>(C, ST_TAILP, nb, 0, a)));
+        else SGLD_TRY((emit<OpTailP<T, 3>>(C, ST_TAILP, nb, 0, a)));
         if (part && nbX2) *nbX2 = nb;
     }
     for (int k = kLast; k >= 2; --k)
@@ -2361,9 +2897,8 @@ int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2,
     if (L == 2 && partialX2) {  // level 2 is the coarsest: x_2 . r_2 by its own pass
         const Span S2 = Span::all(3 * C->lv[1].g.nodes());
         *nbX2 = grid_blocks(S2.count);
-        hipLaunchKernelGGL(k_dot<T>, dim3(*nbX2), dim3(256), 0, C->s, (const T *)lvl_x<T>(Lc),
-                           (const T *)lvl_r<T>(Lc), S2, partialX2, ctl);
-        SGLD_HIP(hipGetLastError());
+        SGLD_TRY(emit<OpDot<T>>(C, ST_DOT, *nbX2, 0,
+                                DotA<T>{(const T *)lvl_x<T>(Lc), (const T *)lvl_r<T>(Lc), S2, partialX2, ctl}));
     }
     return SGLD_OK;
 }
@@ -2412,26 +2947,25 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     if (cols) {
         const int64_t items = ((l0.g.NX() + 1) / 2) * l0.g.NZ();
         const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(items, 65536));
-        hipLaunchKernelGGL(k_prolong_dir_cols<T>, dim3(nb), dim3(256), colBytes, C->s, (const T *)lvl_x<T>(l1),
-                           l1.g, l0.g, (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, (const int32_t *)nullptr,
-                           items, (const Ctl *)ctl);
+        SGLD_TRY(emit<OpDirCols<T>>(C, ST_DIR, nb, colBytes,
+                                    DirColsA<T>{(const T *)lvl_x<T>(l1), l1.g, l0.g, (const T *)r, fine_dinv<T>(C),
+                                                l0.nflag.p, x, p, (const int32_t *)nullptr, items,
+                                                (const Ctl *)ctl}));
     } else {
-        const T *xc = lvl_x<T>(l1);
-        const auto *dv = fine_dinv<T>(C);
-        auto kern = l1.span == 2 ? (nt ? k_prolong_dir<T, 2, true> : k_prolong_dir<T, 2, false>)
-                                 : (nt ? k_prolong_dir<T, 4, true> : k_prolong_dir<T, 4, false>);
-        hipLaunchKernelGGL(kern, dim3(C->dirXcd ? nbx : nbv), dim3(256), 0, C->s, xc, l1.g, d0, (const T *)r, dv,
-                           l0.nflag.p, x, p, S, C->dirXcd ? 1 : 0, (const Ctl *)ctl);
+        const DirA<T> a{lvl_x<T>(l1), l1.g, d0, (const T *)r, fine_dinv<T>(C), l0.nflag.p, x, p, S,
+                        C->dirXcd ? 1 : 0, (const Ctl *)ctl};
+        const int nb = C->dirXcd ? nbx : nbv;
+        if (l1.span == 2) SGLD_TRY(nt ? (emit<OpDir<T, 2, true>>(C, ST_DIR, nb, 0, a))
+                                      : (emit<OpDir<T, 2, false>>(C, ST_DIR, nb, 0, a)));
+        else SGLD_TRY(nt ? (emit<OpDir<T, 4, true>>(C, ST_DIR, nb, 0, a))
+                         : (emit<OpDir<T, 4, false>>(C, ST_DIR, nb, 0, a)));
     }
-    SGLD_HIP(hipGetLastError());
     SGLD_TRY(kt_end(C));
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
     SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, stream_name(f64, nt, 2)));
-    auto upd = nt ? k_update_r<T, true> : k_update_r<T, false>;
-    hipLaunchKernelGGL(upd, dim3(nbv), dim3(256), 0, C->s, r,
-                       (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl);
-    SGLD_HIP(hipGetLastError());
+    const UpdA<T> ua{r, (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl};
+    SGLD_TRY(nt ? (emit<OpUpd<T, true>>(C, ST_UPD, nbv, 0, ua)) : (emit<OpUpd<T, false>>(C, ST_UPD, nbv, 0, ua)));
     SGLD_TRY(kt_end(C));
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
     int nb2 = 0;
@@ -2442,6 +2976,8 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
 
 int true_residual(sgld_ctx *C, double *res, double *normr);
 int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb);
+int residual_to32_enqueue(sgld_ctx *C, double invScale);
+void residual_to32_sums(const sgld_ctx *C, double *rr, double *bb);
 
 // A solve as a resumable state machine: pcg_begin enqueues and runs the set-up,
 // pcg_enqueue_batch queues `batch` iterations plus the copy of the device
@@ -2558,6 +3094,9 @@ int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
     return SGLD_OK;
 }
 
+int mixed_update_begin(sgld_ctx *C, SolveState &S, bool *update);
+int mixed_update_end(sgld_ctx *C, SolveState &S);
+
 // Host step after a completed batch (the control block is in C->hctl).
 int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     const Ctl st = *C->hctl;
@@ -2580,14 +3119,36 @@ int pcg_after_batch(sgld_ctx *C, SolveState &S) {
     // stretch began (or after maxStretch iterations), fold the FP32
     // correction into the FP64 solution and replace the residual by the FP64
     // true residual (reliable update).
+    bool update = false;
+    SGLD_TRY(mixed_update_begin(C, S, &update));
+    if (!update) return SGLD_OK;
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return mixed_update_end(C, S);
+}
+
+// Reliable update of the mixed solve, in two halves so that a host driving
+// many solves can queue every solve's first half before it waits: the first
+// folds the FP32 correction into x and queues r = f - K x (FP64 K.u) with
+// the copy of its norms' partials; the second, once the stream has passed
+// that copy, decides and queues the continuation.
+int mixed_update_begin(sgld_ctx *C, SolveState &S, bool *update) {
+    const Ctl st = *C->hctl;
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
     S.its = st.its;
-    if (!st.done && S.its - S.itsAtUpdate < S.maxStretch) return SGLD_OK;
+    *update = st.done || S.its - S.itsAtUpdate >= S.maxStretch;
+    if (!*update) return SGLD_OK;
     S.itsAtUpdate = S.its;
     hipLaunchKernelGGL(k_fold_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, (const float *)C->p32.p,
                        S.scale, V, (const Ctl *)C->ctl.p);  // with the pending x32 += lambda p32
     SGLD_HIP(hipGetLastError());
+    return residual_to32_enqueue(C, 1.0 / S.scale);
+}
+
+int mixed_update_end(sgld_ctx *C, SolveState &S) {
+    const Ctl st = *C->hctl;
     double rr = 0, bb = 0;
-    SGLD_TRY(residual_to32(C, 1.0 / S.scale, &rr, &bb));
+    residual_to32_sums(C, &rr, &bb);
     const double normr = std::sqrt(rr), res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     ++S.updates;
     S.res = res;
@@ -2693,23 +3254,30 @@ int pcg_solve(sgld_ctx *C, SolveState &S) {
 // Reliable update of the mixed solve: r32 = (b - K x) * invScale (FP64 K.u
 // and difference, FP32 store) with rr = ||b - K x||^2 and bb = ||b||^2;
 // host-synchronous.
-int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb) {
+// r = f - K x (FP64 K.u) written scaled to FP32, with ||r||^2 and ||f||^2
+// (summed on the device, k_sum2) copied to the context's pinned buffer;
+// residual_to32_sums reads them once the stream has passed the copy.
+int residual_to32_enqueue(sgld_ctx *C, double invScale) {
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
+    if (!C->hpart) SGLD_HIP(hipHostMalloc((void **)&C->hpart, sizeof(double) * 2, hipHostMallocDefault));
     SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, nullptr, nullptr));
     hipLaunchKernelGGL(k_residual_norms32, dim3(nbv), dim3(256), 0, C->s, C->b.p, (const double *)C->q.p, C->r32.p,
                        invScale, C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1));
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, C->s, (const double *)slot(C, 0), (const double *)slot(C, 1),
+                       nbv, slot(C, 2));
     SGLD_HIP(hipGetLastError());
-    std::vector<double> part(2 * kRedBlocks);
-    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks, hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipMemcpyAsync(C->hpart, slot(C, 2), sizeof(double) * 2, hipMemcpyDeviceToHost, C->s));
+    return SGLD_OK;
+}
+void residual_to32_sums(const sgld_ctx *C, double *rr, double *bb) {
+    *rr = C->hpart[0];
+    *bb = C->hpart[1];
+}
+int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb) {
+    SGLD_TRY(residual_to32_enqueue(C, invScale));
     SGLD_HIP(hipStreamSynchronize(C->s));
-    double a = 0, c = 0;
-    for (int i = 0; i < nbv; ++i) {
-        a += part[i];
-        c += part[kRedBlocks + i];
-    }
-    *rr = a;
-    *bb = c;
+    residual_to32_sums(C, rr, bb);
     return SGLD_OK;
 }
 
@@ -3074,6 +3642,7 @@ extern "C" int sgld_ctx_destroy(sgld_ctx *C) {
     if (!C) return SGLD_OK;
     if (C->s) (void)hipStreamSynchronize(C->s);
     if (C->hctl) (void)hipHostFree(C->hctl);
+    if (C->hpart) (void)hipHostFree(C->hpart);
     if (C->e0) (void)hipEventDestroy(C->e0);
     if (C->e1) (void)hipEventDestroy(C->e1);
     for (hipEvent_t e : C->kt.pool) (void)hipEventDestroy(e);
@@ -3587,6 +4156,255 @@ extern "C" int sgld_ctx_autotune(sgld_ctx *C, int reps, double *report) {
 }
 
 // --------------------- many instances, one replayed graph --------------------
+namespace {
+
+// The reliable update of the mixed solve as device launches, recorded for
+// the merged solve ahead of each iteration (steps before the iteration's):
+// check (gate open when the FP32 stretch has stopped or hit its length
+// limit), fold, r = f - K x in FP64, decide (stop, or set up the stretch),
+// and rho = z.r of the new residual through the V-cycle, all gated.
+int enqueue_mixed_update_device(sgld_ctx *C, MixCtl *mix, Ctl *gate) {
+    const Span V = dof_span(C);
+    const int nbv = grid_blocks(V.count);
+    SGLD_TRY(emit<OpMixCheck>(C, 1, 1, 0, MixCheckA{C->ctl.p, gate, mix}));
+    SGLD_TRY(emit<OpMixFold>(C, 2, nbv, 0, MixFoldA{C->x.p, C->x32.p, C->p32.p, V, C->ctl.p, gate, mix}));
+    SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, gate, nullptr));  // step ST_KU
+    SGLD_TRY(emit<OpMixNorms>(C, ST_KU + 10, nbv, 0,
+                              MixNormsA{C->b.p, C->q.p, C->r32.p, C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1), gate,
+                                        mix}));
+    SGLD_TRY(emit<OpMixDecide>(C, ST_KU + 20, 1, 0, MixDecideA{slot(C, 0), slot(C, 1), nbv, C->ctl.p, gate, mix}));
+    SGLD_TRY(emit<OpMixDinvRR>(C, ST_KU + 30, nbv, 0, MixDinvRRA{C->r32.p, C->lv[0].dinv16.p, V, slot(C, 3), gate}));
+    int nb2 = 0;
+    SGLD_TRY(enqueue_vcycle_coarse<float>(C, C->r32.p, slot(C, 2), &nb2, gate));
+    return emit<OpMixFinRho>(C, ST_FIN_RHO, 1, 0,
+                             MixFinRhoA{slot(C, 2), nb2, slot(C, 3), nbv, C->ctl.p, gate, mix});
+}
+
+// The multi-instance iteration: each instance's PCG iteration is recorded
+// once (enqueue_iteration with the context's recorder set), and the recorded
+// launches of all instances are merged step by step into launches that cover
+// every instance (k_multi: instance i runs blocks [start[i], start[i+1])).
+// Launches of one step with different kernel instantiations (span, staged
+// outputs, coarse variant, tail depth) stay separate launches.
+struct MultiIteration {
+    struct Launch {
+        void (*multi)(const void *, const int *, int, int, size_t, hipStream_t);
+        int n, grid, step;
+        size_t shm;
+        DevBuf<char> args;
+        DevBuf<int> start;
+    };
+    std::vector<Launch> launches;
+    // the iteration's launches (steps >= kIterBase), or the update block ahead of it
+    int enqueue(hipStream_t s, bool update, bool iteration) const {
+        for (const Launch &L : launches) {
+            if (L.step >= kIterBase ? !iteration : !update) continue;
+            L.multi(L.args.p, L.start.p, L.n, L.grid, L.shm, s);
+            SGLD_HIP(hipGetLastError());
+        }
+        return SGLD_OK;
+    }
+    static constexpr int kIterBase = 100000;
+};
+
+// mix / gate (mixed precision): per live instance, the device update state;
+// its update block is recorded ahead of the iteration.
+int build_multi_iteration(sgld_ctx *const *ctxs, const std::vector<int> &live, int precision, MultiIteration &M,
+                          MixCtl *mix = nullptr, Ctl *gate = nullptr) {
+    std::vector<OpRec> rec(live.size());
+    for (size_t k = 0; k < live.size(); ++k) {
+        sgld_ctx *C = ctxs[live[k]];
+        C->rec = &rec[k];
+        int rc = SGLD_OK;
+        if (precision != SGLD_PREC_FP64 && mix) rc = enqueue_mixed_update_device(C, mix + k, gate + k);
+        rec[k].base = MultiIteration::kIterBase;  // the iteration's launches, in both precisions
+        if (rc == SGLD_OK)
+            rc = precision == SGLD_PREC_FP64 ? enqueue_iteration<double>(C, C->x.p, C->r.p, C->p.p, C->q.p)
+                                             : enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->p32.p, C->q32.p);
+        C->rec = nullptr;
+        if (rc != SGLD_OK) return rc;
+    }
+    struct E {
+        int step;
+        const RecOp *op;
+    };
+    std::vector<E> all;
+    for (size_t k = 0; k < live.size(); ++k)
+        for (const RecOp &o : rec[k].ops) all.push_back(E{o.step, &o});
+    // by step, then kernel instantiation; instances keep their order (stable)
+    std::stable_sort(all.begin(), all.end(), [](const E &a, const E &b) {
+        if (a.step != b.step) return a.step < b.step;
+        return (uintptr_t)a.op->multi < (uintptr_t)b.op->multi;
+    });
+    M.launches.clear();
+    M.launches.reserve(all.size());
+    for (size_t i = 0; i < all.size();) {
+        size_t j = i;
+        while (j < all.size() && all[j].step == all[i].step && all[j].op->multi == all[i].op->multi) ++j;
+        const size_t sz = all[i].op->args.size();
+        std::vector<char> args(sz * (j - i));
+        std::vector<int> start(j - i + 1, 0);
+        size_t shm = 0;
+        for (size_t q = i; q < j; ++q) {
+            std::memcpy(args.data() + sz * (q - i), all[q].op->args.data(), sz);
+            start[q - i + 1] = start[q - i] + all[q].op->grid;
+            shm = std::max(shm, all[q].op->shm);
+        }
+        M.launches.emplace_back();
+        MultiIteration::Launch &L = M.launches.back();
+        L.multi = all[i].op->multi;
+        L.step = all[i].step;
+        L.n = (int)(j - i);
+        L.grid = start.back();
+        L.shm = shm;
+        SGLD_TRY(L.args.upload(args.data(), args.size()));
+        SGLD_TRY(L.start.upload(start.data(), start.size()));
+        i = j;
+    }
+    return SGLD_OK;
+}
+
+// Multi-instance solve on merged launches (see MultiIteration): one graph
+// holds `batch` iterations of every live instance and the copy of all their
+// control blocks; between graphs each instance's host step runs on its own
+// stream (convergence; reliable updates in mixed precision).
+int multi_pcg_merged(sgld_ctx *const *ctxs, int n, std::vector<SolveState> &S, int precision, int batch,
+                     double t0, double *latency) {
+    std::vector<int> live;
+    for (int i = 0; i < n; ++i)
+        if (!S[i].finished) live.push_back(i);
+    if (live.empty()) return SGLD_OK;
+    static const bool times = std::getenv("SGLD_MULTI_TIMES") != nullptr;
+    const double tb0 = now_s();
+    const int nl = (int)live.size();
+    const bool mixed = precision != SGLD_PREC_FP64;
+    // mixed precision: the reliable updates run on the device (see
+    // enqueue_mixed_update_device), seeded from each solve's begin
+    DevBuf<MixCtl> mix;
+    DevBuf<Ctl> gate;
+    std::vector<MixCtl> hmix(nl);
+    if (mixed) {
+        std::vector<Ctl> hg(nl);
+        for (int k = 0; k < nl; ++k) {
+            const SolveState &st = S[live[k]];
+            hmix[k] = MixCtl{st.scale, st.normb, st.delta, st.tol, st.res, 0, st.maxStretch, 0, 0, 1, 0};
+            hg[k] = Ctl{};
+            hg[k].done = 1;
+        }
+        SGLD_TRY(mix.upload(hmix.data(), nl));
+        SGLD_TRY(gate.upload(hg.data(), nl));
+    }
+    MultiIteration M;
+    SGLD_TRY(build_multi_iteration(ctxs, live, precision, M, mixed ? mix.p : nullptr, mixed ? gate.p : nullptr));
+    std::vector<Ctl *> ptrs(nl);
+    for (int k = 0; k < nl; ++k) ptrs[k] = ctxs[live[k]]->ctl.p;
+    DevBuf<Ctl *> dptrs;
+    DevBuf<Ctl> gathered;
+    SGLD_TRY(dptrs.upload(ptrs.data(), nl));
+    SGLD_TRY(gathered.alloc(nl));
+    struct Res {
+        hipStream_t s = nullptr;
+        Ctl *host = nullptr;
+        MixCtl *hostMix = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        ~Res() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            if (host) (void)hipHostFree(host);
+            if (hostMix) (void)hipHostFree(hostMix);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } R;
+    SGLD_HIP(hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking));
+    SGLD_HIP(hipHostMalloc((void **)&R.host, sizeof(Ctl) * nl, hipHostMallocDefault));
+    SGLD_HIP(hipHostMalloc((void **)&R.hostMix, sizeof(MixCtl) * nl, hipHostMallocDefault));
+    SGLD_HIP(hipStreamBeginCapture(R.s, hipStreamCaptureModeThreadLocal));
+    int rc = SGLD_OK;
+    // mixed: the device update block ahead of every `every`-th iteration (a
+    // stopped stretch waits at most every - 1 iterations for its update).  Its
+    // launches cost even when no instance updates (their grids are sized for
+    // an update), so a block per iteration measured slower: 32 uneven cuboids
+    // (31.9 M DOF) 85.7 ms of graphs at every = 1, 77.0 at 2, 74.3 at 4 (67.3
+    // with the owner K.u throughout).
+    static const int every = [] {
+        const char *env = std::getenv("SGLD_MULTI_UPDATE_EVERY");
+        return env ? std::max(1, std::atoi(env)) : 4;
+    }();
+    for (int b = 0; b < batch && rc == SGLD_OK; ++b) rc = M.enqueue(R.s, mixed && b % every == 0, true);
+    if (rc == SGLD_OK) {
+        hipLaunchKernelGGL(k_gather_ctl, dim3((nl + 63) / 64), dim3(64), 0, R.s, (Ctl *const *)dptrs.p, nl,
+                           gathered.p);
+        if (hipGetLastError() != hipSuccess) rc = SGLD_ERR_HIP;
+    }
+    if (rc == SGLD_OK && hipMemcpyAsync(R.host, gathered.p, sizeof(Ctl) * nl, hipMemcpyDeviceToHost, R.s) != hipSuccess)
+        rc = SGLD_ERR_HIP;
+    if (rc == SGLD_OK && mixed &&
+        hipMemcpyAsync(R.hostMix, mix.p, sizeof(MixCtl) * nl, hipMemcpyDeviceToHost, R.s) != hipSuccess)
+        rc = SGLD_ERR_HIP;
+    const hipError_t e = hipStreamEndCapture(R.s, &R.g);
+    if (rc != SGLD_OK) return rc;
+    SGLD_HIP(e);
+    SGLD_HIP(hipGraphInstantiate(&R.x, R.g, nullptr, nullptr, 0));
+    const double tb1 = now_s();
+    double tGraph = 0.0, tHost = 0.0;
+    int nBatches = 0;
+    int remaining = nl;
+    std::vector<char> counted(nl, 0);
+    // every solve stops within maxit iterations (done = 2); in mixed precision
+    // a stopped stretch idles at most `every` iterations for its update
+    int maxit = 0;
+    for (int k = 0; k < nl; ++k) maxit = std::max(maxit, S[live[k]].maxit);
+    const int maxBatches = (every + 1) * (maxit / batch + 2) + 8;
+    while (remaining > 0) {
+        if (nBatches >= maxBatches) {
+            set_error("sgld:internal: the merged multi-instance solve did not stop within %d batches", maxBatches);
+            return SGLD_ERR_HIP;
+        }
+        const double tg = now_s();
+        SGLD_HIP(hipGraphLaunch(R.x, R.s));
+        SGLD_HIP(hipStreamSynchronize(R.s));
+        tGraph += now_s() - tg;
+        ++nBatches;
+        const double th = now_s();
+        // host steps: FP64 convergence (and the stopped iteration's deferred
+        // update); mixed: the device state says which solves have finished
+        for (int k = 0; k < nl; ++k) {
+            const int i = live[k];
+            if (S[i].finished) continue;
+            sgld_ctx *C = ctxs[i];
+            *C->hctl = R.host[k];
+            if (!mixed) {
+                SGLD_TRY(pcg_after_batch(C, S[i]));
+            } else if (R.hostMix[k].finished) {
+                const MixCtl &m = R.hostMix[k];
+                S[i].finished = true;
+                S[i].its = R.host[k].its;
+                S[i].updates = m.updates;
+                S[i].res = m.res;
+                S[i].done = m.doneCode;
+            }
+        }
+        for (int k = 0; k < nl; ++k) {
+            const int i = live[k];
+            if (S[i].finished && !counted[k]) {
+                counted[k] = 1;
+                --remaining;
+                if (latency) latency[i] = now_s() - t0;
+            }
+        }
+        if (!mixed)
+            for (int k = 0; k < nl; ++k) SGLD_HIP(hipStreamSynchronize(ctxs[live[k]]->s));
+        tHost += now_s() - th;
+    }
+    if (times)
+        std::fprintf(stderr, "[multi] since start %.2f ms: build+capture %.2f ms, %d batches: graphs %.2f ms, host steps %.2f ms, %zu launches per iteration\n",
+                     1e3 * (tb0 - t0), 1e3 * (tb1 - tb0), nBatches, 1e3 * tGraph, 1e3 * tHost, M.launches.size());
+    return SGLD_OK;
+}
+
+}  // namespace
+
 extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF, double *const *dU,
                                      double tol, int maxIT, int precision, int batch, int *its, double *relres,
                                      double *latency) {
@@ -3600,6 +4418,26 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         for (int k = 0; k < i; ++k) SGLD_ARG(ctxs[k] != ctxs[i], "sgld:arg: instance %d repeats a context", i);
     }
     SGLD_HIP(hipDeviceSynchronize());
+    static const bool perInstanceGraphs = [] {
+        const char *env = std::getenv("SGLD_MULTI_GRAPHS");
+        return env && std::atoi(env) != 0;
+    }();
+    // The merged launches amortise launch cost over the instances, so the
+    // V-cycle runs level by level there: the fused small-level tail trades
+    // launches for work (its direct restrictions to the coarsest levels read
+    // each fine node 27 + 343 + 3375 times for three fused levels instead of
+    // 27 per level), which a merged launch pays for every instance.
+    struct TailGuard {
+        std::vector<std::pair<sgld_ctx *, bool>> saved;
+        ~TailGuard() {
+            for (auto &c : saved) c.first->tailOn = c.second;
+        }
+    } tailGuard;
+    if (!perInstanceGraphs)
+        for (int i = 0; i < n; ++i) {
+            tailGuard.saved.push_back({ctxs[i], ctxs[i]->tailOn});
+            ctxs[i]->tailOn = false;
+        }
     const double t0 = now_s();
     std::vector<SolveState> S(n);
     for (int i = 0; i < n; ++i) {
@@ -3615,6 +4453,12 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
     for (int i = 0; i < n; ++i) {
         SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
         if (latency) latency[i] = S[i].finished ? now_s() - t0 : -1.0;
+    }
+    bool merged = !perInstanceGraphs;
+    if (merged) {
+        const int rc = multi_pcg_merged(ctxs, n, S, precision, batch, t0, latency);
+        if (rc == SGLD_ERR_ARG) merged = false;  // a kernel variant without a merged launch: per-instance graphs
+        else if (rc != SGLD_OK) return rc;
     }
     // each unfinished instance's batch of iterations is captured once as a
     // graph; the host re-launches it on the instance's own stream as soon as
@@ -3632,7 +4476,7 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
     } G;
     G.g.assign(n, nullptr);
     G.x.assign(n, nullptr);
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && !merged; ++i) {
         if (S[i].finished) continue;
         sgld_ctx *C = ctxs[i];
         SGLD_HIP(hipStreamBeginCapture(C->s, hipStreamCaptureModeThreadLocal));
@@ -3643,7 +4487,7 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         SGLD_HIP(hipGraphInstantiate(&G.x[i], G.g[i], nullptr, nullptr, 0));
     }
     int live = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && !merged; ++i) {
         if (S[i].finished) continue;
         SGLD_HIP(hipGraphLaunch(G.x[i], ctxs[i]->s));
         ++live;
@@ -3681,6 +4525,7 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         if (S[i].done != 1) allConv = 0;
     }
     for (int i = 0; i < n; ++i) ctxs[i]->stats[0] = now_s() - t0;
+    if (std::getenv("SGLD_MULTI_TIMES")) std::fprintf(stderr, "[multi] total %.2f ms\n", 1e3 * (now_s() - t0));
     if (!allConv) {
         set_error("sgld:maxit: not every instance reached tol %.3e within %d iterations", tol, maxIT);
         return SGLD_ERR_MAXIT;

@@ -57,6 +57,23 @@ struct Level {
 struct BatchState;             // batched.hip
 void batch_free(BatchState *b);
 
+// Launches recorded instead of issued, for the multi-instance iteration:
+// while a context's `rec` is set, the kernels of one PCG iteration are
+// appended here (step: the launch's position in the iteration, a sort key
+// shared by all instances; multi: the launcher of the same kernel over many
+// instances; args: the kernel's argument block).
+struct RecOp {
+    int step;
+    void (*multi)(const void *args, const int *start, int n, int grid, size_t shm, hipStream_t s);
+    int grid;
+    size_t shm;
+    std::vector<char> args;
+};
+struct OpRec {
+    std::vector<RecOp> ops;
+    int base = 0;  // added to the steps of the launches recorded next
+};
+
 // HIP-event timing of the finest-level passes of a solve, per kernel and
 // precision (ids in the order of SGLD_KT_* in sgldfea.h).  Each recorded
 // launch carries its iteration's position in the queued batch (-1 outside
@@ -109,6 +126,7 @@ struct sgld_ctx {
     sgld::DevBuf<double> filterWork;  // density filter: lattice copy of x and the weights, kept
     sgld::DevBuf<sgld::Ctl> ctl;
     sgld::Ctl *hctl = nullptr;  // pinned host mirror
+    double *hpart = nullptr;    // pinned: ||r||^2 and ||f||^2 of a reliable update
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool haveFixed = false, haveModulus = false, assembled = false;
@@ -145,6 +163,7 @@ struct sgld_ctx {
     double stats[8] = {0};
     sgld::KernelTimer kt;
     sgld::BatchState *batch = nullptr;  // multi-RHS buffers, allocated on first use
+    sgld::OpRec *rec = nullptr;          // set while the multi-instance iteration is recorded
     int64_t nDof() const { return 3 * lv[0].g.nodes(); }
 };
 

@@ -81,6 +81,49 @@ def test_multi_equals_single(fleet):
     assert np.linalg.norm(U - U1) <= 1e-6 * np.linalg.norm(U1)
 
 
+@pytest.mark.parametrize("prec", [solver.PREC_FP64, solver.PREC_MIXED])
+def test_multi_merged_equals_single_solves(prec):
+    """Every instance's iteration runs in the merged launches with its own
+    grid and arguments, so each instance's solution is its single solve's
+    bit for bit (one instance on the dense node-centric K.u, one irregular).
+    The merged solve runs the V-cycle level by level, so the single solves
+    here do too (SGLD_TAIL=0 when the contexts are created)."""
+    import os
+    import torch
+    kinds = [(32, 16, 16), "holes", (24, 16, 8), (16, 16, 16)]
+    print("case: merged multi vs single solves", kinds, prec, flush=True)
+    solvers, data = [], []
+    old_tail = os.environ.get("SGLD_TAIL")
+    os.environ["SGLD_TAIL"] = "0"
+    for i, k in enumerate(kinds):
+        d = instance(k, 50 + i)
+        g = solver.FEASolver(d["voxels"], d["L"])
+        g.set_fixed(d["fixed"])
+        g.set_modulus(d["E"])
+        g.assemble(0.6)
+        if i == 3:
+            g.set_schedule("dense", 16, None)
+        solvers.append(g)
+        data.append(d)
+    if old_tail is None:
+        del os.environ["SGLD_TAIL"]
+    else:
+        os.environ["SGLD_TAIL"] = old_tail
+    dF = [torch.from_numpy(d["F"]).cuda() for d in data]
+    dU = [torch.zeros_like(f) for f in dF]
+    its, rr, lat, ok = solver.multi_pcg_device(solvers, [f.data_ptr() for f in dF], [u.data_ptr() for u in dU],
+                                               tol=1e-8, maxIT=1000, precision=prec, batch=8)
+    torch.cuda.synchronize()
+    assert ok
+    for k, (g, d) in enumerate(zip(solvers, data)):
+        U1, its1, rr1 = g.pcg(d["F"], tol=1e-8, maxIT=1000, precision=prec)
+        U = dU[k].cpu().numpy()
+        assert int(its[k]) == its1, (k, its[k], its1)
+        assert np.array_equal(U, U1), (k, np.abs(U - U1).max())
+    for g in solvers:
+        g.close()
+
+
 def test_sensitivity_matches_oracle(fleet):
     for g, m, d, ref, _ in fleet:
         penal, E0, Emin, scale = 3.0, 1.0, 1e-9, 0.37
