@@ -409,7 +409,7 @@ def run_batched(a):
     if best == "mixed_sequential":  # dominant kernel: the single-case FP32 K.u
         roof = {"bound": "hbm", "achieved": bytes32 / ms32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": bytes32 / ms32 / 1e6 / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_kbyu_tiled<float>", "bytes_per_launch": bytes32,
+                "kernel": "k_kbyu_owner<float>", "bytes_per_launch": bytes32,
                 "measured_copy_GBs": copy_gbs, "frac_of_measured": bytes32 / ms32 / 1e6 / copy_gbs}
     else:
         roof = {"bound": "mfma", "achieved": useful / ms_b / 1e9, "peak": BF16_PEAK_TFLOPS,

@@ -234,16 +234,20 @@ int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restri
                           int blocksPerCU);
 /* n independent problems (one context each, distinct) solved together (no
  * counterpart in the reference, whose topology-optimisation loop solves one
- * design at a time).  Each instance's PCG iterations are captured once,
- * `batch` (even) iterations long, as a graph, which is replayed on the
- * instance's own stream until it has converged; a host loop re-launches each
- * instance as soon as its previous batch completed, after the host step of
- * that instance (convergence check; for SGLD_PREC_MIXED the reliable
- * updates), so instances of different sizes overlap.  dF[i], dU[i]:
- * device vectors of instance i (U in: initial guess).  its, relres: per
- * instance (FP64 true relative residual); latency[i] (optional): seconds from
- * the call until instance i was seen converged.  Returns SGLD_ERR_MAXIT if
- * any instance missed tol. */
+ * design at a time).  Every kernel of the PCG iteration is launched once over
+ * all instances (each instance runs its own block range with its own
+ * arguments, so its iterates equal those of its own solve bit for bit); for
+ * SGLD_PREC_MIXED the reliable updates run on the device as well, in gated
+ * launches ahead of every 4th iteration (SGLD_MULTI_UPDATE_EVERY).  `batch`
+ * (even) iterations are replayed as one graph between host checks of the
+ * control blocks.  The V-cycle runs level by level during the call (the
+ * contexts' fused small-level tail is off, restored on return).  A context on
+ * the LDS-atomic K.u (SGLD_KU_OWNER=0), or SGLD_MULTI_GRAPHS=1, falls back to
+ * one graph per instance replayed on the instance's own stream.  dF[i],
+ * dU[i]: device vectors of instance i (U in: initial guess).  its, relres:
+ * per instance (FP64 true relative residual); latency[i] (optional): seconds
+ * from the call until instance i was seen converged.  Returns SGLD_ERR_MAXIT
+ * if any instance missed tol. */
 int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF, double *const *dU,
                           double tol, int maxIT, int precision, int batch, int *its,
                           double *relres, double *latency);
