@@ -814,6 +814,7 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
                                                          const uint8_t *__restrict__ cflag,
                                                          int czc, int tilesR, int tilesC,
                                                          const int32_t *__restrict__ tiles,
+                                                         const uint8_t *__restrict__ fflag,
                                                          const Ctl *ctl) {
     if (stopped(ctl)) return;
     __shared__ T sh[2][kRFN * 3];
@@ -845,12 +846,22 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
     // Loads are unconditional (a valid address for staged nodes outside the
     // lattice) and stay in flight; stash() zeroes the invalid ones.  With
     // conditional loads the compiler waits for the prefetch at each branch.
+    // With fflag (irregular domains), a void fine node (its residual is 0)
+    // loads from the vector's first entries instead, one line every void node
+    // shares, so the tile's void parts move no bytes; its flags are loaded a
+    // layer before its residual so that the address does not wait for them.
     T pf[kRPer][3];
     bool pok[kRPer];
+    uint8_t fl[kRPer];
+    auto fetch_flags = [&](int lf) {
+#pragma unroll
+        for (int i = 0; i < kRPer; ++i)
+            fl[i] = (fflag && goff[i] >= 0 && lf >= 0 && lf < NZf) ? fflag[goff[i] + plf * lf] : (uint8_t)kSolid;
+    };
     auto fetch = [&](int lf) {
 #pragma unroll
         for (int i = 0; i < kRPer; ++i) {
-            pok[i] = goff[i] >= 0 && lf >= 0 && lf < NZf;
+            pok[i] = goff[i] >= 0 && lf >= 0 && lf < NZf && (fl[i] & kSolid);
             const T *src = pok[i] ? rf + 3 * (goff[i] + plf * lf) : rf;
             pf[i][0] = src[0];
             pf[i][1] = src[1];
@@ -869,9 +880,12 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
         }
     };
     const int lfBeg = 2 * L0 - 1, lfEnd = 2 * (L1 - 1) + 1;  // fine layers [lfBeg, lfEnd]
+    fetch_flags(lfBeg);
     fetch(lfBeg);
     stash(sh[0]);
+    fetch_flags(lfBeg + 1);
     fetch(lfBeg + 1);
+    fetch_flags(lfBeg + 2);
     __syncthreads();
     T acc[3] = {0, 0, 0}, nxt[3] = {0, 0, 0};
     int buf = 0;
@@ -911,7 +925,10 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
         }
         if (lf < lfEnd) {
             stash(sh[buf ^ 1]);  // layer lf+1 (fetched one iteration ago)
-            if (lf + 2 <= lfEnd) fetch(lf + 2);
+            if (lf + 2 <= lfEnd) {
+                fetch(lf + 2);       // with the flags loaded one iteration ago
+                fetch_flags(lf + 3);
+            }
             __syncthreads();
             buf ^= 1;
         }
@@ -923,8 +940,9 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
                                                          const uint8_t *__restrict__ cflag,
                                                          int czc, int tilesR, int tilesC,
                                                          const int32_t *__restrict__ tiles,
+                                                         const uint8_t *__restrict__ fflag,
                                                          const Ctl *ctl) {
-    restrict2_tiled_body<T>(VB::self(), rf, gf, rc, gc, cflag, czc, tilesR, tilesC, tiles, ctl);
+    restrict2_tiled_body<T>(VB::self(), rf, gf, rc, gc, cflag, czc, tilesR, tilesC, tiles, fflag, ctl);
 }
 
 // Component a of (P x_c)(n), span S, at any fine node n (the caller zeroes
@@ -1770,17 +1788,18 @@ template <class T, int S> struct OpRes {
 };
 template <class T> struct Res2A {
     const T *rf; Grid gf; T *rc; Grid gc; const uint8_t *cflag; int czc, tilesR, tilesC; const int32_t *tiles;
-    const Ctl *ctl;
+    const uint8_t *fflag; const Ctl *ctl;
 };
 template <class T> struct OpRes2 {
     using A = Res2A<T>;
     static constexpr int kThreads = 256, kOcc = 0;
     __device__ static void run(VB vb, const A &a) {
-        restrict2_tiled_body<T>(vb, a.rf, a.gf, a.rc, a.gc, a.cflag, a.czc, a.tilesR, a.tilesC, a.tiles, a.ctl);
+        restrict2_tiled_body<T>(vb, a.rf, a.gf, a.rc, a.gc, a.cflag, a.czc, a.tilesR, a.tilesC, a.tiles, a.fflag,
+                                a.ctl);
     }
     static void single(const A &a, int grid, size_t shm, hipStream_t s) {
         hipLaunchKernelGGL(k_restrict2_tiled<T>, dim3(grid), dim3(256), shm, s, a.rf, a.gf, a.rc, a.gc, a.cflag,
-                           a.czc, a.tilesR, a.tilesC, a.tiles, a.ctl);
+                           a.czc, a.tilesR, a.tilesC, a.tiles, a.fflag, a.ctl);
     }
 };
 template <class T> struct ProA {
@@ -2761,7 +2780,8 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
         if (nbt > 0)
             SGLD_TRY(emit<OpRes2<T>>(C, ST_RES + k, nbt, 0,
                                      Res2A<T>{rf, F.g, rc, Cc.g, Cc.nflag.p, czc, tilesR, tilesC,
-                                              list ? (const int32_t *)Cc.rtiles.p : nullptr, ctl}));
+                                              list ? (const int32_t *)Cc.rtiles.p : nullptr,
+                                              C->full || !C->restrictSkipVoid ? nullptr : F.nflag.p, ctl}));
         return SGLD_OK;
     }
     const ResA<T> a{rf, F.g, rc, Cc.g, dc, Cc.nflag.p, (int)!C->full, ctl};
@@ -3570,6 +3590,7 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     for (int q = 0; q < 2; ++q) C->streamNT[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
     if (const char *env = std::getenv("SGLD_STREAM_NT")) C->streamNT[0] = C->streamNT[1] = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_XCD")) C->dirXcd = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_RESTRICT_SKIP_VOID")) C->restrictSkipVoid = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_VEC_BLOCKS"))
         C->vecBlocks = std::max(64, std::min(kRedBlocks, std::atoi(env)));
     double km[9];

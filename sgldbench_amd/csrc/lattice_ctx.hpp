@@ -141,6 +141,7 @@ struct sgld_ctx {
     // and FP32 (125 M-DOF ellipsoids, 17 M coarse nodes: tiled 218 us, node-centric 222 us)
     int64_t restrictTiledMin32 = (int64_t)1 << 22;
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
+    bool restrictSkipVoid = true;  // irregular domains: the tiled restriction stages no void fine node
     int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
     int vecBlocks = sgld::kVecBlocks;  // grid of the finest-level direction and residual updates
     bool dirXcd = true;  // direction update with one contiguous DOF range per XCD
