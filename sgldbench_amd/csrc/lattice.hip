@@ -182,19 +182,24 @@ __device__ __forceinline__ void mode_forces(T (&uh)[3][8], T Ee, const ModeK<T> 
     // numbers define it (km9: own-axis linear, cross-axis linear, bilinear
     // with own axis, bilinear without, trilinear, then the four couplings).
     // E scales the nine coefficients (9 multiplies instead of 21 on the modes)
-    const T dLo = Ee * km.v[0], dLx = Ee * km.v[1], dBo = Ee * km.v[2], dBx = Ee * km.v[3], dT = Ee * km.v[4];
+    // (the 3 x 3 blocks as d x_i + c (x + y + z) with d = own - coupling:
+    // one shared sum and product instead of three)
+    const T dLo = Ee * (km.v[0] - km.v[5]), dLx = Ee * km.v[1], dBo = Ee * km.v[2], dBx = Ee * (km.v[3] - km.v[8]);
+    const T dT = Ee * km.v[4];
     const T c0 = Ee * km.v[5], c1 = Ee * km.v[6], c2 = Ee * km.v[7], c3 = Ee * km.v[8];
     {
         const T x = uh[0][1], yv = uh[1][2], z = uh[2][4];
-        uh[0][1] = dLo * x + c0 * (yv + z);
-        uh[1][2] = dLo * yv + c0 * (x + z);
-        uh[2][4] = dLo * z + c0 * (x + yv);
+        const T cs = c0 * (x + yv + z);
+        uh[0][1] = dLo * x + cs;
+        uh[1][2] = dLo * yv + cs;
+        uh[2][4] = dLo * z + cs;
     }
     {
         const T x = uh[0][6], yv = uh[1][5], z = uh[2][3];
-        uh[0][6] = dBx * x + c3 * (yv + z);
-        uh[1][5] = dBx * yv + c3 * (x + z);
-        uh[2][3] = dBx * z + c3 * (x + yv);
+        const T cs = c3 * (x + yv + z);
+        uh[0][6] = dBx * x + cs;
+        uh[1][5] = dBx * yv + cs;
+        uh[2][3] = dBx * z + cs;
     }
     {
         const T x = uh[0][2], yv = uh[1][1];
@@ -642,6 +647,7 @@ __device__ __forceinline__ void kbyu_owner_body(VB vb, const T *__restrict__ u, 
             for (int q = 0; q < 4; ++q) Alo[a][q] = T(0);
         T Ecur = (einside && Z0 - 1 >= 0) ? pE[-ple] : T(0);
         bool eokCur = true;
+        T dotw = T(0);  // this work item's share of u.y, in T (one FMA per value)
         // stores staged node layer L (its values are in ost[L & 1])
         auto flush = [&](int L) {
             const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * L, vBytes);
@@ -696,13 +702,14 @@ __device__ __forceinline__ void kbyu_owner_body(VB vb, const T *__restrict__ u, 
             if (StageOut && el - 1 >= Z0) flush(el - 1);  // staged in the previous step
             if (out) {
                 const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * el, vBytes);
+                const uint32_t keep = (fl & kSolid) ? (~fl & 7u) : 0u;  // components kept (solid, free)
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const T v0 = left[a] + (ej >= 1 ? xw[256 * a + t - kTE] : T(0));
-                    const T v = ((fl & kSolid) && !(fl & (1u << a))) ? v0 : T(0);
+                    const T v = (keep & (1u << a)) ? v0 : T(0);
                     if constexpr (StageOut) ost[el & 1][3 * t + a] = v;
                     else buf_store(v, yr, yOff + (uint32_t)sizeof(T) * (uint32_t)a);
-                    dotacc += (double)sPrev[f01 + a] * (double)v;
+                    dotw += sPrev[f01 + a] * v;
                 }
             }
             T *tmp = sPrev;
@@ -721,6 +728,7 @@ __device__ __forceinline__ void kbyu_owner_body(VB vb, const T *__restrict__ u, 
             __syncthreads();  // the last layer staged
             if (Zend - 1 >= Z0) flush(Zend - 1);
         }
+        dotacc += (double)dotw;
     }
     if (partial) {
         const double s = block_sum(dotacc);
