@@ -61,6 +61,19 @@ __device__ __forceinline__ V ld_stream(const V *p) {
     else return *p;
 }
 
+// Entry of a finest-level weight or FP32 correction vector: the FP64 solve's
+// vectors are lattice vectors (entry i of the lattice at i); the bfloat16
+// weights and the FP32 correction of the mixed solve hold only the DOF chunks
+// of the span, in span order (lattice entry i at its span position t; t = i
+// on a full cuboid), since only streaming passes read them: on an irregular
+// domain the passes then touch a third of the address range (125 M-DOF
+// ellipsoids: residual update 360 -> 338 us, direction update 633 -> 614 us,
+// measured on the same access pattern).
+template <class T>
+__device__ __forceinline__ int64_t fine_pos(int64_t i, int64_t t) {
+    return sizeof(T) == 4 ? t : i;
+}
+
 template <> struct FineW<double> {
     using D = double;
     template <bool NT = false>
@@ -93,6 +106,15 @@ struct VB {
     for (int64_t t_ = (int64_t)(vb).b * blockDim.x + threadIdx.x; t_ < (S).count;               \
          t_ += (int64_t)(vb).g * blockDim.x)                                                        \
         if (const int64_t i = (S).at(t_); i < (S).n)
+// The same, with the entry's position t in the span as well.
+#define FOR_SPAN_T(i, t, S)                                                                       \
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (S).count;              \
+         t += (int64_t)gridDim.x * blockDim.x)                                                      \
+        if (const int64_t i = (S).at(t); i < (S).n)
+#define FOR_SPAN_VT(i, t, S, vb)                                                                  \
+    for (int64_t t = (int64_t)(vb).b * blockDim.x + threadIdx.x; t < (S).count;                  \
+         t += (int64_t)(vb).g * blockDim.x)                                                         \
+        if (const int64_t i = (S).at(t); i < (S).n)
 
 // ----------------------------------------------------------------------------
 // K.u: y = sum_e E_e Ke u_e at every lattice node, gathered node-centrically
@@ -1022,11 +1044,11 @@ __global__ void __launch_bounds__(256) k_prolong(const T *__restrict__ xc, Grid 
 // into one L2 rather than into all eight.  Positions of one wavefront still
 // share t >> 6 (what Span::at needs for 64-entry chunks).  Needs the grid to
 // be a multiple of 8.
-#define FOR_SPAN_XCD_V(i, S, vb)                                                                        \
+#define FOR_SPAN_XCD_VT(i, t, S, vb)                                                                    \
     for (int64_t u_ = ((S).count + 63) >> 6, e_ = min<int64_t>((S).count, (u_ * ((vb).b % 8 + 1) / 8) << 6), \
-                 t_ = ((u_ * ((vb).b % 8) / 8) << 6) + (int64_t)((vb).b / 8) * blockDim.x + threadIdx.x; \
-         t_ < e_; t_ += (int64_t)((vb).g / 8) * blockDim.x)                                           \
-        if (const int64_t i = (S).at(t_); i < (S).n)
+                 t = ((u_ * ((vb).b % 8) / 8) << 6) + (int64_t)((vb).b / 8) * blockDim.x + threadIdx.x;  \
+         t < e_; t += (int64_t)((vb).g / 8) * blockDim.x)                                             \
+        if (const int64_t i = (S).at(t); i < (S).n)
 
 // ----------------------------------------------------------------------------
 // Finest-level V-cycle output folded into the PCG direction update
@@ -1051,21 +1073,22 @@ __device__ __forceinline__ void prolong_dir_body(VB vb, const T *__restrict__ xc
     // one contiguous range per XCD keeps the coarse gathers in its L2
     // (125 M-DOF ellipsoids, FP32: 4.12 GB fetched + written per launch with
     // blocks interleaved over the XCDs, against 2.81 GB algorithmic)
-    auto body = [&](int64_t i) {
+    auto body = [&](int64_t i, int64_t t) {
+        const int64_t k = fine_pos<T>(i, t);  // dinv and x (span order in FP32)
         const uint32_t n = d3.div((uint32_t)i);
         const int a = (int)((uint32_t)i - 3 * n);
         const T pv = interp_dof<T, S>(xc, gc, df, n, a);
-        const T sv = FineW<T>::template get<NT>(dinv, i) * ld_stream<NT>(&rf[i]);
+        const T sv = FineW<T>::template get<NT>(dinv, k) * ld_stream<NT>(&rf[i]);
         const uint8_t f = fflag[n];
         const T po = ld_stream<NT>(&p[i]);
         const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
-        x[i] = ld_stream<NT>(&x[i]) + lam * po;
+        x[k] = ld_stream<NT>(&x[k]) + lam * po;
         p[i] = z + beta * po;
     };
     if (xsplit) {
-        FOR_SPAN_XCD_V(i, dofs, vb) body(i);
+        FOR_SPAN_XCD_VT(i, t, dofs, vb) body(i, t);
     } else {
-        FOR_SPAN_V(i, dofs, vb) body(i);
+        FOR_SPAN_VT(i, t, dofs, vb) body(i, t);
     }
 }
 template <class T, int S, bool NT>
@@ -1227,12 +1250,12 @@ __device__ __forceinline__ void update_r_body(VB vb, T *__restrict__ r, const T 
     }
     const T lt = (T)lambda;
     double rr = 0.0, dr = 0.0;
-    FOR_SPAN_V(i, S, vb) {
+    FOR_SPAN_VT(i, t, S, vb) {
         const T rn = ld_stream<NT>(&r[i]) - lt * ld_stream<NT>(&q[i]);
         r[i] = rn;
         const double rd = (double)rn;
         rr += rd * rd;
-        dr += (double)FineW<T>::template get<NT>(dinv, i) * rd * rd;
+        dr += (double)FineW<T>::template get<NT>(dinv, fine_pos<T>(i, t)) * rd * rd;
     }
     block_sum2(rr, dr);
     if (threadIdx.x == 0) {
@@ -1253,9 +1276,9 @@ template <class T>
 __global__ void __launch_bounds__(256) k_dinv_rr(const T *__restrict__ r, const typename FineW<T>::D *__restrict__ dinv,
                                                  Span S, double *__restrict__ partial) {
     double acc = 0.0;
-    FOR_SPAN(i, S) {
+    FOR_SPAN_T(i, t, S) {
         const double rd = (double)r[i];
-        acc += (double)FineW<T>::get(dinv, i) * rd * rd;
+        acc += (double)FineW<T>::get(dinv, fine_pos<T>(i, t)) * rd * rd;
     }
     const double s = block_sum(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -1274,14 +1297,14 @@ __global__ void k_residual_masked(const double *__restrict__ b, const double *__
 }
 
 // Reliable update of the mixed mode: x += scale (x32 + lambda p32), x32 = 0
-// (the FP32 correction, with the stopped iteration's deferred lambda p32,
-// folded into the FP64 solution).
+// (the FP32 correction, in span order, with the stopped iteration's deferred
+// lambda p32, folded into the FP64 solution).
 __global__ void k_fold_mixed(double *__restrict__ x, float *__restrict__ x32, const float *__restrict__ p32,
                              double scale, Span S, const Ctl *ctl) {
     const float lam = (float)ctl->lambda;
-    FOR_SPAN(i, S) {
-        x[i] += scale * (double)(x32[i] + lam * p32[i]);
-        x32[i] = 0.0f;
+    FOR_SPAN_T(i, t, S) {
+        x[i] += scale * (double)(x32[t] + lam * p32[i]);
+        x32[t] = 0.0f;
     }
 }
 
@@ -1941,9 +1964,9 @@ struct OpMixFold {
         if (a.gate->done) return;
         const float lam = (float)a.ctl->lambda;
         const double scale = a.mix->scale;
-        FOR_SPAN_V(i, a.S, vb) {
-            a.x[i] += scale * (double)(a.x32[i] + lam * a.p32[i]);
-            a.x32[i] = 0.0f;
+        FOR_SPAN_VT(i, t, a.S, vb) {
+            a.x[i] += scale * (double)(a.x32[t] + lam * a.p32[i]);
+            a.x32[t] = 0.0f;
         }
     }
     static void single(const A &, int, size_t, hipStream_t) {}
@@ -2023,9 +2046,9 @@ struct OpMixDinvRR {
     __device__ static void run(VB vb, const A &a) {
         if (a.gate->done) return;
         double acc = 0.0;
-        FOR_SPAN_V(i, a.S, vb) {
+        FOR_SPAN_VT(i, t, a.S, vb) {
             const double rd = (double)a.r[i];
-            acc += (double)FineW<float>::get(a.dinv, i) * rd * rd;
+            acc += (double)FineW<float>::get(a.dinv, t) * rd * rd;
         }
         const double s = block_sum(acc);
         if (threadIdx.x == 0) a.partial[vb.b] = s;
@@ -2056,8 +2079,7 @@ __global__ void k_gather_ctl(Ctl *const *ctls, int n, Ctl *out) {
 // dinv = 2 omega / diag on solid nodes.
 __global__ void k_diag0(const double *__restrict__ Ke, const double *__restrict__ E, Grid gd,
                         const uint8_t *__restrict__ nflag,
-                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32,
-                        uint16_t *__restrict__ dinv16) {
+                        double omega, double *__restrict__ dinv, float *__restrict__ dinv32) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= gd.nodes()) return;
     const int64_t NY = gd.NY(), NX = gd.NX();
@@ -2077,8 +2099,13 @@ __global__ void k_diag0(const double *__restrict__ Ke, const double *__restrict_
         const double v = solid ? 2.0 * omega / d[a] : 0.0;
         dinv[3 * g + a] = v;
         dinv32[3 * g + a] = (float)v;
-        dinv16[3 * g + a] = to_bf16((float)v);
     }
+}
+
+// The mixed solve's bfloat16 weights in span order (entry i of the lattice at
+// its span position t).
+__global__ void k_pack_w16(const float *__restrict__ dinv32, uint16_t *__restrict__ dinv16, Span S) {
+    FOR_SPAN_T(i, t, S) dinv16[t] = to_bf16(dinv32[i]);
 }
 
 // Coarse diagonal from element matrices (Solving_AssembleFEAstencil.m:82-99).
@@ -3086,7 +3113,9 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     S.itsAtUpdate = 0;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nbv), dim3(256), 0, C->s, C->r32.p, C->r.p,
                        1.0 / S.scale, V);
-    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f, V);
+    // x32 in span order: positions t <= lattice entry i < nDof
+    hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->x32.p, 0.0f,
+                       Span::all(std::min<int64_t>(V.count, C->nDof())));
     SGLD_HIP(hipGetLastError());
     *C->hctl = Ctl{};
     // the FP32 recursion's stop: its residual, relative to the stretch's
@@ -3762,8 +3791,14 @@ int assemble_impl(sgld_ctx *C, double omega) {
     C->omega = omega;
     Level &l0 = C->lv[0];
     hipLaunchKernelGGL(k_diag0, dim3(blocks_for(l0.g.nodes(), 256)), dim3(256), 0, C->s, C->KeDev.p, C->E.p, l0.g,
-                       l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p, l0.dinv16.p);
+                       l0.nflag.p, omega, l0.dinv.p, l0.dinv32.p);
     SGLD_HIP(hipGetLastError());
+    {
+        const Span V = dof_span(C);
+        hipLaunchKernelGGL(k_pack_w16, dim3(grid_blocks(V.count)), dim3(256), 0, C->s, (const float *)l0.dinv32.p,
+                           l0.dinv16.p, V);
+        SGLD_HIP(hipGetLastError());
+    }
     DevBuf<double> KsPrev, KsCur;
     for (int k = 1; k < C->L; ++k) {
         Level &lf = C->lv[k - 1], &lc = C->lv[k];
