@@ -3030,6 +3030,8 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
 }
 
 int true_residual(sgld_ctx *C, double *res, double *normr);
+int true_residual_enqueue(sgld_ctx *C);
+void true_residual_sums(const sgld_ctx *C, double *res, double *normr);
 int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb);
 int residual_to32_enqueue(sgld_ctx *C, double invScale);
 void residual_to32_sums(const sgld_ctx *C, double *rr, double *bb);
@@ -3053,7 +3055,10 @@ struct SolveState {
     int maxStretch = 96;  // force a reliable update after this many FP32 iterations
 };
 
-int fp64_begin(sgld_ctx *C, SolveState &S) {
+// The set-up of a solve in two halves: the first queues work up to the first
+// value the host needs, the second (once the stream has passed it) decides
+// and queues the rest; pcg_begin runs both with a wait between them.
+int fp64_begin_enqueue(sgld_ctx *C, SolveState &S) {
     // Solving_PreconditionedConjugateGradientSolver.m:7-12 on lattice vectors (x holds x0)
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
@@ -3071,7 +3076,9 @@ int fp64_begin(sgld_ctx *C, SolveState &S) {
     SGLD_TRY(enqueue_rho<double>(C, C->r.p, FIN_RHO0));  // rho = z.r, beta = 0: first p = z
     hipLaunchKernelGGL(k_fill<double>, dim3(nbv), dim3(256), 0, C->s, C->p.p, 0.0, V);  // beta 0 x p
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
-    SGLD_HIP(hipStreamSynchronize(C->s));
+    return SGLD_OK;
+}
+int fp64_begin_finish(sgld_ctx *C, SolveState &S) {
     if (!(C->hctl->normb > 0.0)) {  // F = 0: solution 0, nothing to iterate
         S.finished = true;
         S.its = 0;
@@ -3081,11 +3088,12 @@ int fp64_begin(sgld_ctx *C, SolveState &S) {
     return SGLD_OK;
 }
 
-int mixed_begin(sgld_ctx *C, SolveState &S) {
+int mixed_begin_enqueue(sgld_ctx *C, SolveState &) { return true_residual_enqueue(C); }
+int mixed_begin_finish(sgld_ctx *C, SolveState &S) {
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
     double res = 0, normr = 0;
-    SGLD_TRY(true_residual(C, &res, &normr));
+    true_residual_sums(C, &res, &normr);
     S.its = 0;
     S.updates = 0;
     S.res = res;
@@ -3130,9 +3138,17 @@ int mixed_begin(sgld_ctx *C, SolveState &S) {
     return SGLD_OK;
 }
 
-int pcg_begin(sgld_ctx *C, SolveState &S) {
+int pcg_begin_enqueue(sgld_ctx *C, SolveState &S) {
     S.finished = false;
-    return S.precision == SGLD_PREC_FP64 ? fp64_begin(C, S) : mixed_begin(C, S);
+    return S.precision == SGLD_PREC_FP64 ? fp64_begin_enqueue(C, S) : mixed_begin_enqueue(C, S);
+}
+int pcg_begin_finish(sgld_ctx *C, SolveState &S) {
+    return S.precision == SGLD_PREC_FP64 ? fp64_begin_finish(C, S) : mixed_begin_finish(C, S);
+}
+int pcg_begin(sgld_ctx *C, SolveState &S) {
+    SGLD_TRY(pcg_begin_enqueue(C, S));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    return pcg_begin_finish(C, S);
 }
 
 // `batch` iterations.  The last iteration's y += lambda p stays pending (it is
@@ -3339,11 +3355,17 @@ int residual_to32(sgld_ctx *C, double invScale, double *rr, double *bb) {
 }
 
 // True FP64 relative residual ||b - K x|| / ||b||; leaves b - Kx in C->r.
-int true_residual(sgld_ctx *C, double *res, double *normr) {
+int true_residual_enqueue(sgld_ctx *C) { return residual64_enqueue(C, C->b.p, C->x.p, C->r.p); }
+void true_residual_sums(const sgld_ctx *C, double *res, double *normr) {
     double rr = 0, bb = 0;
-    SGLD_TRY(residual64(C, C->b.p, C->x.p, C->r.p, &rr, &bb));
+    residual64_sums(C, &rr, &bb);
     *normr = std::sqrt(rr);
     *res = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+}
+int true_residual(sgld_ctx *C, double *res, double *normr) {
+    SGLD_TRY(true_residual_enqueue(C));
+    SGLD_HIP(hipStreamSynchronize(C->s));
+    true_residual_sums(C, res, normr);
     return SGLD_OK;
 }
 
@@ -3351,21 +3373,31 @@ int true_residual(sgld_ctx *C, double *res, double *normr) {
 
 namespace sgld {
 
-int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb) {
+// r = mask(b - K x) with ||r||^2 and ||b||^2 summed on the device (k_sum2,
+// fixed order) and copied to the context's pinned pair; residual64_sums
+// reads them once the stream has passed the copy (so that a driver of many
+// solves can queue every solve's residual before it waits).
+int residual64_enqueue(sgld_ctx *C, const double *b, const double *x, double *r) {
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
+    if (!C->hpart) SGLD_HIP(hipHostMalloc((void **)&C->hpart, sizeof(double) * 2, hipHostMallocDefault));
     SGLD_TRY(enqueue_kbyu<double>(C, x, C->q.p, nullptr, nullptr, nullptr));
-    std::vector<double> part(2 * kRedBlocks);
     hipLaunchKernelGGL(k_residual_norms, dim3(nbv), dim3(256), 0, C->s, b, (const double *)C->q.p, r,
                        C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1));
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, C->s, (const double *)slot(C, 0), (const double *)slot(C, 1),
+                       nbv, slot(C, 2));
     SGLD_HIP(hipGetLastError());
-    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 0), sizeof(double) * 2 * kRedBlocks,
-                            hipMemcpyDeviceToHost, C->s));
+    SGLD_HIP(hipMemcpyAsync(C->hpart, slot(C, 2), sizeof(double) * 2, hipMemcpyDeviceToHost, C->s));
+    return SGLD_OK;
+}
+void residual64_sums(const sgld_ctx *C, double *rr, double *bb) {
+    *rr = C->hpart[0];
+    *bb = C->hpart[1];
+}
+int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb) {
+    SGLD_TRY(residual64_enqueue(C, b, x, r));
     SGLD_HIP(hipStreamSynchronize(C->s));
-    double a = 0, c = 0;
-    for (int i = 0; i < nbv; ++i) { a += part[i]; c += part[kRedBlocks + i]; }
-    *rr = a;
-    *bb = c;
+    residual64_sums(C, rr, bb);
     return SGLD_OK;
 }
 
@@ -4512,7 +4544,11 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         S[i].batch = batch;
         SGLD_TRY(to_lattice(C, dF[i], C->b.p));
         SGLD_TRY(to_lattice(C, dU[i], C->x.p));
-        SGLD_TRY(pcg_begin(C, S[i]));
+        SGLD_TRY(pcg_begin_enqueue(C, S[i]));  // every instance's set-up queued before the first wait
+    }
+    for (int i = 0; i < n; ++i) {
+        SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
+        SGLD_TRY(pcg_begin_finish(ctxs[i], S[i]));
     }
     for (int i = 0; i < n; ++i) {
         SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
@@ -4576,11 +4612,16 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
     }
     for (int i = 0; i < n; ++i) SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
     int allConv = 1;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n; ++i) {  // every instance's copy-back and true residual queued before the first wait
         sgld_ctx *C = ctxs[i];
         SGLD_TRY(from_lattice(C, C->x.p, dU[i]));
+        SGLD_TRY(true_residual_enqueue(C));
+    }
+    for (int i = 0; i < n; ++i) {
+        sgld_ctx *C = ctxs[i];
+        SGLD_HIP(hipStreamSynchronize(C->s));
         double tres = 0, normr = 0;
-        SGLD_TRY(true_residual(C, &tres, &normr));
+        true_residual_sums(C, &tres, &normr);
         its[i] = S[i].its;
         relres[i] = tres;
         C->stats[1] = S[i].its;

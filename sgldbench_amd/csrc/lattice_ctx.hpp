@@ -192,6 +192,8 @@ int check_ready(sgld_ctx *C);
 // r = b - K x on lattice FP64 vectors (fixed and void DOFs 0), with
 // rr = r.r and bb = b.b; host-synchronous.
 int residual64(sgld_ctx *C, const double *b, const double *x, double *r, double *rr, double *bb);
+int residual64_enqueue(sgld_ctx *C, const double *b, const double *x, double *r);
+void residual64_sums(const sgld_ctx *C, double *rr, double *bb);
 int ensure_efwd(sgld_ctx *C);  // voxel -> 1-based element id map of the finest level
 // Clears the batched coarsest-level x (written at free DOFs only) if allocated.
 int batch_clear_coarse(sgld_ctx *C);
