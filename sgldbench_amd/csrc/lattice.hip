@@ -1676,7 +1676,8 @@ __device__ __forceinline__ void fin_body(const double *__restrict__ partial, int
             ctl->rr = s;
             ctl->res = sqrt(s) / ctl->normb;
             ctl->its += 1;
-            if (ctl->res < ctl->tol) ctl->done = 1;
+            if (ctl->res < ctl->tol && (ctl->flags & kCtlSoft)) ctl->flags |= kCtlWant;
+            if (ctl->res < ctl->tol && !(ctl->flags & kCtlSoft)) ctl->done = 1;
             else if (ctl->its >= ctl->maxit) ctl->done = 2;
             break;
         case FIN_RHO:
@@ -1947,8 +1948,8 @@ struct OpMixCheck {
     static constexpr int kThreads = 64, kOcc = 0;
     __device__ static void run(VB, const A &a) {
         if (threadIdx.x != 0) return;
-        const bool need = !a.mix->finished &&
-                          (a.ctl->done != 0 || a.ctl->its - a.mix->itsAtUpdate >= a.mix->maxStretch);
+        const bool need = !a.mix->finished && (a.ctl->done != 0 || (a.ctl->flags & kCtlWant) ||
+                                               a.ctl->its - a.mix->itsAtUpdate >= a.mix->maxStretch);
         a.gate->done = need ? 0 : 1;
         if (need) a.mix->itsAtUpdate = a.ctl->its;
     }
@@ -2030,6 +2031,7 @@ struct OpMixDecide {
         }
         m.keep = keep;
         c.done = 0;
+        c.flags &= ~kCtlWant;
         c.rho = rhoDir;
         c.lambda = 0.0;
         c.normb = normr / m.scale;
@@ -3053,6 +3055,10 @@ struct SolveState {
     double scale = 1.0, normb = 0.0, delta = 0.1;
     int itsAtUpdate = 0;
     int maxStretch = 96;  // force a reliable update after this many FP32 iterations
+    // FP32 stretches that reach their tolerance keep iterating until the
+    // reliable update (Ctl kCtlSoft): for merged multi-instance solves, whose
+    // updates run every few iterations
+    bool soft = false;
 };
 
 // The set-up of a solve in two halves: the first queues work up to the first
@@ -3131,6 +3137,7 @@ int mixed_begin_finish(sgld_ctx *C, SolveState &S) {
     C->hctl->tol = std::max(S.delta, 0.5 * S.tol * S.normb / normr);
     C->hctl->maxit = S.maxit;
     C->hctl->normb = normr / S.scale;
+    C->hctl->flags = S.soft ? kCtlSoft : 0;
     SGLD_HIP(hipMemcpyAsync(C->ctl.p, C->hctl, sizeof(Ctl), hipMemcpyHostToDevice, C->s));
     SGLD_TRY(enqueue_rho<float>(C, C->r32.p, FIN_RHO0));
     hipLaunchKernelGGL(k_fill<float>, dim3(nbv), dim3(256), 0, C->s, C->p32.p, 0.0f, V);  // beta 0 x p
@@ -3209,7 +3216,7 @@ int mixed_update_begin(sgld_ctx *C, SolveState &S, bool *update) {
     const Span V = dof_span(C);
     const int nbv = grid_blocks(V.count);
     S.its = st.its;
-    *update = st.done || S.its - S.itsAtUpdate >= S.maxStretch;
+    *update = st.done || (st.flags & kCtlWant) || S.its - S.itsAtUpdate >= S.maxStretch;
     if (!*update) return SGLD_OK;
     S.itsAtUpdate = S.its;
     hipLaunchKernelGGL(k_fold_mixed, dim3(nbv), dim3(256), 0, C->s, C->x.p, C->x32.p, (const float *)C->p32.p,
@@ -3245,6 +3252,7 @@ int mixed_update_end(sgld_ctx *C, SolveState &S) {
     }
     *C->hctl = st;
     C->hctl->done = 0;
+    C->hctl->flags &= ~kCtlWant;
     C->hctl->rho = rhoDir;
     C->hctl->lambda = 0.0;  // applied above
     C->hctl->normb = normr / S.scale;  // the new stretch's start
@@ -4534,6 +4542,13 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
             tailGuard.saved.push_back({ctxs[i], ctxs[i]->tailOn});
             ctxs[i]->tailOn = false;
         }
+    // A mixed instance whose FP32 stretch has reached its tolerance keeps
+    // iterating until the next update block instead of idling there (the
+    // blocks run every few iterations): 32 cuboids, merged iterations for the
+    // slowest instance and solve time below.  SGLD_MULTI_SOFT=0 idles, as a
+    // single solve does, so that each instance repeats its single solve bit for bit.
+    const char *softEnv = std::getenv("SGLD_MULTI_SOFT");
+    const bool soft = !perInstanceGraphs && (!softEnv || std::atoi(softEnv) != 0);
     const double t0 = now_s();
     std::vector<SolveState> S(n);
     for (int i = 0; i < n; ++i) {
@@ -4542,6 +4557,7 @@ extern "C" int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double 
         S[i].tol = tol;
         S[i].maxit = maxIT;
         S[i].batch = batch;
+        S[i].soft = soft;
         SGLD_TRY(to_lattice(C, dF[i], C->b.p));
         SGLD_TRY(to_lattice(C, dU[i], C->x.p));
         SGLD_TRY(pcg_begin_enqueue(C, S[i]));  // every instance's set-up queued before the first wait

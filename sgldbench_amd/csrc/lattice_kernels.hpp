@@ -75,8 +75,12 @@ struct ModeK {
 // batch of iterations can be queued (or graph-captured) without host syncs.
 struct Ctl {
     double rho, pq, rr, normb, lambda, beta, res, tol;
-    int its, maxit, done, pad;
+    int its, maxit, done, flags;
 };
+// Ctl::flags.  kCtlSoft: reaching tol does not stop the recursion but sets
+// kCtlWant (the mixed solve's FP32 stretch keeps iterating until its reliable
+// update runs, instead of idling until then).
+constexpr int kCtlSoft = 1, kCtlWant = 2;
 
 // Entries visited by a grid-stride pass over a vector of length n: all of
 // them when `chunks` is null, else the listed chunks of 2^log entries (chunk

@@ -68,8 +68,9 @@ def test_multi_matches_oracle(fleet, prec, batch):
             assert abs(int(its[k]) - its_o) <= 1, (k, its[k], its_o)
 
 
-def test_multi_equals_single(fleet):
+def test_multi_equals_single(fleet, monkeypatch):
     import torch
+    monkeypatch.setenv("SGLD_MULTI_SOFT", "0")  # stopped stretches idle until their update, as in a single solve
     g, m, d, ref, _ = fleet[2]
     dF = torch.from_numpy(d["F"]).cuda()
     dU = torch.zeros_like(dF)
@@ -95,6 +96,8 @@ def test_multi_merged_equals_single_solves(prec):
     solvers, data = [], []
     old_tail = os.environ.get("SGLD_TAIL")
     os.environ["SGLD_TAIL"] = "0"
+    old_soft = os.environ.get("SGLD_MULTI_SOFT")
+    os.environ["SGLD_MULTI_SOFT"] = "0"  # stopped stretches idle until their update, as in a single solve
     for i, k in enumerate(kinds):
         d = instance(k, 50 + i)
         g = solver.FEASolver(d["voxels"], d["L"])
@@ -114,6 +117,10 @@ def test_multi_merged_equals_single_solves(prec):
     its, rr, lat, ok = solver.multi_pcg_device(solvers, [f.data_ptr() for f in dF], [u.data_ptr() for u in dU],
                                                tol=1e-8, maxIT=1000, precision=prec, batch=8)
     torch.cuda.synchronize()
+    if old_soft is None:
+        del os.environ["SGLD_MULTI_SOFT"]
+    else:
+        os.environ["SGLD_MULTI_SOFT"] = old_soft
     assert ok
     for k, (g, d) in enumerate(zip(solvers, data)):
         U1, its1, rr1 = g.pcg(d["F"], tol=1e-8, maxIT=1000, precision=prec)
