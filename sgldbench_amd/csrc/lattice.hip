@@ -1230,25 +1230,16 @@ __global__ void __launch_bounds__(1024) k_sum2(const double *__restrict__ partRR
 // Solving_PreconditionedConjugateGradientSolver.m:16-17: r -= lambda q with
 // partials of r.r (stopping test) and of sum dinv r^2 (the fine smoother's
 // share of the next rho).
+// lambda = rho / p.q comes from the control block (k_fin FIN_PQ of the K.u
+// partials, one block: summing them in every block of this pass cost 8 KB
+// of reads per block, about 40 us per merged launch over 32 instances).
 template <class T, bool NT>
 __device__ __forceinline__ void update_r_body(VB vb, T *__restrict__ r, const T *__restrict__ q,
                                                   const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
-                                                  const double *__restrict__ partPQ, int nPQ, Ctl *ctl) {
+                                                  Ctl *ctl) {
     if (stopped(ctl)) return;
-    // lambda = rho / p.q from the K.u partials, summed by every block in the
-    // same order (so every block gets the same bits); block 0 records p.q and
-    // lambda for the deferred x += lambda p.  This replaces a separate
-    // single-block finishing launch.
-    double pq = 0.0;
-    for (int i = threadIdx.x; i < nPQ; i += blockDim.x) pq += partPQ[i];
-    pq = block_allsum(pq);
-    const double lambda = ctl->rho / pq;
-    if (vb.b == 0 && threadIdx.x == 0) {
-        ctl->pq = pq;
-        ctl->lambda = lambda;
-    }
-    const T lt = (T)lambda;
+    const T lt = (T)ctl->lambda;
     double rr = 0.0, dr = 0.0;
     FOR_SPAN_VT(i, t, S, vb) {
         const T rn = ld_stream<NT>(&r[i]) - lt * ld_stream<NT>(&q[i]);
@@ -1267,8 +1258,8 @@ template <class T, bool NT>
 __global__ void __launch_bounds__(256) k_update_r(T *__restrict__ r, const T *__restrict__ q,
                                                   const typename FineW<T>::D *__restrict__ dinv, Span S,
                                                   double *__restrict__ partRR, double *__restrict__ partDR,
-                                                  const double *__restrict__ partPQ, int nPQ, Ctl *ctl) {
-    update_r_body<T, NT>(VB::self(), r, q, dinv, S, partRR, partDR, partPQ, nPQ, ctl);
+                                                  Ctl *ctl) {
+    update_r_body<T, NT>(VB::self(), r, q, dinv, S, partRR, partDR, ctl);
 }
 
 // partial of sum dinv r^2 (rho of a residual that k_update_r did not produce)
@@ -1654,7 +1645,7 @@ __global__ void k_fill(T *__restrict__ v, T val, Span S) {
     FOR_SPAN(i, S) v[i] = val;
 }
 
-enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_RR, FIN_RHO };  // (lambda = rho / p.q: in k_update_r)
+enum FinOp { FIN_NORMB = 0, FIN_RHO0, FIN_RR, FIN_RHO, FIN_PQ };
 
 // Second reduction stage (fixed order, deterministic) and the scalar updates
 // of the PCG recurrence.
@@ -1683,6 +1674,10 @@ __device__ __forceinline__ void fin_body(const double *__restrict__ partial, int
         case FIN_RHO:
             ctl->beta = s / ctl->rho;
             ctl->rho = s;
+            break;
+        case FIN_PQ:  // Solving_PreconditionedConjugateGradientSolver.m:15
+            ctl->pq = s;
+            ctl->lambda = ctl->rho / s;
             break;
     }
 }
@@ -1779,18 +1774,17 @@ template <class T> struct OpKuDense {
     }
 };
 template <class T> struct UpdA {
-    T *r; const T *q; const typename FineW<T>::D *dinv; Span S; double *partRR; double *partDR;
-    const double *partPQ; int nPQ; Ctl *ctl;
+    T *r; const T *q; const typename FineW<T>::D *dinv; Span S; double *partRR; double *partDR; Ctl *ctl;
 };
 template <class T, bool NT> struct OpUpd {
     using A = UpdA<T>;
     static constexpr int kThreads = 256, kOcc = 0;
     __device__ static void run(VB vb, const A &a) {
-        update_r_body<T, NT>(vb, a.r, a.q, a.dinv, a.S, a.partRR, a.partDR, a.partPQ, a.nPQ, a.ctl);
+        update_r_body<T, NT>(vb, a.r, a.q, a.dinv, a.S, a.partRR, a.partDR, a.ctl);
     }
     static void single(const A &a, int grid, size_t shm, hipStream_t s) {
         hipLaunchKernelGGL((k_update_r<T, NT>), dim3(grid), dim3(256), shm, s, a.r, a.q, a.dinv, a.S, a.partRR,
-                           a.partDR, a.partPQ, a.nPQ, a.ctl);
+                           a.partDR, a.ctl);
     }
 };
 struct FinA {
@@ -2567,7 +2561,7 @@ namespace {
 // instance's launches appear in increasing order, so the multi-instance
 // iteration can merge the instances' launches step by step.
 enum Step {
-    ST_DIR = 100, ST_KU = 200, ST_UPD = 300, ST_FIN_RR = 400,
+    ST_DIR = 100, ST_KU = 200, ST_FIN_PQ = 250, ST_UPD = 300, ST_FIN_RR = 400,
     ST_RES = 500,     // + level
     ST_TAILR = 600, ST_CLO = 700, ST_CUP = 710, ST_TAILP = 800,
     ST_PRO = 1000,    // - level
@@ -2856,8 +2850,8 @@ int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ct
 }
 
 int fin(sgld_ctx *C, const double *partial, int nb, int op, const double *partial2 = nullptr, int nb2 = 0) {
-    return emit<OpFin>(C, op == FIN_RR ? ST_FIN_RR : op == FIN_RHO ? ST_FIN_RHO : ST_OTHER, 1, 0,
-                       FinA{partial, nb, partial2, nb2, op, C->ctl.p});
+    const int step = op == FIN_RR ? ST_FIN_RR : op == FIN_RHO ? ST_FIN_RHO : op == FIN_PQ ? ST_FIN_PQ : ST_OTHER;
+    return emit<OpFin>(C, step, 1, 0, FinA{partial, nb, partial2, nb2, op, C->ctl.p});
 }
 
 // Top t0 of the fused small-level tail (k_tail_restrict / k_tail_prolong),
@@ -3020,8 +3014,9 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     SGLD_TRY(kt_end(C));
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
+    SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
     SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, stream_name(f64, nt, 2)));
-    const UpdA<T> ua{r, (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), (const double *)slot(C, 0), nb, ctl};
+    const UpdA<T> ua{r, (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), ctl};
     SGLD_TRY(nt ? (emit<OpUpd<T, true>>(C, ST_UPD, nbv, 0, ua)) : (emit<OpUpd<T, false>>(C, ST_UPD, nbv, 0, ua)));
     SGLD_TRY(kt_end(C));
     SGLD_TRY(fin(C, slot(C, 1), nbv, FIN_RR));
