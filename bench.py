@@ -235,13 +235,20 @@ def run_multi(a):
         s.set_modulus(d["E"])
         s.assemble(d["omega"])
         t = s.autotune(reps=3)
+        # The autotuner times single launches.  In the merged multi-instance
+        # launches throughput counts, and each level's restriction is one
+        # launch only if every instance runs the same kernel there:
+        # * the dense node-centric K.u wins single launches on small meshes by
+        #   latency, the mode-space K.u costs less per element in merged ones
+        #   (32 uneven cuboids, mixed: 85.7 -> 77.0 ms of graphs);
+        # * the restriction is pinned to LDS-tiled into level 1 and
+        #   node-centric below for every instance.
         if t["kbyu_variant"] != "tiled":
-            # the autotuner times one launch, where the dense node-centric K.u
-            # wins on small meshes by latency; in the merged multi-instance
-            # launches throughput counts, where the mode-space K.u costs less
-            # per element (32 uneven cuboids, mixed: 85.7 -> 77.0 ms of graphs)
-            s.set_schedule("tiled", 12, t["restriction"])
+            s.set_schedule("tiled", 12, "tiled-first")
             t = dict(t, kbyu_variant="tiled (pinned for the merged launches)", z_chunk=12)
+        else:
+            s.set_schedule("tiled", 0, "tiled-first")  # keeps the autotuned z-chunk of each precision
+        t = dict(t, restriction="tiled into level 1, node-centric below (pinned for the merged launches)")
         tune.append(t)
         S.append(s)
         dF.append(torch.from_numpy(d["F"]).to(dev))
@@ -256,6 +263,13 @@ def run_multi(a):
         return [s.sensitivity_device(u.data_ptr(), x.data_ptr(), c.data_ptr(), penal=d["penal"])
                 for s, u, x, c, d in zip(S, dU, dx, ddc, ins)]
 
+    penal = ins[0]["penal"]
+    assert all(d["penal"] == penal for d in ins)
+
+    def sens_merged():  # one launch over all instances' elements
+        return solver.multi_sensitivity_device(S, [u.data_ptr() for u in dU], [x.data_ptr() for x in dx],
+                                               [c.data_ptr() for c in ddc], penal=penal).tolist()
+
     def step_multi(batch, prec=solver.PREC_MIXED):
         for u in dU:
             u.zero_()
@@ -263,7 +277,7 @@ def run_multi(a):
         t = time.perf_counter()
         its, rr, lat, ok = solver.multi_pcg_device(S, [f.data_ptr() for f in dF], [u.data_ptr() for u in dU],
                                                    tol=1e-8, maxIT=2000, precision=prec, batch=batch)
-        comp = sens_all()
+        comp = sens_merged()
         torch.cuda.synchronize()
         return time.perf_counter() - t, its, rr, lat, ok, comp
 

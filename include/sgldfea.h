@@ -225,11 +225,14 @@ int sgld_ctx_sensitivity_device(sgld_ctx *ctx, const double *dU, const double *d
  * restrictions, [7] FP32 z-chunk. */
 int sgld_ctx_autotune(sgld_ctx *ctx, int reps, double *report);
 /* Pins the schedule the autotuner chooses: kbyuVariant 1 = patch-tiled
- * mode-space K.u with z-chunks of zChunk node layers and a persistent grid of
+ * mode-space K.u with z-chunks of zChunk node layers (0: each precision keeps
+ * its current z-chunk) and a persistent grid of
  * blocksPerCU blocks per compute unit (0 = as many as fit), both precisions;
  * 0 = node-centric
  * dense K.u; restrictTiled -1 = by level size, 0 = node-centric, 1 =
- * LDS-tiled span-2 restriction.  No counterpart in the reference. */
+ * LDS-tiled span-2 restriction, 2 = LDS-tiled into level 1 and node-centric
+ * below (one restriction kernel per level when many instances share the
+ * launches of sgld_multi_pcg_device).  No counterpart in the reference. */
 int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restrictTiled,
                           int blocksPerCU);
 /* n independent problems (one context each, distinct) solved together (no
@@ -251,6 +254,15 @@ int sgld_ctx_set_schedule(sgld_ctx *ctx, int kbyuVariant, int zChunk, int restri
 int sgld_multi_pcg_device(sgld_ctx *const *ctxs, int n, const double *const *dF, double *const *dU,
                           double tol, int maxIT, int precision, int batch, int *its,
                           double *relres, double *latency);
+/* sgld_ctx_sensitivity_device for n instances (distinct contexts) in one
+ * launch over all their elements and one host synchronisation (no
+ * counterpart in the reference).  penal, E0, Emin and scale are shared;
+ * dU[i], dx[i], ddc[i]: device vectors of instance i; compliance[i]: host
+ * output.  Each instance's dc and compliance equal its own
+ * sgld_ctx_sensitivity_device call bit for bit. */
+int sgld_multi_sensitivity_device(sgld_ctx *const *ctxs, int n, const double *const *dU,
+                                  const double *const *dx, double penal, double E0, double Emin,
+                                  double scale, double *const *ddc, double *compliance);
 
 /* Hs = TopOpti_SetupDensityFilter_mex(rMin, numElements, eleMapForward, resX,
  * resY, resZ) (src/MEXfuncs/TopOpti_SetupDensityFilter_mex.cpp:10-51):

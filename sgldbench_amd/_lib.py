@@ -61,6 +61,8 @@ SIGNATURES = {
     "sgld_ctx_set_schedule": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
     "sgld_multi_pcg_device": [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.c_int,
                               C.c_int, C.c_int, _ip, _dp, _dp],
+    "sgld_multi_sensitivity_device": [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.POINTER(_vp), _dp],
     "sgld_density_filter_setup": [C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
     "sgld_density_filter": [_dp, C.c_double, _i64, _ip, C.c_int, C.c_int, C.c_int, _dp],
     "sgld_ctx_density_filter_device": [_vp, _vp, C.c_double, _vp],

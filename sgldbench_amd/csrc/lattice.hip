@@ -1949,8 +1949,16 @@ struct OpMixCheck {
     }
     static void single(const A &, int, size_t, hipStream_t) {}  // merged solves only
 };
+// The gated update passes (fold, norms, weighted rho) are launched with a
+// fraction of the blocks of a single-solve pass: launches in which no
+// instance updates (most of them) then cost a few microseconds instead of
+// about 30 us for the exits of 57 k blocks (32 uneven cuboids).  Each
+// launched block runs the virtual blocks b, b + g, ... of the single-solve
+// grid of vbs blocks, so the per-block partial sums, and with them every
+// iterate, stay those of the single solve.
+#define FOR_VBLOCKS(v, vb, vbs) for (unsigned v = (vb).b; v < (unsigned)(vbs); v += (vb).g)
 struct MixFoldA {
-    double *x; float *x32; const float *p32; Span S; const Ctl *ctl; const Ctl *gate; const MixCtl *mix;
+    double *x; float *x32; const float *p32; Span S; const Ctl *ctl; const Ctl *gate; const MixCtl *mix; int vbs;
 };
 struct OpMixFold {
     using A = MixFoldA;
@@ -1959,16 +1967,19 @@ struct OpMixFold {
         if (a.gate->done) return;
         const float lam = (float)a.ctl->lambda;
         const double scale = a.mix->scale;
-        FOR_SPAN_VT(i, t, a.S, vb) {
-            a.x[i] += scale * (double)(a.x32[t] + lam * a.p32[i]);
-            a.x32[t] = 0.0f;
+        FOR_VBLOCKS(v, vb, a.vbs) {
+            const VB w{v, (unsigned)a.vbs};
+            FOR_SPAN_VT(i, t, a.S, w) {
+                a.x[i] += scale * (double)(a.x32[t] + lam * a.p32[i]);
+                a.x32[t] = 0.0f;
+            }
         }
     }
     static void single(const A &, int, size_t, hipStream_t) {}
 };
 struct MixNormsA {
     const double *b; const double *q; float *r32; const uint8_t *nflag; Span S; double *partRR; double *partBB;
-    const Ctl *gate; const MixCtl *mix;
+    const Ctl *gate; const MixCtl *mix; int vbs;
 };
 struct OpMixNorms {
     using A = MixNormsA;
@@ -1976,19 +1987,22 @@ struct OpMixNorms {
     __device__ static void run(VB vb, const A &a) {
         if (a.gate->done) return;
         const double invScale = 1.0 / a.mix->scale;
-        double rr = 0.0, bb = 0.0;
-        FOR_SPAN_V(i, a.S, vb) {
-            const uint32_t node = (uint32_t)(i / 3), c = (uint32_t)(i - 3 * (int64_t)node);
-            const double bi = a.b[i];
-            const double ri = (a.nflag[node] & (1u << c)) ? 0.0 : bi - a.q[i];
-            a.r32[i] = (float)(ri * invScale);
-            rr += ri * ri;
-            bb += bi * bi;
-        }
-        block_sum2(rr, bb);
-        if (threadIdx.x == 0) {
-            a.partRR[vb.b] = rr;
-            a.partBB[vb.b] = bb;
+        FOR_VBLOCKS(v, vb, a.vbs) {
+            const VB w{v, (unsigned)a.vbs};
+            double rr = 0.0, bb = 0.0;
+            FOR_SPAN_V(i, a.S, w) {
+                const uint32_t node = (uint32_t)(i / 3), c = (uint32_t)(i - 3 * (int64_t)node);
+                const double bi = a.b[i];
+                const double ri = (a.nflag[node] & (1u << c)) ? 0.0 : bi - a.q[i];
+                a.r32[i] = (float)(ri * invScale);
+                rr += ri * ri;
+                bb += bi * bi;
+            }
+            block_sum2(rr, bb);
+            if (threadIdx.x == 0) {
+                a.partRR[v] = rr;
+                a.partBB[v] = bb;
+            }
         }
     }
     static void single(const A &, int, size_t, hipStream_t) {}
@@ -2034,20 +2048,23 @@ struct OpMixDecide {
     static void single(const A &, int, size_t, hipStream_t) {}
 };
 struct MixDinvRRA {
-    const float *r; const uint16_t *dinv; Span S; double *partial; const Ctl *gate;
+    const float *r; const uint16_t *dinv; Span S; double *partial; const Ctl *gate; int vbs;
 };
 struct OpMixDinvRR {
     using A = MixDinvRRA;
     static constexpr int kThreads = 256, kOcc = 0;
     __device__ static void run(VB vb, const A &a) {
         if (a.gate->done) return;
-        double acc = 0.0;
-        FOR_SPAN_VT(i, t, a.S, vb) {
-            const double rd = (double)a.r[i];
-            acc += (double)FineW<float>::get(a.dinv, t) * rd * rd;
+        FOR_VBLOCKS(v, vb, a.vbs) {
+            const VB w{v, (unsigned)a.vbs};
+            double acc = 0.0;
+            FOR_SPAN_VT(i, t, a.S, w) {
+                const double rd = (double)a.r[i];
+                acc += (double)FineW<float>::get(a.dinv, t) * rd * rd;
+            }
+            const double s = block_sum(acc);
+            if (threadIdx.x == 0) a.partial[v] = s;
         }
-        const double s = block_sum(acc);
-        if (threadIdx.x == 0) a.partial[vb.b] = s;
     }
     static void single(const A &, int, size_t, hipStream_t) {}
 };
@@ -2454,20 +2471,76 @@ __global__ void __launch_bounds__(256) k_compliance(const double *__restrict__ K
     ce[i] = q;
 }
 
+// Element energy u_e^T Ke u_e of one hexahedron from its 24 DOFs (corner k at
+// (x, y, z) = (kCornerX, kCornerY, kCornerZ)[k]).  With the mode-space
+// coefficients of the isotropic cube (modes != 0) it is v^T K' v with v the
+// per-component Walsh-Hadamard modes of the corners (Ke = H^T K' H per
+// component, since K' = H Ke H^T / 64 and H^T H = 8 I): 72 adds for the
+// transforms, about 60 operations in mode_forces and a 21-term dot product,
+// instead of the 576 multiply-adds of the dense form, which the other
+// element matrices use.
+__device__ __forceinline__ double element_energy(const double (&ue)[24], const double *__restrict__ Ke,
+                                                 const ModeK<double> &km, int modes) {
+    if (modes) {
+        double v[3][8], w[3][8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int xk = (k == 1 || k == 2 || k == 5 || k == 6);
+            const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
+            const int zk = (k >= 4);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[a][xk + 2 * yk + 4 * zk] = ue[3 * k + a];
+        }
+        // mode m: bit d set -> +1 at coordinate 1, -1 at coordinate 0 along axis d
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (!(i & bit)) {
+                        const double lo = v[a][i], hi = v[a][i | bit];
+                        v[a][i] = lo + hi;
+                        v[a][i | bit] = hi - lo;
+                    }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int m = 0; m < 8; ++m) w[a][m] = v[a][m];
+        mode_forces<double>(w, 1.0, km);
+        double q = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int m = 1; m < 8; ++m) q += v[a][m] * w[a][m];
+        return q;
+    }
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 24; ++r) {
+        double y = 0.0;
+#pragma unroll
+        for (int c = 0; c < 24; ++c) y += Ke[r + 24 * c] * ue[c];
+        q += ue[r] * y;
+    }
+    return q;
+}
+
 // Compliance sensitivity (TopOpti_GlobalVolumeConstraint.m:113-118 with
 // TopOpti_DeMaterialInterpolation.m:5): ce = u_e^T Ke u_e,
 // dc_e = -penal (E0 - Emin) x_e^(penal-1) ce * scale, and the partial of the
 // compliance sum_e E_e ce (E_e the modulus on the lattice).
-__global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ Ke, const double *__restrict__ u, Grid gd,
-                                                     const int32_t *__restrict__ emap, int64_t ne,
-                                                     const double *__restrict__ x, const double *__restrict__ E,
-                                                     double penal, double dE, double scale,
-                                                     double *__restrict__ dc, double *__restrict__ partial) {
+struct SensA {
+    const double *Ke; const double *u; Grid gd; const int32_t *emap; int64_t ne; const double *x; const double *E;
+    double penal, dE, scale; double *dc; double *partial; ModeK<double> km; int modes;
+};
+__device__ __forceinline__ void sensitivity_body(VB vb, const SensA &a) {
+    const Grid gd = a.gd;
+    const int64_t NY = gd.NY(), pl = gd.NY() * gd.NX();
     double cacc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = emap ? emap[i] : i;
+    for (int64_t i = (int64_t)vb.b * blockDim.x + threadIdx.x; i < a.ne; i += (int64_t)vb.g * blockDim.x) {
+        const int64_t v = a.emap ? a.emap[i] : i;
         const int er = (int)(v % gd.ny), ec = (int)((v / gd.ny) % gd.nx), el = (int)(v / ((int64_t)gd.ny * gd.nx));
-        const int64_t NY = gd.NY(), pl = gd.NY() * gd.NX();
         double ue[24];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -2475,24 +2548,38 @@ __global__ void __launch_bounds__(256) k_sensitivity(const double *__restrict__ 
             const int yk = (k == 2 || k == 3 || k == 6 || k == 7);
             const int zk = (k >= 4);
             const int64_t g = (er + 1 - yk) + NY * (ec + xk) + pl * (el + zk);
-            ue[3 * k] = u[3 * g];
-            ue[3 * k + 1] = u[3 * g + 1];
-            ue[3 * k + 2] = u[3 * g + 2];
+            ue[3 * k] = a.u[3 * g];
+            ue[3 * k + 1] = a.u[3 * g + 1];
+            ue[3 * k + 2] = a.u[3 * g + 2];
         }
-        double q = 0.0;
-#pragma unroll
-        for (int r = 0; r < 24; ++r) {
-            double y = 0.0;
-#pragma unroll
-            for (int c = 0; c < 24; ++c) y += Ke[r + 24 * c] * ue[c];
-            q += ue[r] * y;
-        }
-        dc[i] = -penal * dE * pow(x[i], penal - 1.0) * q * scale;
-        cacc += E[v] * q;
+        const double q = element_energy(ue, a.Ke, a.km, a.modes);
+        a.dc[i] = -a.penal * a.dE * pow(a.x[i], a.penal - 1.0) * q * a.scale;
+        cacc += a.E[v] * q;
     }
     const double s = block_sum(cacc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s * scale;
+    if (threadIdx.x == 0) a.partial[vb.b] = s * a.scale;
 }
+struct OpSens {
+    using A = SensA;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) { sensitivity_body(vb, a); }
+    static void single(const A &, int, size_t, hipStream_t) {}  // merged launches only
+};
+// One block per instance: the sum of its nb partials (in a fixed order) into *out.
+struct SumA {
+    const double *partial; int nb; double *out;
+};
+struct OpSum {
+    using A = SumA;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB, const A &a) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < a.nb; i += blockDim.x) acc += a.partial[i];
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) *a.out = s;
+    }
+    static void single(const A &, int, size_t, hipStream_t) {}  // merged launches only
+};
 
 // Nodal stresses (FEA_StressAnalysis.m:33-59), node-centric: each solid node
 // gathers, from each solid element around it, E_e T_j u_e (T_j: the 6 rows of
@@ -2797,9 +2884,11 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     // FP32 only beyond the Infinity Cache (256x128x128 first level, FP64 /
     // FP32: tiled 31 / 17 us, node-centric 35 / 13 us; 512^3 ellipsoids FP32
     // node-centric 222 us for 0.56 GB)
+    // (restrictTiled 2: tiled into level 1 only, node-centric below, one
+    // kernel per level for every instance of a merged multi-instance solve)
     const bool tiled = C->restrictTiled < 0
                            ? Cc.g.nodes() >= (sizeof(T) == 8 ? C->restrictTiledMin : C->restrictTiledMin32)
-                           : C->restrictTiled != 0;
+                           : C->restrictTiled == 2 ? k == 1 : C->restrictTiled != 0;
     const NodeDiv dc = NodeDiv::make(Cc.g);
     if (Cc.span == 2 && tiled) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
@@ -3267,7 +3356,8 @@ int element_data(sgld_ctx *C) {
     for (int i = 0; i < 576; ++i) k32[i] = (float)C->Ke[i];
     SGLD_TRY(C->KeDev32.upload(k32, 576));
     double km[9];
-    if (mode_coefficients(C->Ke, km))
+    C->haveModes = mode_coefficients(C->Ke, km) ? 1 : 0;
+    if (C->haveModes)
         for (int i = 0; i < 9; ++i) {
             C->km64.v[i] = km[i];
             C->km32.v[i] = (float)km[i];
@@ -4162,23 +4252,75 @@ extern "C" int sgld_ctx_kernel_times(const sgld_ctx *C, double *out) {
 }
 
 // ------------------------- sensitivities, autotuning ------------------------
+SensA sens_args(sgld_ctx *C, const double *u, const double *dx, double penal, double E0, double Emin,
+                double scale, double *ddc) {
+    return SensA{C->KeDev.p, u, C->lv[0].g, C->full ? nullptr : C->eleMap.p, C->lv[0].numElements, dx, C->E.p,
+                 penal, E0 - Emin, scale, ddc, slot(C, 3), C->km64, C->haveModes};
+}
+
+extern "C" int sgld_multi_sensitivity_device(sgld_ctx *const *ctxs, int n, const double *const *dU,
+                                             const double *const *dx, double penal, double E0, double Emin,
+                                             double scale, double *const *ddc, double *compliance);
+
+// One instance through the merged launch: the kernel the merged call runs,
+// so that its results and the merged call's agree bit for bit (two kernels
+// built from one body may round differently).
 extern "C" int sgld_ctx_sensitivity_device(sgld_ctx *C, const double *dU, const double *dx, double penal,
                                            double E0, double Emin, double scale, double *ddc, double *compliance) {
     SGLD_ARG(C && dU && dx && ddc, "sgld:arg: null argument");
-    SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
-    SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
-    const int64_t ne = C->lv[0].numElements;
-    SGLD_TRY(to_lattice(C, dU, C->z.p));
-    const int nb = grid_blocks(ne);
-    hipLaunchKernelGGL(k_sensitivity, dim3(nb), dim3(256), 0, C->s, C->KeDev.p, C->z.p, C->lv[0].g,
-                       C->full ? nullptr : C->eleMap.p, ne, dx, C->E.p, penal, E0 - Emin, scale, ddc, slot(C, 3));
-    SGLD_HIP(hipGetLastError());
-    std::vector<double> part(nb);
-    SGLD_HIP(hipMemcpyAsync(part.data(), slot(C, 3), sizeof(double) * nb, hipMemcpyDeviceToHost, C->s));
-    SGLD_HIP(hipStreamSynchronize(C->s));
     double c = 0.0;
-    for (double v : part) c += v;
+    SGLD_TRY(sgld_multi_sensitivity_device(&C, 1, &dU, &dx, penal, E0, Emin, scale, &ddc, &c));
     if (compliance) *compliance = c;
+    return SGLD_OK;
+}
+
+extern "C" int sgld_multi_sensitivity_device(sgld_ctx *const *ctxs, int n, const double *const *dU,
+                                             const double *const *dx, double penal, double E0, double Emin,
+                                             double scale, double *const *ddc, double *compliance) {
+    SGLD_ARG(ctxs && dU && dx && ddc && compliance && n >= 1, "sgld:arg: null argument or n < 1");
+    SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
+    for (int i = 0; i < n; ++i) {
+        SGLD_ARG(ctxs[i] && dU[i] && dx[i] && ddc[i], "sgld:arg: null argument of instance %d", i);
+        SGLD_ARG(ctxs[i]->haveModulus, "sgld:state: instance %d: call sgld_ctx_set_modulus first", i);
+        for (int k = 0; k < i; ++k) SGLD_ARG(ctxs[k] != ctxs[i], "sgld:arg: instance %d repeats a context", i);
+    }
+    // full cuboids read U where it lies; the others scatter it onto their
+    // lattice first (on their own streams, which are drained before the
+    // merged launch on the first context's stream)
+    std::vector<SensA> sa(n);
+    std::vector<SumA> su(n);
+    std::vector<int> start(n + 1, 0), iota(n + 1);
+    DevBuf<double> out;
+    SGLD_TRY(out.alloc(n));
+    for (int i = 0; i < n; ++i) {
+        sgld_ctx *C = ctxs[i];
+        const double *u = dU[i];
+        if (!C->full) {
+            SGLD_TRY(to_lattice(C, dU[i], C->z.p));
+            u = C->z.p;
+        }
+        const int nb = grid_blocks(C->lv[0].numElements);
+        sa[i] = sens_args(C, u, dx[i], penal, E0, Emin, scale, ddc[i]);
+        su[i] = SumA{slot(C, 3), nb, out.p + i};
+        start[i + 1] = start[i] + nb;
+        iota[i] = i;
+    }
+    iota[n] = n;
+    for (int i = 0; i < n; ++i) SGLD_HIP(hipStreamSynchronize(ctxs[i]->s));
+    DevBuf<SensA> dsa;
+    DevBuf<SumA> dsu;
+    DevBuf<int> dstart, diota;
+    SGLD_TRY(dsa.upload(sa.data(), n));
+    SGLD_TRY(dsu.upload(su.data(), n));
+    SGLD_TRY(dstart.upload(start.data(), n + 1));
+    SGLD_TRY(diota.upload(iota.data(), n + 1));
+    hipStream_t s = ctxs[0]->s;
+    multi_launch<OpSens>(dsa.p, dstart.p, n, start[n], 0, s);
+    SGLD_HIP(hipGetLastError());
+    multi_launch<OpSum>(dsu.p, diota.p, n, n, 0, s);
+    SGLD_HIP(hipGetLastError());
+    SGLD_HIP(hipMemcpyAsync(compliance, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    SGLD_HIP(hipStreamSynchronize(s));
     return SGLD_OK;
 }
 
@@ -4264,15 +4406,21 @@ namespace {
 // and rho = z.r of the new residual through the V-cycle, all gated.
 int enqueue_mixed_update_device(sgld_ctx *C, MixCtl *mix, Ctl *gate) {
     const Span V = dof_span(C);
-    const int nbv = grid_blocks(V.count);
+    const int nbv = grid_blocks(V.count);  // virtual blocks: the single-solve grid of these passes
+    static const int coarsen = [] {
+        const char *env = std::getenv("SGLD_MULTI_UPDATE_COARSEN");
+        return env ? std::max(1, std::atoi(env)) : 4;
+    }();
+    const int nbu = (nbv + coarsen - 1) / coarsen;  // launched blocks
     SGLD_TRY(emit<OpMixCheck>(C, 1, 1, 0, MixCheckA{C->ctl.p, gate, mix}));
-    SGLD_TRY(emit<OpMixFold>(C, 2, nbv, 0, MixFoldA{C->x.p, C->x32.p, C->p32.p, V, C->ctl.p, gate, mix}));
+    SGLD_TRY(emit<OpMixFold>(C, 2, nbu, 0, MixFoldA{C->x.p, C->x32.p, C->p32.p, V, C->ctl.p, gate, mix, nbv}));
     SGLD_TRY(enqueue_kbyu<double>(C, C->x.p, C->q.p, nullptr, gate, nullptr));  // step ST_KU
-    SGLD_TRY(emit<OpMixNorms>(C, ST_KU + 10, nbv, 0,
+    SGLD_TRY(emit<OpMixNorms>(C, ST_KU + 10, nbu, 0,
                               MixNormsA{C->b.p, C->q.p, C->r32.p, C->lv[0].nflag.p, V, slot(C, 0), slot(C, 1), gate,
-                                        mix}));
+                                        mix, nbv}));
     SGLD_TRY(emit<OpMixDecide>(C, ST_KU + 20, 1, 0, MixDecideA{slot(C, 0), slot(C, 1), nbv, C->ctl.p, gate, mix}));
-    SGLD_TRY(emit<OpMixDinvRR>(C, ST_KU + 30, nbv, 0, MixDinvRRA{C->r32.p, C->lv[0].dinv16.p, V, slot(C, 3), gate}));
+    SGLD_TRY(emit<OpMixDinvRR>(C, ST_KU + 30, nbu, 0,
+                               MixDinvRRA{C->r32.p, C->lv[0].dinv16.p, V, slot(C, 3), gate, nbv}));
     int nb2 = 0;
     SGLD_TRY(enqueue_vcycle_coarse<float>(C, C->r32.p, slot(C, 2), &nb2, gate));
     return emit<OpMixFinRho>(C, ST_FIN_RHO, 1, 0,
@@ -4496,6 +4644,12 @@ int multi_pcg_merged(sgld_ctx *const *ctxs, int n, std::vector<SolveState> &S, i
             for (int k = 0; k < nl; ++k) SGLD_HIP(hipStreamSynchronize(ctxs[live[k]]->s));
         tHost += now_s() - th;
     }
+    if (times) {
+        std::string steps;
+        for (const MultiIteration::Launch &L : M.launches)
+            steps += " " + std::to_string(L.step) + ":" + std::to_string(L.n) + "/" + std::to_string(L.grid);
+        std::fprintf(stderr, "[multi] launches (step:instances/blocks):%s\n", steps.c_str());
+    }
     if (times)
         std::fprintf(stderr, "[multi] since start %.2f ms: build+capture %.2f ms, %d batches: graphs %.2f ms, host steps %.2f ms, %zu launches per iteration\n",
                      1e3 * (tb0 - t0), 1e3 * (tb1 - tb0), nBatches, 1e3 * tGraph, 1e3 * tHost, M.launches.size());
@@ -4654,15 +4808,15 @@ extern "C" int sgld_ctx_set_schedule(sgld_ctx *C, int kbyuVariant, int zChunk, i
     SGLD_ARG(C, "sgld:arg: null context");
     SGLD_ARG(blocksPerCU >= 0 && blocksPerCU <= 8, "sgld:arg: blocksPerCU must be in 0..8 (0 = occupancy)");
     SGLD_ARG(kbyuVariant == 0 || kbyuVariant == 1, "sgld:arg: kbyuVariant must be 0 (dense) or 1 (tiled)");
-    SGLD_ARG(restrictTiled >= -1 && restrictTiled <= 1, "sgld:arg: restrictTiled must be -1, 0 or 1");
+    SGLD_ARG(restrictTiled >= -1 && restrictTiled <= 2, "sgld:arg: restrictTiled must be -1, 0, 1 or 2");
     if (kbyuVariant == 1) {
         double km[9];
         SGLD_ARG(mode_coefficients(C->Ke, km), "sgld:arg: the tiled K.u needs the isotropic-cube Ke structure");
-        SGLD_ARG(zChunk >= 2 && zChunk <= 256, "sgld:arg: zChunk must be in 2..256");
+        SGLD_ARG(zChunk == 0 || (zChunk >= 2 && zChunk <= 256), "sgld:arg: zChunk must be 0 or in 2..256");
         for (int q = 0; q < 2; ++q) {
             const int per = blocksPerCU > 0 ? blocksPerCU : std::max(1, C->kOcc[q]);
             C->ksched[q].cap = (int)std::min<int64_t>((int64_t)C->numCU * per, kRedBlocks);
-            SGLD_TRY(set_kchunk(C, q, zChunk));
+            SGLD_TRY(set_kchunk(C, q, zChunk > 0 ? zChunk : C->ksched[q].tm.cz));
         }
     }
     C->kbyuVariant = kbyuVariant;

@@ -104,6 +104,7 @@ struct sgld_ctx {
     sgld::DevBuf<double> KeDev;   // Ke on the device (FP64, column-major)
     sgld::DevBuf<float> KeDev32;  // and in FP32
     sgld::ModeK<double> km64{};   // mode-space coefficients of Ke (tiled K.u)
+    int haveModes = 0;            // Ke has the isotropic-cube mode structure (km64 / km32 valid)
     sgld::ModeK<float> km32{};
     std::vector<sgld::Level> lv;
     bool full = true;
@@ -136,7 +137,7 @@ struct sgld_ctx {
     // owner K.u, [0] FP64 / [1] FP32: outputs through LDS for contiguous stores
     // (set when the output vector exceeds the Infinity Cache share, see sgld_ctx_create)
     bool kbyuStageOut[2] = {true, true};
-    int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled
+    int restrictTiled = -1;  // span-2 restriction: -1 by size, 0 node-centric, 1 LDS-tiled, 2 tiled into level 1 only
     int64_t restrictTiledMin = (int64_t)1 << 18;    // "by size": FP64 tiled from this many coarse nodes
     // and FP32 (125 M-DOF ellipsoids, 17 M coarse nodes: tiled 218 us, node-centric 222 us)
     int64_t restrictTiledMin32 = (int64_t)1 << 22;

@@ -373,7 +373,7 @@ class FEASolver:
 
     def set_schedule(self, kbyu_variant="tiled", z_chunk=16, restriction=None, blocks_per_cu=0):
         """Pins a K.u / restriction schedule (what autotune picks)."""
-        rt = {None: -1, "node-centric": 0, "tiled": 1}[restriction]
+        rt = {None: -1, "node-centric": 0, "tiled": 1, "tiled-first": 2}[restriction]
         check(lib().sgld_ctx_set_schedule(self._h, 1 if kbyu_variant == "tiled" else 0, z_chunk, rt,
                                           blocks_per_cu))
 
@@ -411,3 +411,16 @@ def multi_pcg_device(solvers, dF_ptrs, dU_ptrs, tol=1e-8, maxIT=500, precision=P
     rc = check(lib().sgld_multi_pcg_device(ctxs, n, fp, up, tol, maxIT, precision, batch, its, rr, lat),
                allow=(ERR_MAXIT,))
     return its, rr, lat, rc == 0
+
+
+def multi_sensitivity_device(solvers, dU_ptrs, dx_ptrs, ddc_ptrs, penal=3.0, E0=1.0, Emin=1e-9, scale=1.0):
+    """FEASolver.sensitivity_device for every solver in one launch
+    (sgld_multi_sensitivity_device).  Returns the compliances (numpy array)."""
+    n = len(solvers)
+    ctxs = (C.c_void_p * n)(*[s._h.value for s in solvers])
+    up = (C.c_void_p * n)(*dU_ptrs)
+    xp = (C.c_void_p * n)(*dx_ptrs)
+    cp = (C.c_void_p * n)(*ddc_ptrs)
+    comp = np.zeros(n)
+    check(lib().sgld_multi_sensitivity_device(ctxs, n, up, xp, penal, E0, Emin, scale, cp, comp))
+    return comp

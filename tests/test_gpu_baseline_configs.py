@@ -136,19 +136,22 @@ def test_cfg4_32_uneven_instances_with_sensitivities():
     ins = problems.cfg4()
     print("case: cfg4 %d instances, %d DOF in total" % (len(ins), sum(d["F"].size for d in ins)), flush=True)
     S = [gpu_solver(d) for d in ins]
+    for s in S:  # the schedule the benchmark pins for the merged launches
+        s.set_schedule("tiled", 12, "tiled-first")
     try:
         dF = [torch.from_numpy(d["F"]).cuda() for d in ins]
         dU = [torch.zeros_like(f) for f in dF]
         its, rr, lat, ok = solver.multi_pcg_device(S, [f.data_ptr() for f in dF], [u.data_ptr() for u in dU],
                                                    tol=1e-8, maxIT=2000, precision=solver.PREC_MIXED, batch=8)
         assert ok
-        res = []
-        for s, u, d in zip(S, dU, ins):
-            dx = torch.from_numpy(d["x"]).cuda()
-            ddc = torch.zeros_like(dx)
-            comp = s.sensitivity_device(u.data_ptr(), dx.data_ptr(), ddc.data_ptr(), penal=d["penal"])
-            torch.cuda.synchronize()
-            res.append((u.cpu().numpy(), ddc.cpu().numpy(), comp))
+        # the sensitivities of all instances in one launch, as the benchmark runs them
+        dx = [torch.from_numpy(d["x"]).cuda() for d in ins]
+        ddc = [torch.zeros_like(x) for x in dx]
+        assert all(d["penal"] == ins[0]["penal"] for d in ins)
+        comp = solver.multi_sensitivity_device(S, [u.data_ptr() for u in dU], [x.data_ptr() for x in dx],
+                                               [c.data_ptr() for c in ddc], penal=ins[0]["penal"])
+        torch.cuda.synchronize()
+        res = [(u.cpu().numpy(), c.cpu().numpy(), comp[i]) for i, (u, c) in enumerate(zip(dU, ddc))]
     finally:
         for s in S:
             s.close()
@@ -158,10 +161,16 @@ def test_cfg4_32_uneven_instances_with_sensitivities():
         assert trr <= 1e-8, (i, d["name"], trr)
         ce = m.unit_compliance(U)
         c_o = float(np.sum(d["E"] * ce))
-        dc_o = -d["penal"] * (1.0 - 1e-9) * d["x"] ** (d["penal"] - 1.0) * ce
-        assert abs(comp - c_o) <= 1e-12 * abs(c_o), (i, comp, c_o)
-        # per-element u^T Ke u cancels: the bound is eps times sum |Ke||u||u|
-        assert np.max(np.abs(dc - dc_o)) <= 1e-10 * np.max(np.abs(dc_o)), i
+        dPdx = d["penal"] * (1.0 - 1e-9) * d["x"] ** (d["penal"] - 1.0)
+        dc_o = -dPdx * ce
+        # u_e^T Ke u_e cancels (a cantilever's displacements are mostly rigid
+        # motion): the oracle's dense product rounds at eps |u_e|^T |Ke| |u_e|,
+        # while the GPU's mode-space form drops the translation exactly, so
+        # the two agree to a small multiple of that bound
+        mag = orc.unit_compliance_mex(np.abs(U), m.get(1, "eNodMat"), np.abs(m.get(1, "Ke")))
+        eps = np.finfo(np.float64).eps
+        assert abs(comp - c_o) <= 1e-13 * abs(c_o) + 32 * eps * float(np.sum(d["E"] * mag)), (i, comp, c_o)
+        assert np.all(np.abs(dc - dc_o) <= 1e-14 * np.max(np.abs(dc_o)) + 32 * eps * dPdx * mag), i
         # energy identity at convergence: F.U = sum_e E_e ce_e to the solve's accuracy
         assert abs(float(d["F"] @ U) - c_o) <= 1e-6 * abs(c_o), i
     print("  all %d instances: true rel-res <= 1e-8, sensitivities match the oracle" % len(ins), flush=True)

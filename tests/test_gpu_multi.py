@@ -143,7 +143,33 @@ def test_sensitivity_matches_oracle(fleet):
         assert abs(c / scale - d["F"] @ ref) <= 1e-8 * abs(d["F"] @ ref)
 
 
-@pytest.mark.parametrize("sched", [("tiled", 8, "node-centric"), ("tiled", 32, "tiled"), ("dense", 16, None)])
+def test_multi_sensitivity_equals_single(fleet):
+    """One launch over every instance's elements (full cuboids read U in
+    place, the holed mesh scatters it first) gives each instance's own
+    sgld_ctx_sensitivity_device results bit for bit."""
+    import torch
+    penal, E0, Emin, scale = 3.0, 1.0, 1e-9, 0.37
+    dU = [torch.from_numpy(ref).cuda() for _, _, _, ref, _ in fleet]
+    dx = [torch.from_numpy(d["x"]).cuda() for _, _, d, _, _ in fleet]
+    one = [torch.zeros_like(x) for x in dx]
+    many = [torch.zeros_like(x) for x in dx]
+    c1 = [g.sensitivity_device(u.data_ptr(), x.data_ptr(), c.data_ptr(), penal, E0, Emin, scale)
+          for (g, *_), u, x, c in zip(fleet, dU, dx, one)]
+    cm = solver.multi_sensitivity_device([g for g, *_ in fleet], [u.data_ptr() for u in dU],
+                                         [x.data_ptr() for x in dx], [c.data_ptr() for c in many],
+                                         penal, E0, Emin, scale)
+    torch.cuda.synchronize()
+    for k, (g, m, d, ref, _) in enumerate(fleet):
+        assert cm[k] == c1[k], (k, cm[k], c1[k])
+        assert torch.equal(many[k], one[k]), k
+        ce = m.unit_compliance(ref)
+        want = -penal * (E0 - Emin) * d["x"] ** (penal - 1) * ce * scale
+        assert np.abs(many[k].cpu().numpy() - want).max() <= 1e-10 * np.abs(want).max(), k
+        assert abs(cm[k] - scale * d["E"] @ ce) <= 1e-10 * abs(scale * d["E"] @ ce), k
+
+
+@pytest.mark.parametrize("sched", [("tiled", 8, "node-centric"), ("tiled", 32, "tiled"), ("dense", 16, None),
+                                   ("tiled", 0, "tiled-first")])
 def test_schedules_match_oracle(sched):
     # 37 node layers: z-chunks of 8 and 32 leave a partial last chunk
     vox = holes(40, 24, 36, 5, nholes=4, r=3)
