@@ -468,18 +468,20 @@ def run_batched(a):
     print(json.dumps(out), flush=True)
 
 
-def kernel_bytes(key, n, n2, ne):
+def kernel_bytes(key, n, n2, ne, kernel=""):
     """Algorithmic bytes of one launch of a finest-level pass (each input read
     once, each output written once; node flags and chunk lists excluded):
-    n DOFs of level 1, n2 of level 2, ne elements."""
+    n DOFs of level 1, n2 of level 2, ne elements.  The residual update fused
+    with the first restriction (k_update_restrict) also writes level 2's
+    residual."""
     s = 8 if key.endswith("64") else 4
     dw = 8 if s == 8 else 2  # Jacobi weights: FP64, or bfloat16 in the FP32 inner solve
     if key.startswith("dir"):       # read x_2, r, dinv, p, x; write p, x
         return n * (5 * s + dw) + n2 * s
     if key.startswith("kbyu"):      # read u, E; write y
         return 2 * n * s + ne * s
-    if key.startswith("update_r"):  # read r, q, dinv; write r
-        return n * (3 * s + dw)
+    if key.startswith("update_r"):  # read r, q, dinv; write r (and r_2 when fused)
+        return n * (3 * s + dw) + (n2 * s if "restrict" in kernel else 0)
     if key.startswith("restrict"):  # read r_1; write r_2
         return (n + n2) * s
     raise KeyError(key)
@@ -675,7 +677,7 @@ def run_single(a):
             if not v["work_launches"]:
                 continue
             avg = v["work_ms"] / v["work_launches"]
-            b = kernel_bytes(key, n, n2, ne)
+            b = kernel_bytes(key, n, n2, ne, v["kernel"])
             kern[path][key] = {"kernel": v["kernel"], "launches": v["launches"], "work_launches": v["work_launches"],
                                "avg_us": 1e3 * avg, "avg_us_all_launches": 1e3 * v["ms"] / v["launches"],
                                "bytes_per_launch": b, "GB_s": b / avg / 1e6, "frac": b / avg / 1e6 / HBM_PEAK_GBS,

@@ -37,6 +37,28 @@ __device__ __forceinline__ double block_sum(double v) {
     return v;  // valid in thread 0
 }
 
+__device__ __forceinline__ void block_sum2(double &a, double &b) {
+    __shared__ double sh[2][16];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+    }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    a = (threadIdx.x < nw) ? sh[0][threadIdx.x] : 0.0;
+    b = (threadIdx.x < nw) ? sh[1][threadIdx.x] : 0.0;
+    if (w == 0)
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 64);
+            b += __shfl_down(b, o, 64);
+        }
+}
+
 __device__ __forceinline__ bool stopped(const Ctl *ctl) { return ctl && ctl->done; }
 
 // Finest-level Jacobi weights dinv = 2 omega / diag: FP64 for the FP64 solve;
@@ -975,6 +997,192 @@ __global__ void __launch_bounds__(256) k_restrict2_tiled(const T *__restrict__ r
     restrict2_tiled_body<T>(VB::self(), rf, gf, rc, gc, cflag, czc, tilesR, tilesC, tiles, fflag, ctl);
 }
 
+// Residual update fused with the first restriction (span 2, LDS-tiled as
+// restrict2_tiled_body): r' = r - lambda q with the partials of r'.r' and
+// sum dinv r'^2 (Solving_PreconditionedConjugateGradientSolver.m:16-17), and
+// r_2 = P^T r' (Solving_Vcycle.m:13 on level 2) from the same staged values,
+// so the residual is read once per iteration instead of twice.
+// Every tile forms r' at all the fine nodes it stages; it writes back, and
+// sums, only the nodes it owns: fine node n belongs to the tile of coarse
+// node floor(n / 2) along each axis (the staged row, column and layer below
+// the tile's own belong to the neighbours).  A solid fine node's owner coarse
+// node is a node of a solid coarse element, so the listed tiles own every
+// solid fine node and no void one is written.  Neighbouring tiles read the
+// shared nodes' r while their owner writes r', so r' goes to a second vector
+// (rOut; the iterations alternate the two).  Persistent over the tiles, one
+// pair of partials per block.  dinvL: the finest weights in lattice order.
+template <class T>
+__device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict__ rf, const T *__restrict__ q,
+                                                     T *__restrict__ rOut,
+                                                     const typename FineW<T>::D *__restrict__ dinvL, Grid gf,
+                                                     T *__restrict__ rc, Grid gc, const uint8_t *__restrict__ cflag,
+                                                     int czc, int tilesR, int tilesC, int64_t nTiles,
+                                                     const int32_t *__restrict__ tiles,
+                                                     const uint8_t *__restrict__ fflag, double *__restrict__ partRR,
+                                                     double *__restrict__ partDR, const Ctl *ctl) {
+    if (stopped(ctl)) return;
+    __shared__ T sh[2][kRFN * 3];
+    const T lt = (T)ctl->lambda;
+    const int t = threadIdx.x;
+    const int NYf = gf.ny + 1, NXf = gf.nx + 1, NZf = gf.nz + 1;
+    const int NYc = gc.ny + 1, NXc = gc.nx + 1, NZc = gc.nz + 1;
+    const int64_t plf = (int64_t)NYf * NXf, plc = (int64_t)NYc * NXc;
+    const int ci = t % kRR, cj = t / kRR;
+    double rr = 0.0, dr = 0.0;
+    for (int64_t w = vb.b; w < nTiles; w += vb.g) {
+        int bid = tiles ? tiles[w] : (int)w;
+        const int tr = bid % tilesR;
+        bid /= tilesR;
+        const int tc = bid % tilesC;
+        const int tz = bid / tilesC;
+        const int R0 = tr * kRR, C0 = tc * kRC;
+        const int L0 = tz * czc, L1 = min(L0 + czc, NZc);  // coarse layers [L0, L1)
+        const int Rc = R0 + ci, Cc = C0 + cj;
+        const bool mine = Rc < NYc && Cc < NXc;
+        int64_t goff[kRPer];
+        bool own[kRPer];
+#pragma unroll
+        for (int i = 0; i < kRPer; ++i) {
+            const int k = t + 256 * i;
+            goff[i] = -1;
+            own[i] = false;
+            if (k < kRFN) {
+                const int kr = k % kRFR, kc = k / kRFR;
+                const int r = 2 * R0 - 1 + kr, c = 2 * C0 - 1 + kc;
+                if (r >= 0 && r < NYf && c >= 0 && c < NXf) goff[i] = r + (int64_t)NYf * c;
+                own[i] = kr >= 1 && kc >= 1;
+            }
+        }
+        const int lfBeg = 2 * L0 - 1, lfEnd = 2 * (L1 - 1) + 1;  // fine layers [lfBeg, lfEnd]
+        // as restrict2_tiled_body: unconditional loads (void and outside
+        // nodes read the vectors' first entries), the flags of a layer loaded
+        // one fetch before its values
+        uint8_t fl[kRPer];
+        auto fetch_flags = [&](int lf) {
+#pragma unroll
+            for (int i = 0; i < kRPer; ++i)
+                fl[i] = (fflag && goff[i] >= 0 && lf >= 0 && lf < NZf) ? fflag[goff[i] + plf * lf] : (uint8_t)kSolid;
+        };
+        auto fetch = [&](int lf, T (&v)[kRPer][6], bool (&ok)[kRPer]) {
+#pragma unroll
+            for (int i = 0; i < kRPer; ++i) {
+                ok[i] = goff[i] >= 0 && lf >= 0 && lf < NZf && lf <= lfEnd && (fl[i] & kSolid);
+                const int64_t o = ok[i] ? 3 * (goff[i] + plf * lf) : 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    v[i][a] = rf[o + a];
+                    v[i][3 + a] = q[o + a];
+                }
+            }
+        };
+        // r' of layer lf (fetched into v) into LDS; the owned nodes to rOut
+        auto stash = [&](T *dst, int lf, const T (&v)[kRPer][6], const bool (&ok)[kRPer]) {
+            const bool ownLayer = lf >= 2 * L0;  // the layer below the tile's belongs to the tile below
+#pragma unroll
+            for (int i = 0; i < kRPer; ++i) {
+                const int k = t + 256 * i;
+                if (k < kRFN) {
+                    T u[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        u[a] = ok[i] ? v[i][a] - lt * v[i][3 + a] : T(0);
+                        dst[3 * k + a] = u[a];
+                    }
+                    if (ok[i] && own[i] && ownLayer) {
+                        const int64_t g = goff[i] + plf * lf;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            rOut[3 * g + a] = u[a];
+                            const double ud = (double)u[a];
+                            rr += ud * ud;
+                            dr += (double)FineW<T>::get(dinvL, 3 * g + a) * ud * ud;
+                        }
+                    }
+                }
+            }
+        };
+        // (a second register set fetching two layers ahead measured slower
+        // in FP32: 568 -> 679 us per launch on the 125 M-DOF ellipsoids)
+        T vA[kRPer][6];
+        bool okA[kRPer];
+        fetch_flags(lfBeg);
+        fetch(lfBeg, vA, okA);
+        stash(sh[0], lfBeg, vA, okA);
+        fetch_flags(lfBeg + 1);
+        fetch(lfBeg + 1, vA, okA);
+        fetch_flags(lfBeg + 2);
+        __syncthreads();
+        T acc[3] = {0, 0, 0};
+        int buf = 0;
+        // layer lf from LDS; `cur` holds layer lf + 1, which is stashed here,
+        // and then receives layer lf + 2
+        auto step = [&](int lf, T (&cur)[kRPer][6], bool (&okc)[kRPer]) {
+            const T *lay = sh[buf];
+            T P[3] = {0, 0, 0};
+#pragma unroll
+            for (int dc = -1; dc <= 1; ++dc)
+#pragma unroll
+                for (int dr2 = -1; dr2 <= 1; ++dr2) {
+                    const T wgt = T((dc == 0 ? 2 : 1) * (dr2 == 0 ? 2 : 1)) * T(0.25);
+                    const T *src = lay + 3 * ((2 * ci + 1 + dr2) + kRFR * (2 * cj + 1 + dc));
+                    P[0] += wgt * src[0];
+                    P[1] += wgt * src[1];
+                    P[2] += wgt * src[2];
+                }
+            const bool even = ((lf - lfBeg) & 1) == 1;  // lfBeg = 2L0-1 is odd
+            if (even) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) acc[a] += P[a];
+            } else {
+                T nx[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    acc[a] += T(0.5) * P[a];
+                    nx[a] = T(0.5) * P[a];
+                }
+                const int L = (lf - 1) / 2;
+                if (lf > lfBeg && mine && L >= L0 && L < L1) {
+                    const int64_t G = Rc + (int64_t)NYc * Cc + plc * L;
+                    const bool solid = cflag[G] & kSolid;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) rc[3 * G + a] = solid ? acc[a] : T(0);
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) acc[a] = nx[a];
+            }
+            if (lf < lfEnd) {
+                stash(sh[buf ^ 1], lf + 1, cur, okc);
+                if (lf + 2 <= lfEnd) {
+                    fetch(lf + 2, cur, okc);
+                    fetch_flags(lf + 3);
+                }
+                __syncthreads();
+                buf ^= 1;
+            }
+        };
+        for (int lf = lfBeg; lf <= lfEnd; ++lf) step(lf, vA, okA);
+        __syncthreads();  // the next tile restages both buffers
+    }
+    block_sum2(rr, dr);
+    if (threadIdx.x == 0) {
+        partRR[vb.b] = rr;
+        partDR[vb.b] = dr;
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_update_restrict(const T *__restrict__ rf, const T *__restrict__ q,
+                                                         T *__restrict__ rOut,
+                                                         const typename FineW<T>::D *__restrict__ dinvL, Grid gf,
+                                                         T *__restrict__ rc, Grid gc, const uint8_t *__restrict__ cflag,
+                                                         int czc, int tilesR, int tilesC, int64_t nTiles,
+                                                         const int32_t *__restrict__ tiles,
+                                                         const uint8_t *__restrict__ fflag,
+                                                         double *__restrict__ partRR, double *__restrict__ partDR,
+                                                         const Ctl *ctl) {
+    update_restrict_body<T>(VB::self(), rf, q, rOut, dinvL, gf, rc, gc, cflag, czc, tilesR, tilesC, nTiles, tiles,
+                            fflag, partRR, partDR, ctl);
+}
+
 // Component a of (P x_c)(n), span S, at any fine node n (the caller zeroes
 // void nodes).  Branch-free: per axis the two coarse neighbours are q and
 // q + (m > 0) with weights (S - m)/S and m/S, so all eight loads are in the
@@ -1181,28 +1389,6 @@ __device__ __forceinline__ double block_allsum(double v) {
     }
     __syncthreads();
     return sh[16];
-}
-
-__device__ __forceinline__ void block_sum2(double &a, double &b) {
-    __shared__ double sh[2][16];
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_down(a, o, 64);
-        b += __shfl_down(b, o, 64);
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        sh[0][w] = a;
-        sh[1][w] = b;
-    }
-    __syncthreads();
-    const int nw = (blockDim.x + 63) >> 6;
-    a = (threadIdx.x < nw) ? sh[0][threadIdx.x] : 0.0;
-    b = (threadIdx.x < nw) ? sh[1][threadIdx.x] : 0.0;
-    if (w == 0)
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_down(a, o, 64);
-            b += __shfl_down(b, o, 64);
-        }
 }
 
 // ||r||^2 and ||f||^2 of a reliable update from their per-block partials, in
@@ -1828,6 +2014,24 @@ template <class T> struct OpRes2 {
                            a.czc, a.tilesR, a.tilesC, a.tiles, a.fflag, a.ctl);
     }
 };
+template <class T> struct UpdResA {
+    const T *rf; const T *q; T *rOut; const typename FineW<T>::D *dinvL; Grid gf; T *rc; Grid gc;
+    const uint8_t *cflag; int czc, tilesR, tilesC; int64_t nTiles; const int32_t *tiles; const uint8_t *fflag;
+    double *partRR; double *partDR; const Ctl *ctl;
+};
+template <class T> struct OpUpdRes {
+    using A = UpdResA<T>;
+    static constexpr int kThreads = 256, kOcc = 0;
+    __device__ static void run(VB vb, const A &a) {
+        update_restrict_body<T>(vb, a.rf, a.q, a.rOut, a.dinvL, a.gf, a.rc, a.gc, a.cflag, a.czc, a.tilesR,
+                                a.tilesC, a.nTiles, a.tiles, a.fflag, a.partRR, a.partDR, a.ctl);
+    }
+    static void single(const A &a, int grid, size_t shm, hipStream_t s) {
+        hipLaunchKernelGGL(k_update_restrict<T>, dim3(grid), dim3(256), shm, s, a.rf, a.q, a.rOut, a.dinvL, a.gf,
+                           a.rc, a.gc, a.cflag, a.czc, a.tilesR, a.tilesC, a.nTiles, a.tiles, a.fflag, a.partRR,
+                           a.partDR, a.ctl);
+    }
+};
 template <class T> struct ProA {
     const T *xc; Grid gc; T *xf; NodeDiv df; const T *rf; const T *dinv; const uint8_t *fflag; int finest;
     Span dofs; double *partial; const Ctl *ctl;
@@ -2117,6 +2321,9 @@ __global__ void k_diag0(const double *__restrict__ Ke, const double *__restrict_
 
 // The mixed solve's bfloat16 weights in span order (entry i of the lattice at
 // its span position t).
+__global__ void k_pack_w16_lattice(const float *__restrict__ dinv32, uint16_t *__restrict__ dinv16, Span S) {
+    FOR_SPAN(i, S) dinv16[i] = to_bf16(dinv32[i]);
+}
 __global__ void k_pack_w16(const float *__restrict__ dinv32, uint16_t *__restrict__ dinv16, Span S) {
     FOR_SPAN_T(i, t, S) dinv16[t] = to_bf16(dinv32[i]);
 }
@@ -2875,6 +3082,18 @@ bool mode_coefficients(const double *Ke, double *km9) {
     return true;
 }
 
+// Whether the restriction into level k runs the LDS-tiled span-2 kernel.
+template <class T>
+bool restrict_tiled(const sgld_ctx *C, int k) {
+    const Level &Cc = C->lv[k];
+    // (restrictTiled 2: tiled into level 1 only, node-centric below, one
+    // kernel per level for every instance of a merged multi-instance solve)
+    const bool tiled = C->restrictTiled < 0
+                           ? Cc.g.nodes() >= (sizeof(T) == 8 ? C->restrictTiledMin : C->restrictTiledMin32)
+                           : C->restrictTiled == 2 ? k == 1 : C->restrictTiled != 0;
+    return Cc.span == 2 && tiled;
+}
+
 template <class T>
 int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     Level &F = C->lv[k - 1], &Cc = C->lv[k];
@@ -2884,13 +3103,8 @@ int enqueue_restrict(sgld_ctx *C, int k, const T *rf, T *rc, const Ctl *ctl) {
     // FP32 only beyond the Infinity Cache (256x128x128 first level, FP64 /
     // FP32: tiled 31 / 17 us, node-centric 35 / 13 us; 512^3 ellipsoids FP32
     // node-centric 222 us for 0.56 GB)
-    // (restrictTiled 2: tiled into level 1 only, node-centric below, one
-    // kernel per level for every instance of a merged multi-instance solve)
-    const bool tiled = C->restrictTiled < 0
-                           ? Cc.g.nodes() >= (sizeof(T) == 8 ? C->restrictTiledMin : C->restrictTiledMin32)
-                           : C->restrictTiled == 2 ? k == 1 : C->restrictTiled != 0;
     const NodeDiv dc = NodeDiv::make(Cc.g);
-    if (Cc.span == 2 && tiled) {
+    if (restrict_tiled<T>(C, k)) {
         const int tilesR = (int)((Cc.g.NY() + kRR - 1) / kRR), tilesC = (int)((Cc.g.NX() + kRC - 1) / kRC);
         const int czc = C->restrictCz, tilesZ = (int)((Cc.g.NZ() + czc - 1) / czc);
         // irregular domains: only the tiles holding a solid coarse node (the
@@ -2929,7 +3143,7 @@ int enqueue_prolong(sgld_ctx *C, int k, const T *xc, T *xf, const T *rf, double 
 
 // z = M r (Solving_Vcycle.m), partial z.r into `partial` when non-null.
 template <class T>
-int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl);
+int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl, int kFirst = 1);
 
 template <class T>
 int enqueue_vcycle(sgld_ctx *C, const T *r, T *z, double *partial, const Ctl *ctl, int *nbOut) {
@@ -2987,13 +3201,14 @@ TailArgs<T> tail_args(sgld_ctx *C, int t0) {
 // for levels >= 2): restrictions down to the coarsest, its dense solve, and
 // prolongation + smoothing back up to level 2, whose x_2 stays in lv[1]; the
 // small levels run as the fused tail.  partialX2 (if not null) receives the
-// partials of x_2 . r_2 (count in *nbX2).
+// partials of x_2 . r_2 (count in *nbX2).  kFirst 2: level 2's residual is
+// already there (fused with the residual update), r is not read.
 template <class T>
-int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl) {
+int enqueue_vcycle_coarse(sgld_ctx *C, const T *r, double *partialX2, int *nbX2, const Ctl *ctl, int kFirst) {
     const int L = C->L, t0 = tail_top(C);
     const int kLast = t0 > 0 ? t0 : L - 1;  // last level reached by single restrictions
-    const T *rf = r;
-    for (int k = 1; k <= kLast; ++k) {
+    const T *rf = kFirst > 1 ? lvl_r<T>(C->lv[kFirst - 1]) : r;
+    for (int k = kFirst; k <= kLast; ++k) {
         if (k == 1) SGLD_TRY(kt_begin(C, sizeof(T) == 8 ? KT_RES64 : KT_RES32, "first restriction"));
         SGLD_TRY(enqueue_restrict<T>(C, k, rf, lvl_r<T>(C->lv[k]), ctl));
         if (k == 1) SGLD_TRY(kt_end(C));
@@ -3065,8 +3280,29 @@ int enqueue_rho(sgld_ctx *C, const T *r, int op) {
 // recursive residual, and the coarse V-cycle of the new r with rho' and beta.
 // Finest-level traffic: 6 vector passes + 1/8 for the direction update, K.u,
 // 4 for the residual update, 1 + 1/8 for the first restriction.
+// Weights of the fused residual update and restriction: the finest Jacobi
+// weights (FP64, or the FP32 inner solve's bfloat16 ones) in lattice order.
+template <class T> const typename FineW<T>::D *fine_dinv_lattice(sgld_ctx *C);
+template <> const double *fine_dinv_lattice<double>(sgld_ctx *C) { return C->lv[0].dinv.p; }
+template <> const uint16_t *fine_dinv_lattice<float>(sgld_ctx *C) {
+    return C->full ? C->lv[0].dinv16.p : C->dinv16L.p;
+}
+
+// Whether the iterations fuse the residual update with the first restriction:
+// a span-2 LDS-tiled first restriction, and not while a multi-instance
+// iteration is recorded (it replays one iteration, the fused form alternates
+// two residual vectors).
 template <class T>
-int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
+bool fuse_res_applies(const sgld_ctx *C) {
+    return (sizeof(T) == 8 ? C->fuseRes : C->fuseRes32) && !C->rec && C->L >= 2 && restrict_tiled<T>(C, 1);
+}
+template <class T>
+bool fuse_res(const sgld_ctx *C) {  // and its lattice-order weights exist
+    return fuse_res_applies<T>(C) && (sizeof(T) == 8 || C->full || C->dinv16L.p);
+}
+
+template <class T>
+int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q, T *rOut = nullptr) {
     const Span S = dof_span(C);
     Ctl *ctl = C->ctl.p;
     Level &l0 = C->lv[0], &l1 = C->lv[1];
@@ -3104,6 +3340,29 @@ int enqueue_iteration(sgld_ctx *C, T *x, T *r, T *p, T *q) {
     int nb = 0;
     SGLD_TRY(enqueue_kbyu<T>(C, p, q, slot(C, 0), ctl, &nb));
     SGLD_TRY(fin(C, slot(C, 0), nb, FIN_PQ));
+    if (rOut && fuse_res<T>(C)) {
+        // r' = r - lambda q into rOut, fused with its restriction to level 2
+        Level &F = C->lv[0];
+        const int czc = C->restrictCz;
+        const int tilesR = (int)((l1.g.NY() + kRR - 1) / kRR), tilesC = (int)((l1.g.NX() + kRC - 1) / kRC);
+        const int tilesZ = (int)((l1.g.NZ() + czc - 1) / czc);
+        const bool list = !C->full && l1.rtilesCz == czc;
+        const int64_t nTiles = list ? l1.nRtiles : (int64_t)tilesR * tilesC * tilesZ;
+        const int nbf = (int)std::max<int64_t>(1, std::min<int64_t>(nTiles, kRedBlocks));
+        SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32,
+                          f64 ? "k_update_restrict<double>" : "k_update_restrict<float>"));
+        SGLD_TRY(emit<OpUpdRes<T>>(C, ST_UPD, nbf, 0,
+                                   UpdResA<T>{(const T *)r, (const T *)q, rOut, fine_dinv_lattice<T>(C), F.g,
+                                              lvl_r<T>(l1), l1.g, l1.nflag.p, czc, tilesR, tilesC, nTiles,
+                                              list ? (const int32_t *)l1.rtiles.p : nullptr,
+                                              C->full || !C->restrictSkipVoid ? nullptr : F.nflag.p, slot(C, 1),
+                                              slot(C, 3), ctl}));
+        SGLD_TRY(kt_end(C));
+        SGLD_TRY(fin(C, slot(C, 1), nbf, FIN_RR));
+        int nb2 = 0;
+        SGLD_TRY(enqueue_vcycle_coarse<T>(C, rOut, slot(C, 2), &nb2, ctl, 2));
+        return fin(C, slot(C, 2), nb2, FIN_RHO, slot(C, 3), nbf);
+    }
     SGLD_TRY(kt_begin(C, f64 ? KT_UPD64 : KT_UPD32, stream_name(f64, nt, 2)));
     const UpdA<T> ua{r, (const T *)q, fine_dinv<T>(C), S, slot(C, 1), slot(C, 3), ctl};
     SGLD_TRY(nt ? (emit<OpUpd<T, true>>(C, ST_UPD, nbv, 0, ua)) : (emit<OpUpd<T, false>>(C, ST_UPD, nbv, 0, ua)));
@@ -3229,8 +3488,10 @@ int mixed_begin_finish(sgld_ctx *C, SolveState &S) {
     return SGLD_OK;
 }
 
+int ensure_fused_vectors(sgld_ctx *C, int precision);
 int pcg_begin_enqueue(sgld_ctx *C, SolveState &S) {
     S.finished = false;
+    SGLD_TRY(ensure_fused_vectors(C, S.precision));
     return S.precision == SGLD_PREC_FP64 ? fp64_begin_enqueue(C, S) : mixed_begin_enqueue(C, S);
 }
 int pcg_begin_finish(sgld_ctx *C, SolveState &S) {
@@ -3245,13 +3506,45 @@ int pcg_begin(sgld_ctx *C, SolveState &S) {
 // `batch` iterations.  The last iteration's y += lambda p stays pending (it is
 // folded into the next iteration's first pass); pcg_after_batch applies it
 // when the solve stops or before a reliable update.
+// With the fused residual update (fuse_res) iteration i reads the residual
+// from r and writes it to rB for even i, the other way round for odd i; the
+// batch is even, so a batch whose iterations all ran leaves it in r (a solve
+// that stops, or whose FP32 stretch stops, inside a batch does not go on from
+// the recursive residual: it has finished, or a reliable update rewrites r).
+int ensure_fused_vectors(sgld_ctx *C, int precision) {
+    const bool f64 = precision == SGLD_PREC_FP64;
+    if (!(f64 ? fuse_res_applies<double>(C) : fuse_res_applies<float>(C))) return SGLD_OK;
+    if (!f64 && !C->full && !C->dinv16L.p) {  // lattice-order bfloat16 weights (set-up once, kept)
+        const Span V = dof_span(C);
+        SGLD_TRY(C->dinv16L.alloc(C->nDof()));
+        hipLaunchKernelGGL(k_pack_w16_lattice, dim3(grid_blocks(V.count)), dim3(256), 0, C->s,
+                           (const float *)C->lv[0].dinv32.p, C->dinv16L.p, V);
+        SGLD_HIP(hipGetLastError());
+    }
+    if (f64 && !C->rB.p) {
+        SGLD_TRY(C->rB.alloc(C->nDof()));
+        SGLD_TRY(C->rB.zero(C->s));  // void entries are never written
+    }
+    if (!f64 && !C->r32B.p) {
+        SGLD_TRY(C->r32B.alloc(C->nDof()));
+        SGLD_HIP(hipMemsetAsync(C->r32B.p, 0, sizeof(float) * C->r32B.n, C->s));
+    }
+    return SGLD_OK;
+}
+
 int pcg_enqueue_batch(sgld_ctx *C, const SolveState &S) {
     for (int i = 0; i < S.batch; ++i) {
         C->kt.curIter = i;
-        if (S.precision == SGLD_PREC_FP64)
-            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, C->r.p, C->p.p, C->q.p));
-        else
-            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, C->r32.p, C->p32.p, C->q32.p));
+        const bool odd = i & 1;
+        if (S.precision == SGLD_PREC_FP64) {
+            double *rIn = odd && C->rB.p ? C->rB.p : C->r.p;
+            double *rOut = C->rB.p ? (odd ? C->r.p : C->rB.p) : nullptr;
+            SGLD_TRY(enqueue_iteration<double>(C, C->x.p, rIn, C->p.p, C->q.p, rOut));
+        } else {
+            float *rIn = odd && C->r32B.p ? C->r32B.p : C->r32.p;
+            float *rOut = C->r32B.p ? (odd ? C->r32.p : C->r32B.p) : nullptr;
+            SGLD_TRY(enqueue_iteration<float>(C, C->x32.p, rIn, C->p32.p, C->q32.p, rOut));
+        }
     }
     C->kt.curIter = -1;
     SGLD_HIP(hipMemcpyAsync(C->hctl, C->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, C->s));
@@ -3397,6 +3690,9 @@ int pcg_solve_loop(sgld_ctx *C, SolveState &S) {
             if (f > 0.0 && f < 1.0) {
                 const double k = std::ceil(std::log(st.tol / st.res) / std::log(f));
                 if (k >= 0.0 && k + 1.0 < full) S.batch = std::max(1, (int)k + 1);
+                // the fused residual update alternates two vectors: keep batches even
+                const bool pingPong = S.precision == SGLD_PREC_FP64 ? C->rB.p != nullptr : C->r32B.p != nullptr;
+                if (pingPong) S.batch = std::min(full, (S.batch + 1) & ~1);
             }
         }
         // a reliable update starts a new stretch (new reference norm)
@@ -3579,6 +3875,8 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN32")) C->restrictTiledMin32 = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_FUSE_RES")) C->fuseRes = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_FUSE_RES32")) C->fuseRes32 = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_BRESTRICT_TILED")) C->bRestrictTiled = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
@@ -3923,6 +4221,11 @@ int assemble_impl(sgld_ctx *C, double omega) {
         hipLaunchKernelGGL(k_pack_w16, dim3(grid_blocks(V.count)), dim3(256), 0, C->s, (const float *)l0.dinv32.p,
                            l0.dinv16.p, V);
         SGLD_HIP(hipGetLastError());
+        if (C->dinv16L.p) {  // the same weights in lattice order (fused residual update), refreshed
+            hipLaunchKernelGGL(k_pack_w16_lattice, dim3(grid_blocks(V.count)), dim3(256), 0, C->s,
+                               (const float *)l0.dinv32.p, C->dinv16L.p, V);
+            SGLD_HIP(hipGetLastError());
+        }
     }
     DevBuf<double> KsPrev, KsCur;
     for (int k = 1; k < C->L; ++k) {

@@ -122,6 +122,18 @@ struct sgld_ctx {
     int64_t nCoarseFree = 0;
     sgld::DevBuf<double> b, x, r, z, p, p2, q;
     sgld::DevBuf<float> b32, r32, z32, p32, p32b, q32, x32;
+    // Residual update fused with the first restriction (fuseRes: FP64,
+    // fuseRes32: FP32 inner solve): the iterations alternate r / rB (FP64)
+    // and r32 / r32B (FP32), an even batch ending in r / r32; allocated on the
+    // first solve that fuses.  dinv16L: the FP32 inner solve's bfloat16
+    // finest weights in lattice order (irregular domains; lv[0].dinv16 holds
+    // them in span order).  The FP32 form measured no faster than the two
+    // passes on the 125 M-DOF ellipsoids (568 against 376 + 199 us), the FP64
+    // form 1,040 against 832 + 386 us, so FP32 is off by default.
+    bool fuseRes = true, fuseRes32 = false;
+    sgld::DevBuf<double> rB;
+    sgld::DevBuf<float> r32B;
+    sgld::DevBuf<uint16_t> dinv16L;
     sgld::DevBuf<double> partial;
     sgld::DevBuf<double> pdeWork;  // PDE filter work vectors, allocated on first use and kept
     sgld::DevBuf<double> filterWork;  // density filter: lattice copy of x and the weights, kept

@@ -358,6 +358,53 @@ def test_coarsest_solve_is_dense_cholesky_solve(case):
     assert abs(a - b) <= 1e-12 * abs(a)
 
 
+@pytest.mark.parametrize("case", ["cuboid", "holes", "ellipsoids"])
+def test_fused_residual_update_and_restriction(case, monkeypatch):
+    """With the LDS-tiled first restriction the iterations fuse the residual
+    update into it and alternate two residual vectors: both precisions against
+    the oracle, run to run identical, and within rounding of the unfused pair
+    of passes (SGLD_FUSE_RES=0; SGLD_FUSE_RES32 switches the FP32 form, off
+    by default)."""
+    print("case: fused residual update", case, flush=True)
+    if case == "ellipsoids":
+        d = problems.cfg2(32, 12, (4, 10), 3, 20, 2, 16, seed=7)
+        vox, levels, bc = d["voxels"], 3, (d["F"], d["fixed"])
+    else:
+        vox = problems.cuboid(48, 24, 24) if case == "cuboid" else holes(48, 24, 24, 3)
+        levels, bc = 4, None
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("SGLD_FUSE_RES", fuse)
+        monkeypatch.setenv("SGLD_FUSE_RES32", fuse)
+        g, m, F, fixed, E = build_pair(vox, levels, bc=bc)
+        g.set_schedule("tiled", 12, "tiled")  # the tiled restriction on every level
+        Uo, itso, hist, rc = m.pcg(F, 0.6, 1e-8, 1000)
+        assert rc == 0
+        for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+            g.kernel_timing(True)
+            U, its, rr = g.pcg(F, tol=1e-8, maxIT=1000, precision=prec)
+            kt = g.kernel_times()
+            g.kernel_timing(False)
+            name = kt["update_r64" if prec == solver.PREC_FP64 else "update_r32"]["kernel"]
+            assert ("k_update_restrict" in name) == (fuse == "1"), name
+            true = np.linalg.norm(F - m.kbyu(U)) / np.linalg.norm(F)
+            c, co = F @ U, F @ Uo
+            print("  fuse %s precision %d: its %d (oracle %d), true rel-res %.2e" % (fuse, prec, its, itso, true),
+                  flush=True)
+            assert g.converged and rr <= 1e-8 and true <= 1.01e-8
+            assert abs(c - co) <= (1e-9 if prec == solver.PREC_FP64 else 1e-8) * abs(co)
+            if prec == solver.PREC_FP64:
+                assert abs(its - itso) <= 1
+            U2, its2, _ = g.pcg(F, tol=1e-8, maxIT=1000, precision=prec)
+            assert its2 == its and np.array_equal(U2, U)
+            out[(fuse, prec)] = (U, its)
+        g.close()
+    for prec in (solver.PREC_FP64, solver.PREC_MIXED):
+        (U1, i1), (U0, i0) = out[("1", prec)], out[("0", prec)]
+        assert abs(i1 - i0) <= 1
+        assert np.linalg.norm(U1 - U0) <= 1e-7 * np.linalg.norm(U0)
+
+
 @pytest.mark.parametrize("case", ["cuboid", "holes"])
 def test_kbyu_and_pcg_run_to_run_identical(case):
     """K.u sums every node's element contributions in a fixed order, so FP64
