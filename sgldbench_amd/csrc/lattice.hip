@@ -3875,8 +3875,6 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN")) C->restrictTiledMin = std::atoll(env);
     if (const char *env = std::getenv("SGLD_RESTRICT_TILED_MIN32")) C->restrictTiledMin32 = std::atoll(env);
     if (const char *env = std::getenv("SGLD_TAIL")) C->tailOn = std::atoi(env) != 0;
-    if (const char *env = std::getenv("SGLD_FUSE_RES")) C->fuseRes = std::atoi(env) != 0;
-    if (const char *env = std::getenv("SGLD_FUSE_RES32")) C->fuseRes32 = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_COLS")) C->dirCols = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_BRESTRICT_TILED")) C->bRestrictTiled = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_CZ")) C->restrictCz = std::max(1, std::min(16, std::atoi(env)));
@@ -4049,6 +4047,16 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     // it reads survives to the next pass); SGLD_STREAM_NT=0/1 forces either.
     for (int q = 0; q < 2; ++q) C->streamNT[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
     if (const char *env = std::getenv("SGLD_STREAM_NT")) C->streamNT[0] = C->streamNT[1] = std::atoi(env) != 0;
+    // The residual update fused with the first restriction saves a read of r
+    // where r comes from HBM and the separate passes run over chunk and tile
+    // lists: irregular domains with FP64 vectors beyond the same share (125
+    // M-DOF ellipsoids: FP64 solve 1529 -> 1468 ms).  Within the share the
+    // two passes read from the Infinity Cache and the fused tile march is
+    // slower (256x128x128: 46.6 -> 49.7 ms), and on a full 256^3 cuboid it
+    // gained nothing (16 FP64 solves 3.025 -> 3.055 s).
+    C->fuseRes = !C->full && (double)nd * 8.0 > 128.0 * (1 << 20);
+    if (const char *env = std::getenv("SGLD_FUSE_RES")) C->fuseRes = std::atoi(env) != 0;
+    if (const char *env = std::getenv("SGLD_FUSE_RES32")) C->fuseRes32 = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_XCD")) C->dirXcd = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_RESTRICT_SKIP_VOID")) C->restrictSkipVoid = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_VEC_BLOCKS"))
