@@ -100,6 +100,7 @@ template <> struct FineW<double> {
     using D = double;
     template <bool NT = false>
     __device__ __forceinline__ static double get(const double *p, int64_t i) { return ld_stream<NT>(&p[i]); }
+    __device__ __forceinline__ static double value(double raw) { return raw; }
 };
 template <> struct FineW<float> {
     using D = uint16_t;
@@ -107,6 +108,7 @@ template <> struct FineW<float> {
     __device__ __forceinline__ static float get(const uint16_t *p, int64_t i) {
         return __uint_as_float((uint32_t)ld_stream<NT>(&p[i]) << 16);
     }
+    __device__ __forceinline__ static float value(uint16_t raw) { return __uint_as_float((uint32_t)raw << 16); }
 };
 
 // Grid-stride passes over the entries of a Span (all entries, or the listed
@@ -932,6 +934,9 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
         }
     };
     const int lfBeg = 2 * L0 - 1, lfEnd = 2 * (L1 - 1) + 1;  // fine layers [lfBeg, lfEnd]
+    // flag of the coarse node written next (loaded ahead: used where it is
+    // loaded, it cost each coarse layer a memory round trip)
+    uint8_t cfl = (mine && L0 < L1) ? cflag[Rc + (int64_t)NYc * Cc + plc * L0] : (uint8_t)0;
     fetch_flags(lfBeg);
     fetch(lfBeg);
     stash(sh[0]);
@@ -968,7 +973,8 @@ __device__ __forceinline__ void restrict2_tiled_body(VB vb, const T *__restrict_
             const int L = (lf - 1) / 2;  // coarse layer completed by fine layer 2L+1
             if (lf > lfBeg && mine && L >= L0 && L < L1) {
                 const int64_t G = Rc + (int64_t)NYc * Cc + plc * L;
-                const bool solid = cflag[G] & kSolid;
+                const bool solid = cfl & kSolid;
+                if (L + 1 < L1) cfl = cflag[G + plc];  // the next coarse layer's flag, two fine layers ahead
 #pragma unroll
                 for (int a = 0; a < 3; ++a) rc[3 * G + a] = solid ? acc[a] : T(0);
             }
@@ -1063,20 +1069,26 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
             for (int i = 0; i < kRPer; ++i)
                 fl[i] = (fflag && goff[i] >= 0 && lf >= 0 && lf < NZf) ? fflag[goff[i] + plf * lf] : (uint8_t)kSolid;
         };
-        auto fetch = [&](int lf, T (&v)[kRPer][6], bool (&ok)[kRPer]) {
+        // r, q and, with them (a layer ahead, not where they are used), the
+        // weights: loaded in the stash they cost each layer a second memory
+        // round trip
+        using D = typename FineW<T>::D;
+        auto fetch = [&](int lf, T (&v)[kRPer][6], D (&wv)[kRPer][3], bool (&ok)[kRPer]) {
 #pragma unroll
             for (int i = 0; i < kRPer; ++i) {
                 ok[i] = goff[i] >= 0 && lf >= 0 && lf < NZf && lf <= lfEnd && (fl[i] & kSolid);
                 const int64_t o = ok[i] ? 3 * (goff[i] + plf * lf) : 0;
+                const int64_t ow = ok[i] && own[i] && lf >= 2 * L0 ? o : 0;  // weights of owned nodes only
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     v[i][a] = rf[o + a];
                     v[i][3 + a] = q[o + a];
+                    wv[i][a] = dinvL[ow + a];
                 }
             }
         };
         // r' of layer lf (fetched into v) into LDS; the owned nodes to rOut
-        auto stash = [&](T *dst, int lf, const T (&v)[kRPer][6], const bool (&ok)[kRPer]) {
+        auto stash = [&](T *dst, int lf, const T (&v)[kRPer][6], const D (&wv)[kRPer][3], const bool (&ok)[kRPer]) {
             const bool ownLayer = lf >= 2 * L0;  // the layer below the tile's belongs to the tile below
 #pragma unroll
             for (int i = 0; i < kRPer; ++i) {
@@ -1095,7 +1107,7 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
                             rOut[3 * g + a] = u[a];
                             const double ud = (double)u[a];
                             rr += ud * ud;
-                            dr += (double)FineW<T>::get(dinvL, 3 * g + a) * ud * ud;
+                            dr += (double)FineW<T>::value(wv[i][a]) * ud * ud;
                         }
                     }
                 }
@@ -1104,19 +1116,21 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
         // (a second register set fetching two layers ahead measured slower
         // in FP32: 568 -> 679 us per launch on the 125 M-DOF ellipsoids)
         T vA[kRPer][6];
+        D wA[kRPer][3];
         bool okA[kRPer];
+        uint8_t cfl = (mine && L0 < L1) ? cflag[Rc + (int64_t)NYc * Cc + plc * L0] : (uint8_t)0;
         fetch_flags(lfBeg);
-        fetch(lfBeg, vA, okA);
-        stash(sh[0], lfBeg, vA, okA);
+        fetch(lfBeg, vA, wA, okA);
+        stash(sh[0], lfBeg, vA, wA, okA);
         fetch_flags(lfBeg + 1);
-        fetch(lfBeg + 1, vA, okA);
+        fetch(lfBeg + 1, vA, wA, okA);
         fetch_flags(lfBeg + 2);
         __syncthreads();
         T acc[3] = {0, 0, 0};
         int buf = 0;
         // layer lf from LDS; `cur` holds layer lf + 1, which is stashed here,
         // and then receives layer lf + 2
-        auto step = [&](int lf, T (&cur)[kRPer][6], bool (&okc)[kRPer]) {
+        auto step = [&](int lf, T (&cur)[kRPer][6], D (&wc)[kRPer][3], bool (&okc)[kRPer]) {
             const T *lay = sh[buf];
             T P[3] = {0, 0, 0};
 #pragma unroll
@@ -1143,7 +1157,8 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
                 const int L = (lf - 1) / 2;
                 if (lf > lfBeg && mine && L >= L0 && L < L1) {
                     const int64_t G = Rc + (int64_t)NYc * Cc + plc * L;
-                    const bool solid = cflag[G] & kSolid;
+                    const bool solid = cfl & kSolid;
+                    if (L + 1 < L1) cfl = cflag[G + plc];  // two fine layers ahead
 #pragma unroll
                     for (int a = 0; a < 3; ++a) rc[3 * G + a] = solid ? acc[a] : T(0);
                 }
@@ -1151,16 +1166,16 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
                 for (int a = 0; a < 3; ++a) acc[a] = nx[a];
             }
             if (lf < lfEnd) {
-                stash(sh[buf ^ 1], lf + 1, cur, okc);
+                stash(sh[buf ^ 1], lf + 1, cur, wc, okc);
                 if (lf + 2 <= lfEnd) {
-                    fetch(lf + 2, cur, okc);
+                    fetch(lf + 2, cur, wc, okc);
                     fetch_flags(lf + 3);
                 }
                 __syncthreads();
                 buf ^= 1;
             }
         };
-        for (int lf = lfBeg; lf <= lfEnd; ++lf) step(lf, vA, okA);
+        for (int lf = lfBeg; lf <= lfEnd; ++lf) step(lf, vA, wA, okA);
         __syncthreads();  // the next tile restages both buffers
     }
     block_sum2(rr, dr);
@@ -4054,7 +4069,11 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     // two passes read from the Infinity Cache and the fused tile march is
     // slower (256x128x128: 46.6 -> 49.7 ms), and on a full 256^3 cuboid it
     // gained nothing (16 FP64 solves 3.025 -> 3.055 s).
+    // The FP32 form gains less (bfloat16 weights, half the bytes per node):
+    // 575 -> 533 us per iteration on the same workload, with its weights and
+    // the coarse flags loaded a layer ahead.
     C->fuseRes = !C->full && (double)nd * 8.0 > 128.0 * (1 << 20);
+    C->fuseRes32 = !C->full && (double)nd * 4.0 > 128.0 * (1 << 20);
     if (const char *env = std::getenv("SGLD_FUSE_RES")) C->fuseRes = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_FUSE_RES32")) C->fuseRes32 = std::atoi(env) != 0;
     if (const char *env = std::getenv("SGLD_DIR_XCD")) C->dirXcd = std::atoi(env) != 0;

@@ -127,10 +127,7 @@ struct sgld_ctx {
     // and r32 / r32B (FP32), an even batch ending in r / r32; allocated on the
     // first solve that fuses.  dinv16L: the FP32 inner solve's bfloat16
     // finest weights in lattice order (irregular domains; lv[0].dinv16 holds
-    // them in span order).  The FP32 form measured no faster than the two
-    // passes on the 125 M-DOF ellipsoids (568 against 376 + 199 us), the FP64
-    // form 1,040 against 832 + 386 us, so FP32 is off by default; FP64 is on
-    // once its vectors exceed the Infinity Cache share (sgld_ctx_create).
+    // them in span order).  Both are set by size in sgld_ctx_create.
     bool fuseRes = false, fuseRes32 = false;
     sgld::DevBuf<double> rB;
     sgld::DevBuf<float> r32B;

@@ -363,8 +363,8 @@ def test_fused_residual_update_and_restriction(case, monkeypatch):
     """With the LDS-tiled first restriction the iterations fuse the residual
     update into it and alternate two residual vectors: both precisions against
     the oracle, run to run identical, and within rounding of the unfused pair
-    of passes (SGLD_FUSE_RES=0; SGLD_FUSE_RES32 switches the FP32 form, off
-    by default)."""
+    of passes (SGLD_FUSE_RES=0 and SGLD_FUSE_RES32=0; by default both forms
+    run only on large irregular domains)."""
     print("case: fused residual update", case, flush=True)
     if case == "ellipsoids":
         d = problems.cfg2(32, 12, (4, 10), 3, 20, 2, 16, seed=7)
