@@ -82,6 +82,24 @@ __device__ __forceinline__ V ld_stream(const V *p) {
     if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
+// The matching store, for outputs of a pass beyond the Infinity Cache that
+// the next pass streams from HBM: the caches then keep what the next pass
+// re-reads (125 M-DOF ellipsoids, mixed solve 778 -> 764 ms with the direction
+// and residual updates' stores nontemporal, of which the FP32 K.u after the
+// direction update 498 -> 466 us; the fused residual update's stores
+// 525 -> 511 us).  SGLD_NT_STORE=0 / SGLD_NT_STORE_FUSED=0 at build time:
+// ordinary stores.
+#ifndef SGLD_NT_STORE
+#define SGLD_NT_STORE 1
+#endif
+#ifndef SGLD_NT_STORE_FUSED
+#define SGLD_NT_STORE_FUSED 1
+#endif
+template <bool NT, class V>
+__device__ __forceinline__ void st_stream(V *p, V v) {
+    if constexpr (NT && SGLD_NT_STORE) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 // Entry of a finest-level weight or FP32 correction vector: the FP64 solve's
 // vectors are lattice vectors (entry i of the lattice at i); the bfloat16
@@ -698,6 +716,8 @@ __device__ __forceinline__ void kbyu_owner_body(VB vb, const T *__restrict__ u, 
         auto flush = [&](int L) {
             const __amdgpu_buffer_rsrc_t yr = buf_rsrc(y + sBase + 3 * pl * L, vBytes);
 #pragma unroll
+            // (nontemporal stores here measured slower: 125 M-DOF ellipsoids
+            // FP32 467 -> 579 us)
             for (int q = 0; q < kON / 256; ++q) buf_store(ost[L & 1][t + 256 * q], yr, oOff[q]);
         };
         auto step = [&](int el, T (&ready)[kStageSlots], T (&spare)[kStageSlots]) {
@@ -1104,7 +1124,7 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
                         const int64_t g = goff[i] + plf * lf;
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
-                            rOut[3 * g + a] = u[a];
+                            st_stream<SGLD_NT_STORE_FUSED != 0>(&rOut[3 * g + a], u[a]);
                             const double ud = (double)u[a];
                             rr += ud * ud;
                             dr += (double)FineW<T>::value(wv[i][a]) * ud * ud;
@@ -1305,8 +1325,8 @@ __device__ __forceinline__ void prolong_dir_body(VB vb, const T *__restrict__ xc
         const uint8_t f = fflag[n];
         const T po = ld_stream<NT>(&p[i]);
         const T z = ((f & kSolid) && !(f & (1 << a))) ? pv + sv : T(0);
-        x[k] = ld_stream<NT>(&x[k]) + lam * po;
-        p[i] = z + beta * po;
+        st_stream<NT>(&x[k], ld_stream<NT>(&x[k]) + lam * po);
+        st_stream<NT>(&p[i], z + beta * po);
     };
     if (xsplit) {
         FOR_SPAN_XCD_VT(i, t, dofs, vb) body(i, t);
@@ -1444,7 +1464,7 @@ __device__ __forceinline__ void update_r_body(VB vb, T *__restrict__ r, const T 
     double rr = 0.0, dr = 0.0;
     FOR_SPAN_VT(i, t, S, vb) {
         const T rn = ld_stream<NT>(&r[i]) - lt * ld_stream<NT>(&q[i]);
-        r[i] = rn;
+        st_stream<NT>(&r[i], rn);
         const double rd = (double)rn;
         rr += rd * rd;
         dr += (double)FineW<T>::template get<NT>(dinv, fine_pos<T>(i, t)) * rd * rd;
