@@ -4077,10 +4077,16 @@ extern "C" int sgld_ctx_create(sgld_ctx **out, int resX, int resY, int resZ, con
     SGLD_TRY(C->partial.alloc((int64_t)kRedSlots * kRedBlocks));
     SGLD_TRY(C->ctl.alloc(1));
     SGLD_TRY(element_data(C.get()));
-    // Streaming passes with nontemporal loads once one DOF vector outgrows
-    // half the 256 MB Infinity Cache (a pass streams 3-4 of them, so nothing
-    // it reads survives to the next pass); SGLD_STREAM_NT=0/1 forces either.
-    for (int q = 0; q < 2; ++q) C->streamNT[q] = (double)nd * (q ? 4.0 : 8.0) > 128.0 * (1 << 20);
+    // Streaming passes with nontemporal loads and stores once one DOF vector
+    // outgrows half the 256 MB Infinity Cache (a pass streams 3-4 of them, so
+    // nothing it reads survives to the next pass); FP64 from a quarter of it
+    // (measured: 256x128x128, 103 MB per FP64 vector, FP64 solve 46.9 -> 45.9
+    // ms; part_R256, 122 MiB, 58.1 -> 55.6 ms; in FP32 the same forms gained
+    // on the cuboid's 51 MB vectors, 29.9 -> 29.1 ms, but lost on
+    // part_R256's 61 MiB ones, 37.4 -> 41.0 ms).  SGLD_STREAM_NT=0/1 forces
+    // either.
+    C->streamNT[0] = (double)nd * 8.0 > 64.0 * (1 << 20);
+    C->streamNT[1] = (double)nd * 4.0 > 128.0 * (1 << 20);
     if (const char *env = std::getenv("SGLD_STREAM_NT")) C->streamNT[0] = C->streamNT[1] = std::atoi(env) != 0;
     // The residual update fused with the first restriction saves a read of r
     // where r comes from HBM and the separate passes run over chunk and tile
