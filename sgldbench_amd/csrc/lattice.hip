@@ -1124,7 +1124,10 @@ __device__ __forceinline__ void update_restrict_body(VB vb, const T *__restrict_
                         const int64_t g = goff[i] + plf * lf;
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
-                            st_stream<SGLD_NT_STORE_FUSED != 0>(&rOut[3 * g + a], u[a]);
+                            // nontemporal in FP32 (525 -> 511 us per launch);
+                            // in FP64 they wrote 1.41 GB instead of 1.21 GB
+                            // (partial lines) and ran slower (1,029 -> 1,042 us)
+                            st_stream<SGLD_NT_STORE_FUSED != 0 && sizeof(T) == 4>(&rOut[3 * g + a], u[a]);
                             const double ud = (double)u[a];
                             rr += ud * ud;
                             dr += (double)FineW<T>::value(wv[i][a]) * ud * ud;
