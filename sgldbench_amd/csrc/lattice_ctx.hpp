@@ -153,7 +153,11 @@ struct sgld_ctx {
     int64_t restrictTiledMin32 = (int64_t)1 << 22;
     bool tailOn = true;  // small levels of the V-cycle as the fused two-launch tail
     bool restrictSkipVoid = true;  // irregular domains: the tiled restriction stages no void fine node
-    int restrictCz = 2;  // coarse layers per block of the tiled span-2 restriction (2: 31 us, 4: 33 us on cfg 1)
+    // coarse layers per block of the tiled span-2 restriction and the fused
+    // residual update: 3 (125 M-DOF ellipsoids, mixed / FP64 solve 760 / 1444
+    // ms at 2, 755 / 1441 ms at 3; part_R256 FP64 55.6 -> 54.7 ms; the
+    // 256x128x128 cuboid within 1 %)
+    int restrictCz = 3;
     int vecBlocks = sgld::kVecBlocks;  // grid of the finest-level direction and residual updates
     bool dirXcd = true;  // direction update with one contiguous DOF range per XCD
     bool streamNT[2] = {false, false};  // [0] FP64 / [1] FP32 streaming passes with nontemporal loads
