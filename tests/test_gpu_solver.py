@@ -405,6 +405,33 @@ def test_fused_residual_update_and_restriction(case, monkeypatch):
         assert np.linalg.norm(U1 - U0) <= 1e-7 * np.linalg.norm(U0)
 
 
+def test_fused_residual_update_under_frequent_reliable_updates(monkeypatch):
+    """The alternating residual vectors of the fused pass when FP32 stretches
+    stop inside a batch and at odd iteration counts (delta 0.5, a forced
+    update after 5 iterations): the mixed solve still reaches the FP64 true
+    residual, and agrees with the unfused passes under the same schedule."""
+    print("case: fused residual update, frequent reliable updates", flush=True)
+    vox = holes(48, 24, 24, 3)
+    monkeypatch.setenv("SGLD_MIXED_DELTA", "0.5")
+    monkeypatch.setenv("SGLD_MIXED_STRETCH", "5")
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("SGLD_FUSE_RES", fuse)
+        monkeypatch.setenv("SGLD_FUSE_RES32", fuse)
+        g, m, F, fixed, E = build_pair(vox, 4)
+        g.set_schedule("tiled", 12, "tiled")
+        U, its, rr = g.pcg(F, tol=1e-8, maxIT=2000, precision=solver.PREC_MIXED)
+        st = g.stats()
+        true = np.linalg.norm(F - m.kbyu(U)) / np.linalg.norm(F)
+        print("  fuse %s: its %d, reliable updates %d, true rel-res %.2e" % (fuse, its, st["outer"], true), flush=True)
+        assert g.converged and true <= 1.01e-8 and st["outer"] >= 10
+        out[fuse] = (U, its)
+        g.close()
+    (U1, i1), (U0, i0) = out["1"], out["0"]
+    assert abs(i1 - i0) <= 2
+    assert np.linalg.norm(U1 - U0) <= 1e-7 * np.linalg.norm(U0)
+
+
 @pytest.mark.parametrize("case", ["cuboid", "holes"])
 def test_kbyu_and_pcg_run_to_run_identical(case):
     """K.u sums every node's element contributions in a fixed order, so FP64
