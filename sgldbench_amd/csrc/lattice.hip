@@ -4627,6 +4627,8 @@ extern "C" int sgld_multi_sensitivity_device(sgld_ctx *const *ctxs, int n, const
 extern "C" int sgld_ctx_sensitivity_device(sgld_ctx *C, const double *dU, const double *dx, double penal,
                                            double E0, double Emin, double scale, double *ddc, double *compliance) {
     SGLD_ARG(C && dU && dx && ddc, "sgld:arg: null argument");
+    SGLD_ARG(C->haveModulus, "sgld:state: call sgld_ctx_set_modulus first");
+    SGLD_ARG(penal >= 1.0, "sgld:arg: penal must be >= 1");
     double c = 0.0;
     SGLD_TRY(sgld_multi_sensitivity_device(&C, 1, &dU, &dx, penal, E0, Emin, scale, &ddc, &c));
     if (compliance) *compliance = c;
