@@ -735,7 +735,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is described in DESIGN.md, not run." % a.gpus, file=sys.stderr)
+              "the solver runs on one GPU; a sharded multi-GPU solve is not implemented." % a.gpus, file=sys.stderr)
         sys.exit(2)
     if a.workload == "cfg3":
         run_batched(a)
