@@ -734,8 +734,8 @@ def run_single(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "the solver runs on one GPU; a sharded multi-GPU solve is not implemented." % a.gpus, file=sys.stderr)
+        print("bench.py: the solver runs on one GPU (--gpus %d requested); "
+              "a sharded multi-GPU solve is not implemented." % a.gpus, file=sys.stderr)
         sys.exit(2)
     if a.workload == "cfg3":
         run_batched(a)
